@@ -1,0 +1,194 @@
+"""Throughput benchmark of the FDTD time step on one MI355X.
+
+Workload (default): plain 2nd-order Yee, float32, 1024^3 cells, 200 seeded
+non-overlapping dielectric spheres (radius U[8,40], eps_r U[1,12], background 1),
+12-cell CPML on all faces (m=4, R=1e-6), Ez point source at the centre driven by the
+Gaussian-modulated pulse, Ez probe; synthetic data.  One step = E update + CPML-E +
+source + H update + CPML-H (+ probe gather).
+
+    python bench.py [--steps K] [--warmup W] [--engine fused|kemp] [--n 1024]
+
+Prints one JSON line: Gcell-updates/s over K timed steps (inputs resident in HBM),
+the dominant kernel's roofline (algorithmic bytes / its average launch time, from
+HIP events around each launch on the kernel's stream) and a CPU baseline (the
+oracle, timed on a bounded sample on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + % HBM peak, plain vs batched vs adjoint-grad, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n", type=int, default=1024)
+    p.add_argument("--npml", type=int, default=12)
+    p.add_argument("--spheres", type=int, default=200)
+    p.add_argument("--engine", default="kemp", choices=("fused", "kemp"))
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-n", type=int, default=256)
+    return p.parse_args()
+
+
+def cpml_face_cells(n, npml):
+    """Slab cells of the six faces, each face counted separately (edges included)."""
+    return 6 * npml * n * n
+
+
+def build_sim(args):
+    import torch
+    from fdtd_accelerate_amd import geometry
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+
+    n, npml = args.n, args.npml
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="e", engine=args.engine)
+    lo, hi = (npml,) * 3, (n - npml,) * 3
+    rmax = min(40.0, (n - 2 * npml) / 8)
+    spheres = geometry.random_spheres(2024, args.spheres, (min(8.0, rmax / 2), rmax), (1.0, 12.0), lo, hi)
+    geometry.paint_ce(sim.fields.ces, spheres, (n, n, n))
+    c = n // 2
+    sim.add_source("ez", (c, c, c), (c, c, c), gaussian_pulse())
+    sim.add_probe("ez", (c + n // 4, c, c))
+    torch.cuda.synchronize()
+    return sim
+
+
+def time_kemp(sim, steps, warmup):
+    """In-place engine; events bracket the update_e kernel (the dominant launch) on the
+    fields' stream inside the timed loop."""
+    import torch
+    f = sim.fields
+    core = sim.core
+    others_e = [i for i in f.instance_list if i is not core]
+    for _ in range(warmup):
+        f.update_e()
+        f.update_h()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record(f.stream)
+    for s in range(steps):
+        evs[s][0].record(f.stream)
+        core.update_e()
+        evs[s][1].record(f.stream)
+        for inst in others_e:
+            inst.update_e()
+        f.update_h()
+    t1.record(f.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1)
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    return ms / steps, wall, kern_ms, "update_e", 48.0
+
+
+def time_fused(sim, steps, warmup):
+    import torch
+    from fdtd_accelerate_amd.fused import FusedStepper
+    st = FusedStepper(sim)
+    st.run(warmup, None)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(sim.fields.stream)
+    kern_ms = st.run(steps, None, time_kernel=True)
+    t1.record(sim.fields.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1)
+    return ms / steps, wall, kern_ms, st.kernel_name, st.bytes_per_cell
+
+
+def cpu_baseline(args):
+    """The oracle (float32, OpenMP) on a bounded sample: an n_s^3 Yee+CPML grid with
+    the same npml and a random-sphere CE map, timed over a few steps on this host."""
+    import numpy as np
+    from oracle import oracle as O
+    from fdtd_accelerate_amd import geometry
+    ns = args.cpu_sample_n
+    npml = args.npml
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    spheres = geometry.random_spheres(7, 12, (4.0, 12.0), (1.0, 12.0), (npml,) * 3, (ns - npml,) * 3)
+    ces = geometry.paint_ce_numpy((ns, ns, ns), ns, spheres)
+    s = O.OracleSim(ns, ns, ns, 0, npml, np.float32, ces=ces)
+    s.add_source("ez", (ns // 2,) * 3, (ns // 2,) * 3, O.pulse)
+    s.step()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        s.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or steps >= 200:
+            break
+    gcells = ns ** 3 * steps / el / 1e9
+    return {"value": round(gcells, 4), "unit": "Gcell/s", "cores": cores, "kind": "port",
+            "sample": f"oracle float32 Yee+CPML({npml}) {ns}^3, 12 spheres, {steps} steps, {el:.1f} s, OpenMP {cores} threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the slab-decomposition design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    sim = build_sim(args)
+    runner = time_fused if args.engine == "fused" else time_kemp
+    ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
+    n = args.n
+    cells = n ** 3
+    gcells = cells / (ms * 1e-3) / 1e9
+    step_bytes = 60.0 * cells + 32.0 * cpml_face_cells(n, args.npml)
+    achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gcells, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point source",
+                   "engine": args.engine, "cells": cells},
+        "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": kbytes, "avg_launch_ms": round(kern_ms, 4)},
+        "wall_s": round(wall, 3),
+        "device": torch.cuda.get_device_name(0),
+    }
+    if not args.no_cpu_baseline:
+        del sim
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,90 @@
+"""ctypes binding of ``libmifdtd.so`` (the C ABI declared in ``include/mifdtd.h``).
+
+There is no CPU fallback: importing a compute entry point without the built library
+raises ``RuntimeError`` (build it with ``python -m fdtd_accelerate_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from .build import LIB as LIB_PATH
+
+_I = ctypes.c_int
+_F = ctypes.c_float
+_P = ctypes.c_void_p
+_LL = ctypes.c_longlong
+
+
+class CpmlHalf(ctypes.Structure):
+    """``mifdtd_cpml_half``: npml, device b/a tables, psi1[6], psi2[6] (index 2*axis+side)."""
+    _fields_ = [("npml", _I), ("b", _P), ("a", _P), ("psi1", _P * 6), ("psi2", _P * 6)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mifdtd_last_error": (ctypes.c_char_p, []),
+    "mifdtd_version": (_I, []),
+    "mifdtd_set_pad": (_I, [_I]),
+    "update_e": (None, [_I] * 3 + [_P] * 9),
+    "update_h": (None, [_I] * 3 + [_P] * 9),
+    "mifdtd_update_e": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "mifdtd_update_h": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "update_x_e": (None, [_I] * 5 + [_P] * 10),
+    "update_x_h": (None, [_I] * 5 + [_P] * 8),
+    "update_y_e": (None, [_I] * 5 + [_P] * 10),
+    "update_y_h": (None, [_I] * 5 + [_P] * 8),
+    "update_z_e": (None, [_I] * 5 + [_P] * 10),
+    "update_z_h": (None, [_I] * 5 + [_P] * 8),
+    "mifdtd_cpml_e": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
+    "mifdtd_cpml_h": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
+    "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
+    "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
+}
+
+# Symbols declared in include/mifdtd.h that the library must export.
+EXPORTS = tuple(_SIGS)
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """Load (once) and return the ctypes library; raise if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built: the HIP path has no fallback. "
+                    "Run `python -m fdtd_accelerate_amd.build` (or __graft_entry__.build()).")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    msg = load().mifdtd_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (rc={rc}): {last_error()}")
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point and raise RuntimeError on failure."""
+    check(getattr(load(), name)(*args), name)
+
+
+def stream_handle(stream=None):
+    """Raw hipStream_t of a torch stream (default: torch's current stream)."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

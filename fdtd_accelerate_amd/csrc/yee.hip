@@ -1,0 +1,752 @@
+// In-place Yee kernels behind the reference's own call surface: the KEMP core
+// update_e/update_h (K7/cpu/src/core.c), kuFDTD's CPML face updates
+// (KU/cpu/core/cpml_non_kappa.c), the IncidentDirect source add and the probe gather.
+//
+// Layout: C-order (nx, ny, nzp) float32, z contiguous.  One thread owns one aligned
+// z-quad (4 floats, one 16-byte load/store per array), consecutive lanes own
+// consecutive quads so a 64-lane wave streams 1 KiB contiguous per array.  The
+// k+1 / k-1 neighbour that falls outside a thread's quad is a scalar load that hits
+// L1/L2 (the neighbouring lane loaded it); as in the reference's flat loop, the k+1
+// neighbour of a row's last element is the next row's first (k-1 of the first: the
+// previous row's last), which only feeds masked lanes.  Masked lanes store value*0
+// like the reference's SSE mask multiply, and arithmetic keeps the reference's
+// evaluation order under -ffp-contract=off, so results are bit-identical (including
+// the sign of masked zeros) to the reference's code.
+
+#include <cstdarg>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace mifdtd {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_status(hipError_t e, const char *where)
+{
+    if (e == hipSuccess) return 0;
+    return fail(static_cast<int>(e), "%s: %s", where, hipGetErrorString(e));
+}
+
+int check_grid(int nx, int ny, int nzp, int pad)
+{
+    if (nx < 2 || ny < 2 || nzp < 4) return fail(MIFDTD_EINVAL, "grid (%d,%d,%d) too small", nx, ny, nzp);
+    if (nzp % 4) return fail(MIFDTD_EALIGN, "nz_pitch %d is not a multiple of 4", nzp);
+    if (pad < 0 || pad > nzp - 2) return fail(MIFDTD_EINVAL, "pad %d invalid for nz_pitch %d", pad, nzp);
+    const unsigned long long quads = (unsigned long long)nx * ny * (nzp / 4);
+    if (quads >= (1ull << 31)) return fail(MIFDTD_ERANGE, "grid (%d,%d,%d) exceeds 2^31 quads", nx, ny, nzp);
+    return 0;
+}
+
+// ------------------------------------------------------------------ core kernels
+constexpr int kBlock = 256;
+
+template <bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_update_e(
+    int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
+    float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
+    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+    const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez)
+{
+    const unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nquads) return;
+    const unsigned row = f / Q;
+    const int k0 = (int)(f - row * Q) * 4;
+    const int i = (int)(row / (unsigned)ny);
+    const int j = (int)row - i * ny;
+    const size_t id = (size_t)row * nzp + k0;
+    const size_t sx = (size_t)ny * nzp;
+
+    const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
+
+    if (j < ny - 1) {
+        const Q4 e = ld4(ex + id);
+        const Q4 c = HAS_C ? ld4(cex + id) : splat4(0.5f);
+        const Q4 h1 = ld4(hz + id + nzp);
+        const float hyn = hy[id + 4];  // next row's first element on the last quad (j+1 exists)
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float h2 = l < 3 ? hy0.v[l + 1] : hyn;
+            const float v = e.v[l] + c.v[l] * ((h1.v[l] - hz0.v[l]) - (h2 - hy0.v[l]));
+            r.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
+        }
+        st4(ex + id, r);
+    }
+    if (i < nx - 1) {
+        const Q4 e = ld4(ey + id);
+        const Q4 c = HAS_C ? ld4(cey + id) : splat4(0.5f);
+        const Q4 h2 = ld4(hz + id + sx);
+        const float hxn = hx[id + 4];  // next row (or plane) start on the last quad (i+1 exists)
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float h1 = l < 3 ? hx0.v[l + 1] : hxn;
+            const float v = e.v[l] + c.v[l] * ((h1 - hx0.v[l]) - (h2.v[l] - hz0.v[l]));
+            r.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
+        }
+        st4(ey + id, r);
+        if (j < ny - 1) {
+            const Q4 e3 = ld4(ez + id);
+            const Q4 c3 = HAS_C ? ld4(cez + id) : splat4(0.5f);
+            const Q4 g1 = ld4(hy + id + sx);
+            const Q4 g2 = ld4(hx + id + nzp);
+            Q4 r3;
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                const float v = e3.v[l] + c3.v[l] * ((g1.v[l] - hy0.v[l]) - (g2.v[l] - hx0.v[l]));
+                r3.v[l] = (k0 + l < nz) ? v : v * 0.f;
+            }
+            st4(ez + id, r3);
+        }
+    }
+}
+
+template <bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_update_h(
+    int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
+    const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
+    float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+    const float *__restrict__ chx, const float *__restrict__ chy, const float *__restrict__ chz)
+{
+    (void)nx;
+    const unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nquads) return;
+    const unsigned row = f / Q;
+    const int k0 = (int)(f - row * Q) * 4;
+    const int i = (int)(row / (unsigned)ny);
+    const int j = (int)row - i * ny;
+    const size_t id = (size_t)row * nzp + k0;
+    const size_t sx = (size_t)ny * nzp;
+
+    const Q4 ex0 = ld4(ex + id), ey0 = ld4(ey + id), ez0 = ld4(ez + id);
+
+    if (j > 0) {
+        const Q4 h = ld4(hx + id);
+        const Q4 c = HAS_C ? ld4(chx + id) : splat4(0.5f);
+        const Q4 e1 = ld4(ez + id - nzp);
+        const float eyp = ey[id - 1];  // previous row's last element when k0 == 0 (j-1 exists)
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float e2 = l > 0 ? ey0.v[l - 1] : eyp;
+            const float v = h.v[l] - c.v[l] * ((ez0.v[l] - e1.v[l]) - (ey0.v[l] - e2));
+            const int k = k0 + l;
+            r.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
+        }
+        st4(hx + id, r);
+    }
+    if (i > 0) {
+        const Q4 h = ld4(hy + id);
+        const Q4 c = HAS_C ? ld4(chy + id) : splat4(0.5f);
+        const Q4 e2 = ld4(ez + id - sx);
+        const float exp_ = ex[id - 1];  // previous row's last element when k0 == 0 (i-1 exists)
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float e1 = l > 0 ? ex0.v[l - 1] : exp_;
+            const float v = h.v[l] - c.v[l] * ((ex0.v[l] - e1) - (ez0.v[l] - e2.v[l]));
+            const int k = k0 + l;
+            r.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
+        }
+        st4(hy + id, r);
+        if (j > 0) {
+            const Q4 h3 = ld4(hz + id);
+            const Q4 c3 = HAS_C ? ld4(chz + id) : splat4(0.5f);
+            const Q4 g1 = ld4(ey + id - sx);
+            const Q4 g2 = ld4(ex + id - nzp);
+            Q4 r3;
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                const float v = h3.v[l] - c3.v[l] * ((ey0.v[l] - g1.v[l]) - (ex0.v[l] - g2.v[l]));
+                r3.v[l] = (k0 + l < nz) ? v : v * 0.f;
+            }
+            st4(hz + id, r3);
+        }
+    }
+}
+
+static int launch_core(bool e_half, int nx, int ny, int nzp, int pad,
+                       float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+                       const float *c0, const float *c1, const float *c2, hipStream_t st)
+{
+    if (int rc = check_grid(nx, ny, nzp, pad)) return rc;
+    const float *ptrs[6] = {ex, ey, ez, hx, hy, hz};
+    for (const float *p : ptrs) {
+        if (!p) return fail(MIFDTD_ENULL, "field pointer is NULL");
+        if (!aligned16(p)) return fail(MIFDTD_EALIGN, "field pointer %p is not 16-byte aligned", (const void *)p);
+    }
+    const bool has_c = c0 || c1 || c2;
+    if (has_c) {
+        if (!(c0 && c1 && c2)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
+        if (!aligned16(c0) || !aligned16(c1) || !aligned16(c2))
+            return fail(MIFDTD_EALIGN, "coefficient pointer is not 16-byte aligned");
+    }
+    const unsigned Q = (unsigned)(nzp / 4);
+    const unsigned nquads = (unsigned)nx * ny * Q;
+    const unsigned grid = (nquads + kBlock - 1) / kBlock;
+    const int nz = nzp - pad;
+    if (e_half) {
+        if (has_c)
+            k_update_e<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        else
+            k_update_e<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        MIFDTD_CHECK_LAUNCH("update_e launch");
+    } else {
+        if (has_c)
+            k_update_h<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        else
+            k_update_h<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        MIFDTD_CHECK_LAUNCH("update_h launch");
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ CPML kernels
+// Validity masks of the core update (a CPML correction is applied only where the
+// core writes): component c = 0,1,2 for x,y,z.
+__device__ __forceinline__ bool valid_e(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (c == 0) return j < ny - 1 && k < nz - 1;
+    if (c == 1) return i < nx - 1 && k < nz - 1;
+    return i < nx - 1 && j < ny - 1 && k < nz;
+}
+__device__ __forceinline__ bool valid_h(int c, int i, int j, int k, int nz)
+{
+    if (c == 0) return j > 0 && k > 0 && k < nz;
+    if (c == 1) return i > 0 && k > 0 && k < nz;
+    return i > 0 && j > 0 && k < nz;
+}
+
+struct FaceSet {
+    float *psi1[2], *psi2[2];  // [side]
+};
+
+// E half, x or y axis (AXIS 0/1): thread per slab quad, both sides in one grid.
+// Work items: side-major, then p, then the in-plane index.
+template <int AXIS, bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_cpml_e_xy(
+    int nx, int ny, int nzp, int nz, int npml, unsigned per_side, unsigned Q,
+    float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
+    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+    const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
+    FaceSet fs, const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= 2 * per_side) return;
+    const int side = f >= per_side;
+    if (side) f -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    // per_side = npml * (other extent) * Q
+    const unsigned rowq = f / Q;
+    const int k0 = (int)(f - rowq * Q) * 4;
+    int i, j, p;
+    const int n_e = AXIS == 0 ? nx : ny;
+    if (AXIS == 0) {
+        p = (int)(rowq / (unsigned)ny);
+        j = (int)rowq - p * ny;
+        i = side ? n_e - 1 - npml + p : p;
+    } else {
+        i = (int)(rowq / (unsigned)npml);
+        p = (int)rowq - i * npml;
+        j = side ? n_e - 1 - npml + p : p;
+    }
+    const int t = side ? npml + 1 + p : p + 1;
+    const float b = tb[t], a = ta[t];
+    const size_t id = ((size_t)i * ny + j) * nzp + k0;
+    const size_t pid = (size_t)rowq * nzp + k0;
+    const size_t s = AXIS == 0 ? (size_t)ny * nzp : (size_t)nzp;
+    // x: psi1 -> ey from hz, psi2 -> ez from hy ; y: psi1 -> ez from hx, psi2 -> ex from hz
+    float *e1 = AXIS == 0 ? ey : ez;
+    float *e2 = AXIS == 0 ? ez : ex;
+    const float *g1 = AXIS == 0 ? hz : hx;
+    const float *g2 = AXIS == 0 ? hy : hz;
+    const float *c1 = AXIS == 0 ? cey : cez;
+    const float *c2 = AXIS == 0 ? cez : cex;
+    const int comp1 = AXIS == 0 ? 1 : 2, comp2 = AXIS == 0 ? 2 : 0;
+
+    {
+        const Q4 hp = ld4(g1 + id + s), h0 = ld4(g1 + id), ps = ld4(psi1 + pid);
+        Q4 e = ld4(e1 + id);
+        const Q4 c = HAS_C ? ld4(c1 + id) : splat4(0.5f);
+        Q4 np = ps;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            if (valid_e(comp1, i, j, k0 + l, nx, ny, nz)) {
+                const float tt = b * ps.v[l] + a * (hp.v[l] - h0.v[l]);
+                np.v[l] = tt;
+                e.v[l] = e.v[l] - c.v[l] * tt;
+            }
+        }
+        st4(psi1 + pid, np);
+        st4(e1 + id, e);
+    }
+    {
+        const Q4 hp = ld4(g2 + id + s), h0 = ld4(g2 + id), ps = ld4(psi2 + pid);
+        Q4 e = ld4(e2 + id);
+        const Q4 c = HAS_C ? ld4(c2 + id) : splat4(0.5f);
+        Q4 np = ps;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            if (valid_e(comp2, i, j, k0 + l, nx, ny, nz)) {
+                const float tt = b * ps.v[l] + a * (hp.v[l] - h0.v[l]);
+                np.v[l] = tt;
+                e.v[l] = e.v[l] + c.v[l] * tt;
+            }
+        }
+        st4(psi2 + pid, np);
+        st4(e2 + id, e);
+    }
+}
+
+// H half, x or y axis.
+template <int AXIS, bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_cpml_h_xy(
+    int nx, int ny, int nzp, int nz, int npml, unsigned per_side, unsigned Q,
+    const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
+    float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+    const float *__restrict__ chx, const float *__restrict__ chy, const float *__restrict__ chz,
+    FaceSet fs, const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= 2 * per_side) return;
+    const int side = f >= per_side;
+    if (side) f -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    const unsigned rowq = f / Q;
+    const int k0 = (int)(f - rowq * Q) * 4;
+    int i, j, p;
+    const int n_e = AXIS == 0 ? nx : ny;
+    if (AXIS == 0) {
+        p = (int)(rowq / (unsigned)ny);
+        j = (int)rowq - p * ny;
+        i = side ? n_e - npml + p : p;
+    } else {
+        i = (int)(rowq / (unsigned)npml);
+        p = (int)rowq - i * npml;
+        j = side ? n_e - npml + p : p;
+    }
+    const int t = side ? npml + 2 + p : p + 1;
+    const float b = tb[t], a = ta[t];
+    const size_t id = ((size_t)i * ny + j) * nzp + k0;
+    const size_t pid = (size_t)rowq * nzp + k0;
+    const size_t s = AXIS == 0 ? (size_t)ny * nzp : (size_t)nzp;
+    // x: psi1 -> hy from ez, psi2 -> hz from ey ; y: psi1 -> hz from ex, psi2 -> hx from ez
+    float *h1 = AXIS == 0 ? hy : hz;
+    float *h2 = AXIS == 0 ? hz : hx;
+    const float *g1 = AXIS == 0 ? ez : ex;
+    const float *g2 = AXIS == 0 ? ey : ez;
+    const float *c1 = AXIS == 0 ? chy : chz;
+    const float *c2 = AXIS == 0 ? chz : chx;
+    const int comp1 = AXIS == 0 ? 1 : 2, comp2 = AXIS == 0 ? 2 : 0;
+    // Front planes p = 0 are the wall: valid_h() is false there, so id - s is never read.
+    const bool inner = (AXIS == 0 ? i : j) > 0;
+
+    if (inner) {
+        const Q4 g0 = ld4(g1 + id), gm = ld4(g1 + id - s), ps = ld4(psi1 + pid);
+        Q4 h = ld4(h1 + id);
+        const Q4 c = HAS_C ? ld4(c1 + id) : splat4(0.5f);
+        Q4 np = ps;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            if (valid_h(comp1, i, j, k0 + l, nz)) {
+                const float tt = b * ps.v[l] + a * (g0.v[l] - gm.v[l]);
+                np.v[l] = tt;
+                h.v[l] = h.v[l] + c.v[l] * tt;
+            }
+        }
+        st4(psi1 + pid, np);
+        st4(h1 + id, h);
+
+        const Q4 q0 = ld4(g2 + id), qm = ld4(g2 + id - s), ps2 = ld4(psi2 + pid);
+        Q4 h_ = ld4(h2 + id);
+        const Q4 c_ = HAS_C ? ld4(c2 + id) : splat4(0.5f);
+        Q4 np2 = ps2;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            if (valid_h(comp2, i, j, k0 + l, nz)) {
+                const float tt = b * ps2.v[l] + a * (q0.v[l] - qm.v[l]);
+                np2.v[l] = tt;
+                h_.v[l] = h_.v[l] - c_.v[l] * tt;
+            }
+        }
+        st4(psi2 + pid, np2);
+        st4(h2 + id, h_);
+    }
+}
+
+// z axis: thread per slab element (psi layout (nx, ny, npml)).
+template <bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_cpml_e_z(
+    int nx, int ny, int nzp, int nz, int npml, unsigned per_side,
+    float *__restrict__ ex, float *__restrict__ ey,
+    const float *__restrict__ hx, const float *__restrict__ hy,
+    const float *__restrict__ cex, const float *__restrict__ cey,
+    FaceSet fs, const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= 2 * per_side) return;
+    const int side = f >= per_side;
+    if (side) f -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    const unsigned row = f / (unsigned)npml;
+    const int p = (int)(f - row * npml);
+    const int i = (int)(row / (unsigned)ny);
+    const int j = (int)row - i * ny;
+    const int k = side ? nz - 1 - npml + p : p;
+    const int t = side ? npml + 1 + p : p + 1;
+    const float b = tb[t], a = ta[t];
+    const size_t id = (size_t)row * nzp + k;
+    // psi1 -> ex from hy, psi2 -> ey from hx
+    if (valid_e(0, i, j, k, nx, ny, nz)) {
+        const float tt = b * psi1[f] + a * (hy[id + 1] - hy[id]);
+        psi1[f] = tt;
+        ex[id] = ex[id] - (HAS_C ? cex[id] : 0.5f) * tt;
+    }
+    if (valid_e(1, i, j, k, nx, ny, nz)) {
+        const float tt = b * psi2[f] + a * (hx[id + 1] - hx[id]);
+        psi2[f] = tt;
+        ey[id] = ey[id] + (HAS_C ? cey[id] : 0.5f) * tt;
+    }
+}
+
+template <bool HAS_C>
+__global__ __launch_bounds__(kBlock) void k_cpml_h_z(
+    int nx, int ny, int nzp, int nz, int npml, unsigned per_side,
+    const float *__restrict__ ex, const float *__restrict__ ey,
+    float *__restrict__ hx, float *__restrict__ hy,
+    const float *__restrict__ chx, const float *__restrict__ chy,
+    FaceSet fs, const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    (void)nx;
+    unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= 2 * per_side) return;
+    const int side = f >= per_side;
+    if (side) f -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    const unsigned row = f / (unsigned)npml;
+    const int p = (int)(f - row * npml);
+    const int i = (int)(row / (unsigned)ny);
+    const int j = (int)row - i * ny;
+    const int k = side ? nz - npml + p : p;
+    const int t = side ? npml + 2 + p : p + 1;
+    const float b = tb[t], a = ta[t];
+    const size_t id = (size_t)row * nzp + k;
+    // psi1 -> hx from ey, psi2 -> hy from ex  (k > 0 on every valid cell)
+    if (valid_h(0, i, j, k, nz)) {
+        const float tt = b * psi1[f] + a * (ey[id] - ey[id - 1]);
+        psi1[f] = tt;
+        hx[id] = hx[id] + (HAS_C ? chx[id] : 0.5f) * tt;
+    }
+    if (valid_h(1, i, j, k, nz)) {
+        const float tt = b * psi2[f] + a * (ex[id] - ex[id - 1]);
+        psi2[f] = tt;
+        hy[id] = hy[id] - (HAS_C ? chy[id] : 0.5f) * tt;
+    }
+}
+
+static int check_cpml(int nx, int ny, int nzp, int pad, int npml, const float *b, const float *a)
+{
+    if (int rc = check_grid(nx, ny, nzp, pad)) return rc;
+    if (npml < 1 || npml > MIFDTD_MAX_NPML) return fail(MIFDTD_EINVAL, "npml %d outside [1,%d]", npml, MIFDTD_MAX_NPML);
+    const int nz = nzp - pad;
+    if (nx < 2 * npml + 2 || ny < 2 * npml + 2 || nz < 2 * npml + 2)
+        return fail(MIFDTD_EINVAL, "grid (%d,%d,%d) too small for npml %d (need >= 2*npml+2 per axis)", nx, ny, nz, npml);
+    if (!b || !a) return fail(MIFDTD_ENULL, "CPML coefficient table is NULL");
+    return 0;
+}
+
+// One axis of the E half, both sides.
+static int cpml_e_axis(int axis, int nx, int ny, int nzp, int pad, int npml, FaceSet fs,
+                       float *ex, float *ey, float *ez, const float *hx, const float *hy, const float *hz,
+                       bool hc, const float *cex, const float *cey, const float *cez,
+                       const float *b, const float *a, hipStream_t st)
+{
+    if (!fs.psi1[0] && !fs.psi1[1]) return 0;
+    for (int s = 0; s < 2; s++) {
+        if ((fs.psi1[s] == nullptr) != (fs.psi2[s] == nullptr))
+            return fail(MIFDTD_ENULL, "psi1/psi2 of one face must be both set or both NULL");
+        if (axis < 2 && fs.psi1[s] && (!aligned16(fs.psi1[s]) || !aligned16(fs.psi2[s])))
+            return fail(MIFDTD_EALIGN, "psi pointer is not 16-byte aligned");
+    }
+    const int nz = nzp - pad;
+    const unsigned Q = nzp / 4;
+    if (axis == 0) {
+        const unsigned per = (unsigned)npml * ny * Q;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_e_xy<0, true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, cex, cey, cez, fs, b, a);
+        else k_cpml_e_xy<0, false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, cex, cey, cez, fs, b, a);
+    } else if (axis == 1) {
+        const unsigned per = (unsigned)nx * npml * Q;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_e_xy<1, true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, cex, cey, cez, fs, b, a);
+        else k_cpml_e_xy<1, false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, cex, cey, cez, fs, b, a);
+    } else {
+        const unsigned per = (unsigned)nx * ny * npml;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_e_z<true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, ex, ey, hx, hy, cex, cey, fs, b, a);
+        else k_cpml_e_z<false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, ex, ey, hx, hy, cex, cey, fs, b, a);
+    }
+    MIFDTD_CHECK_LAUNCH("cpml E launch");
+    return 0;
+}
+
+static int cpml_h_axis(int axis, int nx, int ny, int nzp, int pad, int npml, FaceSet fs,
+                       const float *ex, const float *ey, const float *ez, float *hx, float *hy, float *hz,
+                       bool hc, const float *chx, const float *chy, const float *chz,
+                       const float *b, const float *a, hipStream_t st)
+{
+    if (!fs.psi1[0] && !fs.psi1[1]) return 0;
+    for (int s = 0; s < 2; s++) {
+        if ((fs.psi1[s] == nullptr) != (fs.psi2[s] == nullptr))
+            return fail(MIFDTD_ENULL, "psi1/psi2 of one face must be both set or both NULL");
+        if (axis < 2 && fs.psi1[s] && (!aligned16(fs.psi1[s]) || !aligned16(fs.psi2[s])))
+            return fail(MIFDTD_EALIGN, "psi pointer is not 16-byte aligned");
+    }
+    const int nz = nzp - pad;
+    const unsigned Q = nzp / 4;
+    if (axis == 0) {
+        const unsigned per = (unsigned)npml * ny * Q;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_h_xy<0, true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, chx, chy, chz, fs, b, a);
+        else k_cpml_h_xy<0, false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, chx, chy, chz, fs, b, a);
+    } else if (axis == 1) {
+        const unsigned per = (unsigned)nx * npml * Q;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_h_xy<1, true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, chx, chy, chz, fs, b, a);
+        else k_cpml_h_xy<1, false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, Q, ex, ey, ez, hx, hy, hz, chx, chy, chz, fs, b, a);
+    } else {
+        const unsigned per = (unsigned)nx * ny * npml;
+        const unsigned g = (2 * per + kBlock - 1) / kBlock;
+        if (hc) k_cpml_h_z<true><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, ex, ey, hx, hy, chx, chy, fs, b, a);
+        else k_cpml_h_z<false><<<g, kBlock, 0, st>>>(nx, ny, nzp, nz, npml, per, ex, ey, hx, hy, chx, chy, fs, b, a);
+    }
+    MIFDTD_CHECK_LAUNCH("cpml H launch");
+    return 0;
+}
+
+// ------------------------------------------------------------------ source / gather
+__global__ __launch_bounds__(kBlock) void k_source(
+    float *__restrict__ f, int ny, int nzp, int i0, int j0, int k0, int li, int lj, int lk,
+    float tval, const float *__restrict__ sv, float svs, int overwrite)
+{
+    const unsigned n = blockIdx.x * kBlock + threadIdx.x;
+    const unsigned tot = (unsigned)li * lj * lk;
+    if (n >= tot) return;
+    const int dk = (int)(n % (unsigned)lk);
+    const unsigned r = n / (unsigned)lk;
+    const int dj = (int)(r % (unsigned)lj);
+    const int di = (int)(r / (unsigned)lj);
+    const size_t id = ((size_t)(i0 + di) * ny + (j0 + dj)) * nzp + (k0 + dk);
+    const float v = tval * (sv ? sv[n] : svs);
+    f[id] = overwrite ? v : f[id] + v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ f, const long long *__restrict__ off,
+                                                   int count, float *__restrict__ out)
+{
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n < count) out[n] = f[off[n]];
+}
+
+}  // namespace mifdtd
+
+// =========================================================================== C ABI
+using namespace mifdtd;
+
+static int g_pad = 0;
+static std::mutex g_pad_mu;
+
+static int current_pad()
+{
+    std::lock_guard<std::mutex> lk(g_pad_mu);
+    return g_pad;
+}
+
+extern "C" {
+
+const char *mifdtd_last_error(void) { return g_err.c_str(); }
+
+int mifdtd_version(void) { return 1; }
+
+int mifdtd_set_pad(int pad)
+{
+    if (pad < 0) return fail(MIFDTD_EINVAL, "pad %d < 0", pad);
+    std::lock_guard<std::mutex> lk(g_pad_mu);
+    g_pad = pad;
+    return 0;
+}
+
+int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey, float *ez,
+                    const float *hx, const float *hy, const float *hz,
+                    const float *cex, const float *cey, const float *cez, mifdtd_stream_t stream)
+{
+    return launch_core(true, nx, ny, nz_pitch, pad, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
+                       const_cast<float *>(hz), cex, cey, cez, to_stream(stream));
+}
+
+int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad, const float *ex, const float *ey, const float *ez,
+                    float *hx, float *hy, float *hz,
+                    const float *chx, const float *chy, const float *chz, mifdtd_stream_t stream)
+{
+    return launch_core(false, nx, ny, nz_pitch, pad, const_cast<float *>(ex), const_cast<float *>(ey),
+                       const_cast<float *>(ez), hx, hy, hz, chx, chy, chz, to_stream(stream));
+}
+
+void update_e(int nx, int ny, int nz, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+              float *cex, float *cey, float *cez)
+{
+    mifdtd_update_e(nx, ny, nz, current_pad(), ex, ey, ez, hx, hy, hz, cex, cey, cez, nullptr);
+}
+
+void update_h(int nx, int ny, int nz, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+              float *chx, float *chy, float *chz)
+{
+    mifdtd_update_h(nx, ny, nz, current_pad(), ex, ey, ez, hx, hy, hz, chx, chy, chz, nullptr);
+}
+
+int mifdtd_cpml_e(int nx, int ny, int nzp, int pad, const mifdtd_cpml_half *cp,
+                  float *ex, float *ey, float *ez, const float *hx, const float *hy, const float *hz,
+                  const float *cex, const float *cey, const float *cez, mifdtd_stream_t stream)
+{
+    if (!cp) return fail(MIFDTD_ENULL, "cpml descriptor is NULL");
+    if (int rc = check_cpml(nx, ny, nzp, pad, cp->npml, cp->b, cp->a)) return rc;
+    if ((cex || cey || cez) && !(cex && cey && cez)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
+    for (int ax = 0; ax < 3; ax++) {
+        FaceSet fs{{cp->psi1[2 * ax], cp->psi1[2 * ax + 1]}, {cp->psi2[2 * ax], cp->psi2[2 * ax + 1]}};
+        if (int rc = cpml_e_axis(ax, nx, ny, nzp, pad, cp->npml, fs, ex, ey, ez, hx, hy, hz, cex != nullptr, cex, cey, cez,
+                                 cp->b, cp->a, to_stream(stream)))
+            return rc;
+    }
+    return 0;
+}
+
+int mifdtd_cpml_h(int nx, int ny, int nzp, int pad, const mifdtd_cpml_half *cp,
+                  const float *ex, const float *ey, const float *ez, float *hx, float *hy, float *hz,
+                  const float *chx, const float *chy, const float *chz, mifdtd_stream_t stream)
+{
+    if (!cp) return fail(MIFDTD_ENULL, "cpml descriptor is NULL");
+    if (int rc = check_cpml(nx, ny, nzp, pad, cp->npml, cp->b, cp->a)) return rc;
+    if ((chx || chy || chz) && !(chx && chy && chz)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
+    for (int ax = 0; ax < 3; ax++) {
+        FaceSet fs{{cp->psi1[2 * ax], cp->psi1[2 * ax + 1]}, {cp->psi2[2 * ax], cp->psi2[2 * ax + 1]}};
+        if (int rc = cpml_h_axis(ax, nx, ny, nzp, pad, cp->npml, fs, ex, ey, ez, hx, hy, hz, chx != nullptr, chx, chy, chz,
+                                 cp->b, cp->a, to_stream(stream)))
+            return rc;
+    }
+    return 0;
+}
+
+// kuFDTD per-face forms (KU/cpu/core/cpml_non_kappa.c argument order, Ncore dropped).
+static int face_e(int axis, int nx, int ny, int nz, int npml, int backward, float *f1, float *f2,
+                  float *g1, float *g2, float *c1, float *c2, float *psi1, float *psi2, float *b, float *a)
+{
+    const int pad = current_pad();
+    if (int rc = check_cpml(nx, ny, nz, pad, npml, b, a)) return rc;
+    if (backward != 0 && backward != 1) return fail(MIFDTD_EINVAL, "backward must be 0 or 1");
+    if (!psi1 || !psi2) return fail(MIFDTD_ENULL, "psi is NULL");
+    FaceSet fs{{nullptr, nullptr}, {nullptr, nullptr}};
+    fs.psi1[backward] = psi1;
+    fs.psi2[backward] = psi2;
+    float *ex = nullptr, *ey = nullptr, *ez = nullptr, *cex = nullptr, *cey = nullptr, *cez = nullptr;
+    const float *hx = nullptr, *hy = nullptr, *hz = nullptr;
+    if (axis == 0) { ey = f1; ez = f2; hy = g1; hz = g2; cey = c1; cez = c2; }
+    if (axis == 1) { ex = f1; ez = f2; hx = g1; hz = g2; cex = c1; cez = c2; }
+    if (axis == 2) { ex = f1; ey = f2; hx = g1; hy = g2; cex = c1; cey = c2; }
+    if ((c1 == nullptr) != (c2 == nullptr)) return fail(MIFDTD_ENULL, "CE arrays must be both set or both NULL");
+    return cpml_e_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, c1 != nullptr, cex, cey, cez, b, a, nullptr);
+}
+
+static int face_h(int axis, int nx, int ny, int nz, int npml, int backward, float *f1, float *f2,
+                  float *g1, float *g2, float *psi1, float *psi2, float *b, float *a)
+{
+    const int pad = current_pad();
+    if (int rc = check_cpml(nx, ny, nz, pad, npml, b, a)) return rc;
+    if (backward != 0 && backward != 1) return fail(MIFDTD_EINVAL, "backward must be 0 or 1");
+    if (!psi1 || !psi2) return fail(MIFDTD_ENULL, "psi is NULL");
+    FaceSet fs{{nullptr, nullptr}, {nullptr, nullptr}};
+    fs.psi1[backward] = psi1;
+    fs.psi2[backward] = psi2;
+    const float *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    float *hx = nullptr, *hy = nullptr, *hz = nullptr;
+    if (axis == 0) { ey = f1; ez = f2; hy = g1; hz = g2; }
+    if (axis == 1) { ex = f1; ez = f2; hx = g1; hz = g2; }
+    if (axis == 2) { ex = f1; ey = f2; hx = g1; hy = g2; }
+    return cpml_h_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, false, nullptr, nullptr, nullptr, b, a, nullptr);
+}
+
+void update_x_e(int nx, int ny, int nz, int npml, int backward, float *ey, float *ez, float *hy, float *hz,
+                float *cey, float *cez, float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_e(0, nx, ny, nz, npml, backward, ey, ez, hy, hz, cey, cez, psi1, psi2, rcmb, rcma);
+}
+void update_x_h(int nx, int ny, int nz, int npml, int backward, float *ey, float *ez, float *hy, float *hz,
+                float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_h(0, nx, ny, nz, npml, backward, ey, ez, hy, hz, psi1, psi2, rcmb, rcma);
+}
+void update_y_e(int nx, int ny, int nz, int npml, int backward, float *ex, float *ez, float *hx, float *hz,
+                float *cex, float *cez, float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_e(1, nx, ny, nz, npml, backward, ex, ez, hx, hz, cex, cez, psi1, psi2, rcmb, rcma);
+}
+void update_y_h(int nx, int ny, int nz, int npml, int backward, float *ex, float *ez, float *hx, float *hz,
+                float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_h(1, nx, ny, nz, npml, backward, ex, ez, hx, hz, psi1, psi2, rcmb, rcma);
+}
+void update_z_e(int nx, int ny, int nz, int npml, int backward, float *ex, float *ey, float *hx, float *hy,
+                float *cex, float *cey, float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_e(2, nx, ny, nz, npml, backward, ex, ey, hx, hy, cex, cey, psi1, psi2, rcmb, rcma);
+}
+void update_z_h(int nx, int ny, int nz, int npml, int backward, float *ex, float *ey, float *hx, float *hy,
+                float *psi1, float *psi2, float *rcmb, float *rcma)
+{
+    face_h(2, nx, ny, nz, npml, backward, ex, ey, hx, hy, psi1, psi2, rcmb, rcma);
+}
+
+int mifdtd_source(float *f, int nx, int ny, int nzp, int i0, int i1, int j0, int j1, int k0, int k1,
+                  float tval, const float *sv, float svs, int overwrite, mifdtd_stream_t stream)
+{
+    if (!f) return fail(MIFDTD_ENULL, "field pointer is NULL");
+    if (i0 < 0 || j0 < 0 || k0 < 0 || i1 >= nx || j1 >= ny || k1 >= nzp || i0 > i1 || j0 > j1 || k0 > k1)
+        return fail(MIFDTD_EINVAL, "source box [%d:%d,%d:%d,%d:%d] outside (%d,%d,%d)", i0, i1, j0, j1, k0, k1, nx, ny, nzp);
+    const int li = i1 - i0 + 1, lj = j1 - j0 + 1, lk = k1 - k0 + 1;
+    const unsigned long long tot = (unsigned long long)li * lj * lk;
+    if (tot >= (1ull << 31)) return fail(MIFDTD_ERANGE, "source box too large");
+    const unsigned g = (unsigned)((tot + kBlock - 1) / kBlock);
+    k_source<<<g, kBlock, 0, to_stream(stream)>>>(f, ny, nzp, i0, j0, k0, li, lj, lk, tval, sv, svs, overwrite);
+    MIFDTD_CHECK_LAUNCH("source launch");
+    return 0;
+}
+
+int mifdtd_gather(const float *f, const long long *offsets, int count, float *out, mifdtd_stream_t stream)
+{
+    if (count <= 0) return 0;
+    if (!f || !offsets || !out) return fail(MIFDTD_ENULL, "gather pointer is NULL");
+    const unsigned g = (unsigned)((count + kBlock - 1) / kBlock);
+    k_gather<<<g, kBlock, 0, to_stream(stream)>>>(f, offsets, count, out);
+    MIFDTD_CHECK_LAUNCH("gather launch");
+    return 0;
+}
+
+}  // extern "C"

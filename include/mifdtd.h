@@ -1,0 +1,150 @@
+/*
+ * mifdtd — MI355X (gfx950) FDTD time-step engine: C ABI.
+ *
+ * Every pointer argument is a DEVICE pointer (e.g. torch.Tensor.data_ptr() of a
+ * CUDA/HIP tensor) unless stated otherwise.  Arrays use the KEMP layout of the
+ * reference: C-order (nx, ny, nz_pitch) float32, z contiguous, nz_pitch = nz + pad
+ * with pad lanes held at zero (K7/cpu/fields.py:27-32, K7/gpu/fields.py:45-50).
+ * nz_pitch must be a multiple of 4 and every array 16-byte aligned.
+ *
+ * Functions returning int return 0 on success, a positive hipError_t from the
+ * runtime, or a negative MIFDTD_E* code for rejected arguments; the message of the
+ * last failure on the calling thread is available from mifdtd_last_error().
+ * The void functions (the reference's own names and argument orders) report
+ * failures only through mifdtd_last_error(), as the reference's void kernels have no
+ * error channel (K7/cpu/src/core.c:11,59).
+ *
+ * All work is enqueued on the given HIP stream (NULL = the null stream) and is
+ * asynchronous with respect to the host.
+ */
+#ifndef MIFDTD_H
+#define MIFDTD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *mifdtd_stream_t; /* hipStream_t */
+
+enum {
+    MIFDTD_OK = 0,
+    MIFDTD_EINVAL = -1,   /* bad size / pad / npml / region */
+    MIFDTD_EALIGN = -2,   /* misaligned pointer or nz_pitch % 4 != 0 */
+    MIFDTD_ENULL = -3,    /* required pointer is NULL */
+    MIFDTD_ERANGE = -4    /* size beyond the supported index range */
+};
+
+const char *mifdtd_last_error(void);
+int mifdtd_version(void);
+
+/* ------------------------------------------------------------------ core ----
+ * Replaces the KEMP ctypes core generated from K7/cpu/src/core.c:11 (update_e) and
+ * :59 (update_h), bound at K7/cpu/core.py:86-105 (argtypes: 3 ints, 6 or 9 arrays),
+ * and the OpenCL kernels K7/gpu/src/core.cl:3,32 (same argument order).
+ * nz is nz_pitch, as in the reference.  The pad (masked lanes) that the reference
+ * bakes into each generated program (K7/cpu/core.py:33-49) is set once with
+ * mifdtd_set_pad().  cex..cez / chx..chz may be NULL, meaning the constant 0.5
+ * (the coefficient-free programs, K7/cpu/core.py:63-65).
+ * Semantics (masks as K7/cpu/src/core.c:31,41,51,79-80,90-91,101):
+ *   ex += cex*((hz[j+1]-hz)-(hy[k+1]-hy))  j<ny-1,          lanes k>=nz-1 -> 0
+ *   ey += cey*((hx[k+1]-hx)-(hz[i+1]-hz))  i<nx-1,          lanes k>=nz-1 -> 0
+ *   ez += cez*((hy[i+1]-hy)-(hx[j+1]-hx))  i<nx-1 && j<ny-1, pad lanes -> 0
+ *   hx -= chx*((ez-ez[j-1])-(ey-ey[k-1]))  j>0,             lane 0 and pad -> 0
+ *   hy -= chy*((ex-ex[k-1])-(ez-ez[i-1]))  i>0,             lane 0 and pad -> 0
+ *   hz -= chz*((ey-ey[i-1])-(ex-ex[j-1]))  i>0 && j>0,      pad lanes -> 0
+ */
+int mifdtd_set_pad(int pad);
+void update_e(int nx, int ny, int nz, float *ex, float *ey, float *ez,
+              float *hx, float *hy, float *hz, float *cex, float *cey, float *cez);
+void update_h(int nx, int ny, int nz, float *ex, float *ey, float *ez,
+              float *hx, float *hy, float *hz, float *chx, float *chy, float *chz);
+
+/* Status-returning, stream-taking forms of the two calls above (explicit pad). */
+int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad,
+                    float *ex, float *ey, float *ez,
+                    const float *hx, const float *hy, const float *hz,
+                    const float *cex, const float *cey, const float *cez,
+                    mifdtd_stream_t stream);
+int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad,
+                    const float *ex, const float *ey, const float *ez,
+                    float *hx, float *hy, float *hz,
+                    const float *chx, const float *chy, const float *chz,
+                    mifdtd_stream_t stream);
+
+/* ------------------------------------------------------------------ CPML ----
+ * Replaces kuFDTD's non-kappa CPML C extension, KU/cpu/core/cpml_non_kappa.c:14
+ * (update_x_e), :91 (update_x_h), :156 (update_y_e), :227 (update_y_h),
+ * :294 (update_z_e), :365 (update_z_h), called from KU/cpu/cpml.py:131-203, and the
+ * CUDA kernels KU/gpu/core/cpml_non_kapa.cu:17-130.  Argument order is kuFDTD's with
+ * the CPU thread count dropped and nz = nz_pitch; the pad is the one set by
+ * mifdtd_set_pad().  rcmb/rcma are DEVICE tables of 2*(npml+1) floats in kuFDTD's
+ * layout (KU/cpu/cpml.py:107-126): E front entry p+1, E back npml+1+p; H front p+1,
+ * H back npml+2+p.
+ *   x: psi1->ey from hz, psi2->ez from hy   psi shape (npml, ny, nz_pitch)
+ *   y: psi1->ez from hx, psi2->ex from hz   psi shape (nx, npml, nz_pitch)
+ *   z: psi1->ex from hy, psi2->ey from hx   psi shape (nx, ny, npml)
+ *   E:  psi = b*psi + a*(H(+1)-H);  E_1 -= CE*psi1;  E_2 += CE*psi2
+ *   H:  psi = b*psi + a*(E-E(-1));  H_1 += 0.5*psi1; H_2 -= 0.5*psi2
+ * Slab planes: E front p, E back n_e-1-npml+p, H front p, H back n_e-npml+p
+ * (p in [0,npml), n_e = nx, ny or nz); updates only cells the core writes.
+ * Requires n_e >= 2*npml+2 on the axis and 1 <= npml <= MIFDTD_MAX_NPML. */
+#define MIFDTD_MAX_NPML 64
+void update_x_e(int nx, int ny, int nz, int npml, int backward,
+                float *ey, float *ez, float *hy, float *hz, float *cey, float *cez,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+void update_x_h(int nx, int ny, int nz, int npml, int backward,
+                float *ey, float *ez, float *hy, float *hz,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+void update_y_e(int nx, int ny, int nz, int npml, int backward,
+                float *ex, float *ez, float *hx, float *hz, float *cex, float *cez,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+void update_y_h(int nx, int ny, int nz, int npml, int backward,
+                float *ex, float *ez, float *hx, float *hz,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+void update_z_e(int nx, int ny, int nz, int npml, int backward,
+                float *ex, float *ey, float *hx, float *hy, float *cex, float *cey,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+void update_z_h(int nx, int ny, int nz, int npml, int backward,
+                float *ex, float *ey, float *hx, float *hy,
+                float *psi1, float *psi2, float *rcmb, float *rcma);
+
+/* All CPML faces of one half step, in the reference order (x f, x b, y f, y b, z f,
+ * z b): one launch per axis.  psi1[2*axis+side], psi2[2*axis+side] may be NULL for
+ * a face that is off.  ch arrays (H half) may be NULL = 0.5. */
+typedef struct {
+    int npml;
+    const float *b, *a;          /* device tables, 2*(npml+1) floats */
+    float *psi1[6], *psi2[6];    /* index 2*axis + side */
+} mifdtd_cpml_half;
+
+int mifdtd_cpml_e(int nx, int ny, int nz_pitch, int pad, const mifdtd_cpml_half *cp,
+                  float *ex, float *ey, float *ez,
+                  const float *hx, const float *hy, const float *hz,
+                  const float *cex, const float *cey, const float *cez,
+                  mifdtd_stream_t stream);
+int mifdtd_cpml_h(int nx, int ny, int nz_pitch, int pad, const mifdtd_cpml_half *cp,
+                  const float *ex, const float *ey, const float *ez,
+                  float *hx, float *hy, float *hz,
+                  const float *chx, const float *chy, const float *chz,
+                  mifdtd_stream_t stream);
+
+/* ---------------------------------------------------------------- sources ----
+ * Replaces the IncidentDirect -> SetFields update of K7/cpu/incident_direct.py:75-77
+ * and K7/cpu/get_set_fields.py:110-123 (and KU/gpu/core/source.cu:11-20):
+ *   f[i0..i1, j0..j1, k0..k1] (+)= tval * (sv ? sv[box index] : svs)
+ * tval is the host's float32 cast of tfunc(tstep); sv is a C-order device array of
+ * the box (inclusive bounds) or NULL. */
+int mifdtd_source(float *f, int nx, int ny, int nz_pitch,
+                  int i0, int i1, int j0, int j1, int k0, int k1,
+                  float tval, const float *sv, float svs, int overwrite,
+                  mifdtd_stream_t stream);
+
+/* Probe / GetFields gather (K7/cpu/get_set_fields.py:47-50, K7/gpu/src/copy.cl:3-14):
+ *   out[n] = f[offsets[n]]  for n < count   (offsets: device int64 element offsets) */
+int mifdtd_gather(const float *f, const long long *offsets, int count, float *out,
+                  mifdtd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIFDTD_H */

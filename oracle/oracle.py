@@ -1,0 +1,323 @@
+"""ORACLE — test infrastructure only.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker (or, for bench.py, as the
+timed CPU baseline).  The product package ``fdtd_accelerate_amd`` never imports it.
+
+What it holds:
+
+* ``lib`` — ctypes handle on ``oracle/liboracle.so`` (``make -C oracle``), the scalar C
+  restatement of the reference hot path in ``oracle_body.h`` (float and double).
+* ``naive_update_e/h`` — a NumPy slice-form restatement of ``K7/naive/core.py:24-47``
+  (a second, independent formulation of the core update used to pin the C one).
+* ``ref_core(dtype, pad)`` — the reference's own KEMP C core compiled in place by
+  ``make -C oracle ref`` into ``oracle/_ref`` (only where /root/reference exists).
+* ``cpml_tables`` / ``pulse`` — closed forms of the CPML profile and the
+  Gaussian-modulated source (``KU/cpu/cpml.py:107-126``, ``KB/source/source.py:247-249``).
+* ``OracleSim`` — the full step (E core, CPML-E faces, E sources, H core, CPML-H faces,
+  H sources, probes) in the order of ``KU/cpu/test/020-dielectric-cpml.py:58-73`` and the
+  instance priorities of ``K7/util/common.py:383-397``.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "liboracle.so")
+_REF_DIR = os.path.join(HERE, "_ref")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+def build(ref: bool = False) -> None:
+    """Build liboracle.so (and, with ``ref``, the reference core in oracle/_ref)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    for sfx, real in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
+        getattr(lib, f"ora_update_e_{sfx}").argtypes = [_I] * 4 + [_P] * 9
+        getattr(lib, f"ora_update_h_{sfx}").argtypes = [_I] * 4 + [_P] * 9
+        getattr(lib, f"ora_cpml_e_{sfx}").argtypes = [_I] * 7 + [_P] * 13
+        getattr(lib, f"ora_cpml_h_{sfx}").argtypes = [_I] * 7 + [_P] * 13
+        getattr(lib, f"ora_src_{sfx}").argtypes = [_P] + [_I] * 9 + [real, _P, real, _I]
+        getattr(lib, f"ora_s4_update_h_{sfx}").argtypes = [_I] * 3 + [real] + [_P] * 9
+        getattr(lib, f"ora_s4_update_e_{sfx}").argtypes = [_I] * 3 + [real] + [_P] * 9
+        for n in ("update_e", "update_h", "cpml_e", "cpml_h", "src", "s4_update_h", "s4_update_e"):
+            getattr(lib, f"ora_{n}_{sfx}").restype = None
+    return lib
+
+
+lib = _load()
+
+
+def _sfx(a: np.ndarray) -> str:
+    if a.dtype == np.float32:
+        return "f32"
+    if a.dtype == np.float64:
+        return "f64"
+    raise TypeError(f"oracle arrays must be float32 or float64, got {a.dtype}")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous or a.flags.f_contiguous
+    return a.ctypes.data
+
+
+# ----------------------------------------------------------------------------- core
+def update_e(ehs, ces, pad: int) -> None:
+    """Core E update in place (``K7/cpu/src/core.c:11-55``); ``ces`` may be None (0.5)."""
+    ex, ey, ez, hx, hy, hz = ehs
+    nx, ny, nzp = ex.shape
+    cs = ces if ces is not None else (None, None, None)
+    getattr(lib, f"ora_update_e_{_sfx(ex)}")(nx, ny, nzp, pad, *[_ptr(a) for a in ehs],
+                                              *[_ptr(c) for c in cs])
+
+
+def update_h(ehs, chs, pad: int) -> None:
+    """Core H update in place (``K7/cpu/src/core.c:59-105``); ``chs`` may be None (0.5)."""
+    ex = ehs[0]
+    nx, ny, nzp = ex.shape
+    cs = chs if chs is not None else (None, None, None)
+    getattr(lib, f"ora_update_h_{_sfx(ex)}")(nx, ny, nzp, pad, *[_ptr(a) for a in ehs],
+                                              *[_ptr(c) for c in cs])
+
+
+def naive_update_e(ehs, ces, pad: int) -> None:
+    """NumPy slice form of ``K7/naive/core.py:24-34`` plus the zeroed lanes of the C core
+    (``K7/cpu/src/core.c:31,41,51``: ex/ey lanes k >= nz-1 and ez pad lanes become 0)."""
+    ex, ey, ez, hx, hy, hz = ehs
+    nzp = ex.shape[2]
+    nz = nzp - pad
+    dt = ex.dtype
+    cex, cey, cez = ces if ces is not None else [np.full(ex.shape, 0.5, dt)] * 3
+    z1 = slice(0, nz - 1)
+    z0 = slice(1, nz)
+    zz = slice(0, nz)
+    ex[:, :-1, z1] += cex[:, :-1, z1] * ((hz[:, 1:, z1] - hz[:, :-1, z1]) - (hy[:, :-1, z0] - hy[:, :-1, z1]))
+    ey[:-1, :, z1] += cey[:-1, :, z1] * ((hx[:-1, :, z0] - hx[:-1, :, z1]) - (hz[1:, :, z1] - hz[:-1, :, z1]))
+    ez[:-1, :-1, zz] += cez[:-1, :-1, zz] * ((hy[1:, :-1, zz] - hy[:-1, :-1, zz]) - (hx[:-1, 1:, zz] - hx[:-1, :-1, zz]))
+    ex[:, :-1, nz - 1:] = 0
+    ey[:-1, :, nz - 1:] = 0
+    ez[:-1, :-1, nz:] = 0
+
+
+def naive_update_h(ehs, chs, pad: int) -> None:
+    """NumPy slice form of ``K7/naive/core.py:37-47`` plus the zeroed lanes of the C core
+    (``K7/cpu/src/core.c:79-80,90-91,101``: hx/hy lane k=0 and pad lanes become 0)."""
+    ex, ey, ez, hx, hy, hz = ehs
+    nzp = ex.shape[2]
+    nz = nzp - pad
+    dt = ex.dtype
+    chx, chy, chz = chs if chs is not None else [np.full(ex.shape, 0.5, dt)] * 3
+    z0 = slice(1, nz)
+    z1 = slice(0, nz - 1)
+    zz = slice(0, nz)
+    hx[:, 1:, z0] -= chx[:, 1:, z0] * ((ez[:, 1:, z0] - ez[:, :-1, z0]) - (ey[:, 1:, z0] - ey[:, 1:, z1]))
+    hy[1:, :, z0] -= chy[1:, :, z0] * ((ex[1:, :, z0] - ex[1:, :, z1]) - (ez[1:, :, z0] - ez[:-1, :, z0]))
+    hz[1:, 1:, zz] -= chz[1:, 1:, zz] * ((ey[1:, 1:, zz] - ey[:-1, 1:, zz]) - (ex[1:, 1:, zz] - ex[1:, :-1, zz]))
+    hx[:, 1:, 0] = 0
+    hy[1:, :, 0] = 0
+    hx[:, 1:, nz:] = 0
+    hy[1:, :, nz:] = 0
+    hz[1:, 1:, nz:] = 0
+
+
+# --------------------------------------------------------------- reference KEMP core
+def ref_available() -> bool:
+    return os.path.isdir("/root/reference") or os.path.isdir(_REF_DIR)
+
+
+def ref_core(dtype, pad: int):
+    """The reference's ``update_e/update_h`` (``K7/cpu/src/core.c``) built in place with the
+    masks of the given pad; ``coeff_use='eh'`` signatures (6 fields + 3 coefficient arrays)."""
+    sfx = "f32" if np.dtype(dtype) == np.float32 else "f64"
+    path = os.path.join(_REF_DIR, f"libkemp_core_{sfx}_pad{pad}.so")
+    if not os.path.exists(path):
+        build(ref=True)
+    cl = ctypes.CDLL(path)
+    for n in ("update_e", "update_h"):
+        fn = getattr(cl, n)
+        fn.argtypes = [_I] * 3 + [_P] * 9
+        fn.restype = None
+    return cl
+
+
+# ------------------------------------------------------------------------- CPML
+def cpml_sigma_max(npml: int, m: int = 4, R: float = 1e-6, eta: float = 1.0, dx: float = 1.0) -> float:
+    """Textbook polynomial-grading sigma_max for a target normal reflection R."""
+    return -(m + 1) * math.log(R) / (2.0 * eta * npml * dx)
+
+
+def cpml_tables(npml: int, m: int = 4, R: float = 1e-6, dt: float = 0.5, dtype=np.float32):
+    """Closed-form CPML tables in the 2(npml+1) layout of ``KU/cpu/cpml.py:107-126``.
+
+    sigma_E front[p+1] = smax*((npml-0.5-p)/npml)^m, back[npml+1+p] = smax*((0.5+p)/npml)^m;
+    sigma_H front[p+1] = smax*((npml-p)/npml)^m,    back[npml+2+p] = smax*((1+p)/npml)^m.
+    kappa = 1, alpha = 0 (normalized units, eps0 = 1):  b = exp(-sigma*dt), a = b - 1
+    where sigma > 0 and a = 0 where sigma = 0.  Computed in float64, cast to ``dtype``.
+    """
+    smax = cpml_sigma_max(npml, m, R)
+    sE = np.zeros(2 * (npml + 1))
+    sH = np.zeros(2 * (npml + 1))
+    for p in range(npml):
+        sE[p + 1] = smax * ((npml - 0.5 - p) / npml) ** m
+        sE[npml + 1 + p] = smax * ((0.5 + p) / npml) ** m
+        sH[p + 1] = smax * ((npml - p) / npml) ** m
+        sH[npml + 2 + p] = smax * ((1.0 + p) / npml) ** m
+    bE = np.exp(-sE * dt)
+    bH = np.exp(-sH * dt)
+    aE = np.where(sE > 0, bE - 1.0, 0.0)
+    aH = np.where(sH > 0, bH - 1.0, 0.0)
+    return tuple(np.ascontiguousarray(t, dtype=dtype) for t in (bE, aE, bH, aH))
+
+
+def psi_shape(axis: int, nx: int, ny: int, nzp: int, npml: int):
+    return [(npml, ny, nzp), (nx, npml, nzp), (nx, ny, npml)][axis]
+
+
+def cpml_e(axis, side, ehs, ces, psi1, psi2, b, a, pad, npml):
+    ex = ehs[0]
+    nx, ny, nzp = ex.shape
+    cs = ces if ces is not None else (None, None, None)
+    getattr(lib, f"ora_cpml_e_{_sfx(ex)}")(axis, side, nx, ny, nzp, pad, npml,
+                                            *[_ptr(x) for x in ehs], *[_ptr(c) for c in cs],
+                                            _ptr(psi1), _ptr(psi2), _ptr(b), _ptr(a))
+
+
+def cpml_h(axis, side, ehs, chs, psi1, psi2, b, a, pad, npml):
+    ex = ehs[0]
+    nx, ny, nzp = ex.shape
+    cs = chs if chs is not None else (None, None, None)
+    getattr(lib, f"ora_cpml_h_{_sfx(ex)}")(axis, side, nx, ny, nzp, pad, npml,
+                                            *[_ptr(x) for x in ehs], *[_ptr(c) for c in cs],
+                                            _ptr(psi1), _ptr(psi2), _ptr(b), _ptr(a))
+
+
+# ----------------------------------------------------------------------- sources
+def pulse(t: float, t0: float = 100.0, width: float = 30.0, period: float = 20.0) -> float:
+    """Gaussian-modulated sinusoid of ``KB/source/source.py:247-249`` in step units:
+    exp(-0.5*((t-t0)/width)^2) * cos(2*pi*(t-t0)/period), evaluated in float64."""
+    return math.exp(-0.5 * ((t - t0) / width) ** 2) * math.cos(2.0 * math.pi * (t - t0) / period)
+
+
+def src_add(f, pt0, pt1, tval, svalue, overwrite=False):
+    nx, ny, nzp = f.shape
+    real = f.dtype.type
+    sv = svalue if isinstance(svalue, np.ndarray) else None
+    svs = 0.0 if sv is not None else float(real(svalue))
+    if sv is not None:
+        sv = np.ascontiguousarray(sv, dtype=f.dtype)
+    getattr(lib, f"ora_src_{_sfx(f)}")(_ptr(f), nx, ny, nzp, pt0[0], pt1[0], pt0[1], pt1[1],
+                                        pt0[2], pt1[2], float(real(tval)), _ptr(sv), svs,
+                                        int(overwrite))
+
+
+# --------------------------------------------------------------------- 4th order
+def s4_update_h(ehs, chs, cl):
+    ex = ehs[0]
+    nx, ny, nz = ex.shape
+    assert ex.flags.f_contiguous
+    getattr(lib, f"ora_s4_update_h_{_sfx(ex)}")(nx, ny, nz, float(ex.dtype.type(cl)),
+                                                 *[_ptr(a) for a in ehs], *[_ptr(c) for c in chs])
+
+
+def s4_update_e(ehs, ces, dl):
+    ex = ehs[0]
+    nx, ny, nz = ex.shape
+    assert ex.flags.f_contiguous
+    getattr(lib, f"ora_s4_update_e_{_sfx(ex)}")(nx, ny, nz, float(ex.dtype.type(dl)),
+                                                 *[_ptr(a) for a in ehs], *[_ptr(c) for c in ces])
+
+
+# --------------------------------------------------------------------- full step
+class OracleSim:
+    """CPU oracle of one Yee+CPML simulation on the KEMP layout.
+
+    Step order (``KU/cpu/test/020-dielectric-cpml.py:58-73``; instance priorities
+    core < pml < incident, ``K7/util/common.py:383-397``):
+      core E -> CPML-E (x f, x b, y f, y b, z f, z b) -> E sources -> core H ->
+      CPML-H (same face order) -> H sources -> probes (read after H, ``K7/main.py:303``).
+    Sources follow ``K7/cpu/incident_direct.py:75-77``: tstep starts at 1 and is
+    incremented after each use; the value is REAL(tfunc(tstep)) * spatial_value.
+    """
+
+    def __init__(self, nx, ny, nz, pad=0, npml=0, dtype=np.float32, ces=None, chs=None,
+                 cpml_faces=((True, True), (True, True), (True, True)), cpml_R=1e-6, cpml_m=4):
+        self.nx, self.ny, self.nz, self.pad = nx, ny, nz, pad
+        self.nzp = nz + pad
+        self.dtype = np.dtype(dtype)
+        shp = (nx, ny, self.nzp)
+        self.ehs = [np.zeros(shp, dtype) for _ in range(6)]
+        self.ces = None if ces is None else [np.ascontiguousarray(c, dtype) for c in ces]
+        self.chs = None if chs is None else [np.ascontiguousarray(c, dtype) for c in chs]
+        self.npml = npml
+        self.faces = cpml_faces
+        self.sources = []   # (str_f, pt0, pt1, tfunc, svalue, overwrite, [tstep])
+        self.probes = []    # (str_f, (i,j,k))
+        self.series = []
+        if npml > 0:
+            self.bE, self.aE, self.bH, self.aH = cpml_tables(npml, cpml_m, cpml_R, dtype=dtype)
+            self.psiE = {}
+            self.psiH = {}
+            for ax in range(3):
+                for sd in range(2):
+                    if self.faces[ax][sd]:
+                        s = psi_shape(ax, nx, ny, self.nzp, npml)
+                        self.psiE[ax, sd] = (np.zeros(s, dtype), np.zeros(s, dtype))
+                        self.psiH[ax, sd] = (np.zeros(s, dtype), np.zeros(s, dtype))
+
+    _names = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+    def field(self, name):
+        return self.ehs[self._names.index(name)]
+
+    def add_source(self, str_f, pt0, pt1, tfunc, svalue=1.0, overwrite=False):
+        self.sources.append([str_f, tuple(pt0), tuple(pt1), tfunc, svalue, overwrite, 1])
+
+    def add_probe(self, str_f, pt):
+        self.probes.append((str_f, tuple(pt)))
+
+    def _sources(self, e_or_h):
+        for s in self.sources:
+            if s[0][0] != e_or_h:
+                continue
+            src_add(self.field(s[0]), s[1], s[2], s[3](s[6]), s[4], s[5])
+            s[6] += 1
+
+    def step(self):
+        update_e(self.ehs, self.ces, self.pad)
+        if self.npml:
+            for ax in range(3):
+                for sd in range(2):
+                    if (ax, sd) in self.psiE:
+                        p1, p2 = self.psiE[ax, sd]
+                        cpml_e(ax, sd, self.ehs, self.ces, p1, p2, self.bE, self.aE, self.pad, self.npml)
+        self._sources("e")
+        update_h(self.ehs, self.chs, self.pad)
+        if self.npml:
+            for ax in range(3):
+                for sd in range(2):
+                    if (ax, sd) in self.psiH:
+                        p1, p2 = self.psiH[ax, sd]
+                        cpml_h(ax, sd, self.ehs, self.chs, p1, p2, self.bH, self.aH, self.pad, self.npml)
+        self._sources("h")
+        if self.probes:
+            self.series.append([self.field(n)[p] for n, p in self.probes])
+
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.step()

@@ -1,0 +1,112 @@
+"""Writes tests/golden/kemp_core.npz: golden vectors of the reference's own KEMP core.
+
+Run in the build container (needs /root/reference): it compiles
+K7/cpu/src/core.c in place (``make -C oracle ref``), drives it through ctypes on seeded
+random inputs, and stores inputs and outputs.  The parameter grid follows the
+reference's test (``K7/cpu/test/test_core.py:69-89``: ufunc e/h, nz 61..64, coeff_use
+'', 'e', 'h', tmax 1 and 10) on smaller grids, plus alternating E/H steps; one case
+at the reference's own grid (40, 50, 64, tmax 10, coeff 'e') is stored as a SHA-256
+digest of the output with its input seed.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+
+
+def rand_fields(rng, nx, ny, nz, nzp, n):
+    out = []
+    for _ in range(n):
+        a = np.zeros((nx, ny, nzp), np.float32)
+        a[:, :, :nz] = rng.random((nx, ny, nz), dtype=np.float32)
+        out.append(a)
+    return out
+
+
+def run_ref(pad, ehs, ces, chs, ufunc, tmax):
+    ref = O.ref_core(np.float32, pad)
+    nx, ny, nzp = ehs[0].shape
+    a = [x.copy() for x in ehs]
+    for _ in range(tmax):
+        if ufunc in ("e", "eh"):
+            ref.update_e(nx, ny, nzp, *[x.ctypes.data for x in a], *[c.ctypes.data for c in ces])
+        if ufunc in ("h", "eh"):
+            ref.update_h(nx, ny, nzp, *[x.ctypes.data for x in a], *[c.ctypes.data for c in chs])
+    return a
+
+
+def coeffs(rng, coeff_use, nx, ny, nz, nzp):
+    half = lambda: np.full((nx, ny, nzp), 0.5, np.float32)
+    ces = rand_fields(rng, nx, ny, nz, nzp, 3) if "e" in coeff_use else [half() for _ in range(3)]
+    chs = rand_fields(rng, nx, ny, nz, nzp, 3) if "h" in coeff_use else [half() for _ in range(3)]
+    for c in ces + chs:
+        c[:, :, nz:] = 0.5
+    return ces, chs
+
+
+def main():
+    O.build(ref=True)
+    rec = {}
+    nx, ny = 5, 6
+    case = 0
+    for nz in (13, 14, 15, 16):
+        nzp = (nz + 3) // 4 * 4
+        pad = nzp - nz
+        for ufunc, coeff_use, tmax in (("e", "", 1), ("e", "e", 1), ("h", "", 1), ("h", "h", 1),
+                                       ("eh", "eh", 5)):
+            rng = np.random.default_rng(1000 + case)
+            if ufunc == "e":
+                ehs = [np.zeros((nx, ny, nzp), np.float32) for _ in range(3)] + rand_fields(rng, nx, ny, nz, nzp, 3)
+            elif ufunc == "h":
+                ehs = rand_fields(rng, nx, ny, nz, nzp, 3) + [np.zeros((nx, ny, nzp), np.float32) for _ in range(3)]
+            else:
+                ehs = rand_fields(rng, nx, ny, nz, nzp, 6)
+            ces, chs = coeffs(rng, coeff_use, nx, ny, nz, nzp)
+            out = run_ref(pad, ehs, ces, chs, ufunc, tmax)
+            p = f"c{case}_"
+            rec[p + "meta"] = np.array([nx, ny, nz, nzp, pad, tmax, {"e": 0, "h": 1, "eh": 2}[ufunc],
+                                        int("e" in coeff_use), int("h" in coeff_use)], np.int64)
+            rec[p + "in"] = np.stack(ehs)
+            if "e" in coeff_use:
+                rec[p + "ce"] = np.stack(ces)
+            if "h" in coeff_use:
+                rec[p + "ch"] = np.stack(chs)
+            rec[p + "out"] = np.stack(out)
+            case += 1
+    rec["ncases"] = np.array(case)
+
+    # the reference test's own grid, stored as a digest (K7/cpu/test/test_core.py:86)
+    nx, ny, nz = 40, 50, 64
+    seed = 4242
+    rng = np.random.default_rng(seed)
+    ehs = [np.zeros((nx, ny, nz), np.float32) for _ in range(3)] + rand_fields(rng, nx, ny, nz, nz, 3)
+    ces = rand_fields(rng, nx, ny, nz, nz, 3)
+    chs = [np.full((nx, ny, nz), 0.5, np.float32) for _ in range(3)]
+    out = run_ref(0, ehs, ces, chs, "e", 10)
+    h = hashlib.sha256(b"".join(o.tobytes() for o in out[:3])).hexdigest()
+    rec["big_seed"] = np.array(seed)
+    rec["big_sha256"] = np.array(h)
+    rec["big_probe"] = np.array([out[0][20, 25, 30], out[1][10, 5, 3], out[2][39 - 1, 48, 63]], np.float32)
+    np.savez_compressed(os.path.join(HERE, "kemp_core.npz"), **rec)
+    print("wrote", case, "cases; big digest", h)
+
+
+def big_inputs(seed):
+    """Inputs of the digest case (shared with the tests)."""
+    nx, ny, nz = 40, 50, 64
+    rng = np.random.default_rng(int(seed))
+    ehs = [np.zeros((nx, ny, nz), np.float32) for _ in range(3)] + rand_fields(rng, nx, ny, nz, nz, 3)
+    ces = rand_fields(rng, nx, ny, nz, nz, 3)
+    return ehs, ces
+
+
+if __name__ == "__main__":
+    main()

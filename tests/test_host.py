@@ -1,0 +1,135 @@
+"""CPU tests of the host side: argument checks with the reference's exceptions, index
+helpers (the reference's doctest values), the C ABI library exports, media generator."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from fdtd_accelerate_amd import util
+from tests.conftest import ROOT
+
+
+def test_check_type_and_value_errors():
+    # K7/util/common.py:400-428 (TestFunctions)
+    with pytest.raises(TypeError):
+        util.check_type("", 2, float)
+    with pytest.raises(TypeError):
+        util.check_type("", 3.2, str)
+    with pytest.raises(TypeError):
+        util.check_type("", [1, 2], list)
+    with pytest.raises(TypeError):
+        util.check_type("", (1.2, 2.3), tuple, int)
+    with pytest.raises(ValueError):
+        util.check_value("", "a", ("b", "c"))
+    util.check_type("a", (1.2, 2.3), tuple, float)
+    util.check_value("a", "a", ("a", "b"))
+
+
+def test_index_helpers_match_reference_doctests():
+    assert util.convert_indices((10, 12, 14), (1, 2, 3)) == (1, 2, 3)
+    assert util.convert_indices((10, 12, 14), (-1, -2, -3)) == (9, 10, 11)
+    assert util.convert_indices((10, 12, 14), (0.1, 0.2, 0.3)) == (1, 2, 4)
+    assert util.slices_two_points((0, 0, 0), (10, 11, 12)) == (slice(0, 11), slice(0, 12), slice(0, 13))
+    assert util.slices_two_points((0, 0, 0), (10, 0, 12)) == (slice(0, 11), 0, slice(0, 13))
+    assert util.slices_two_points((1, 2, 3), (1, 2, 3)) == (1, 2, 3)
+    assert util.shape_two_points((0, 0, 0), (10, 11, 12)) == (11, 12, 13)
+    assert util.shape_two_points((0, 0, 0), (10, 0, 12)) == (11, 13)
+    assert util.shape_two_points((1, 2, 3), (1, 2, 3)) == (1,)
+    assert util.shape_two_points((0, 0, 0), (10, 11, 12), 2) == (22, 12, 13)
+    assert util.shape_two_points((0, 0, 0), (0, 0, 12), is_dummy=True) == (1, 1, 13)
+
+
+def test_instance_priority_order():
+    class I:
+        def __init__(self, p):
+            self.priority_type = p
+    lst = []
+    for p in ("incident", "core", "pml", "core", "incident"):
+        util.append_instance(lst, I(p))
+    assert [x.priority_type for x in lst] == ["core", "core", "pml", "incident", "incident"]
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "mifdtd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(?:void|int|const char\s*\*)\s*\**\s*(\w+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_library_exports_header_symbols():
+    """libmifdtd.so loads without a GPU and exports every function of include/mifdtd.h."""
+    from fdtd_accelerate_amd import _lib
+    lib = _lib.load()
+    names = _header_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.EXPORTS)
+    assert lib.mifdtd_version() >= 1
+
+
+def test_argument_errors_without_gpu():
+    """Rejected arguments come back as status codes before any device work."""
+    from fdtd_accelerate_amd import _lib
+    lib = _lib.load()
+    rc = lib.mifdtd_update_e(4, 4, 6, 0, *([None] * 9), None)
+    assert rc == -2 and "multiple of 4" in _lib.last_error()
+    rc = lib.mifdtd_update_e(4, 4, 8, 0, *([None] * 9), None)
+    assert rc == -3
+    rc = lib.mifdtd_update_e(4, 4, 8, 7, *([16] * 6), None, None, None, None)
+    assert rc == -1 and "pad" in _lib.last_error()
+    rc = lib.mifdtd_update_h(4, 4, 8, 0, *([8] * 6), None, None, None, None)
+    assert rc == -2 and "aligned" in _lib.last_error()
+    d = _lib.CpmlHalf()
+    d.npml = 4
+    rc = lib.mifdtd_cpml_e(8, 8, 8, 0, d, *([16] * 6), None, None, None, None)
+    assert rc == -1 and "too small for npml" in _lib.last_error()
+    rc = lib.mifdtd_source(16, 4, 4, 8, 0, 4, 0, 0, 0, 0, 1.0, None, 1.0, 0, None)
+    assert rc == -1 and "outside" in _lib.last_error()
+    with pytest.raises(RuntimeError):
+        _lib.call("mifdtd_set_pad", -1)
+
+
+def test_fields_requires_gpu():
+    import torch
+    from fdtd_accelerate_amd import Fields
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError):
+        Fields(8, 8, 8)
+    with pytest.raises(ValueError):
+        Fields(8, 8, 8, precision_float="double")
+    with pytest.raises(TypeError):
+        Fields(8.0, 8, 8)
+
+
+def test_random_spheres_deterministic_and_disjoint():
+    from fdtd_accelerate_amd.geometry import random_spheres
+    a = random_spheres(3, 30, (3, 8), (1, 12), (10, 10, 10), (90, 90, 90))
+    b = random_spheres(3, 30, (3, 8), (1, 12), (10, 10, 10), (90, 90, 90))
+    assert [(s.center, s.radius, s.eps_r) for s in a] == [(s.center, s.radius, s.eps_r) for s in b]
+    for i, s in enumerate(a):
+        c = np.array(s.center)
+        assert np.all(c - s.radius >= 10) and np.all(c + s.radius <= 90)
+        assert 1 <= s.eps_r <= 12 and 3 <= s.radius <= 8
+        for t in a[:i]:
+            assert np.linalg.norm(c - np.array(t.center)) >= s.radius + t.radius
+
+
+def test_paint_ce_at_yee_points():
+    """CE = 0.5/eps at each component's staggered point: a sphere centred on an Ex
+    point includes it; a point just outside the radius keeps the background."""
+    import torch
+    from fdtd_accelerate_amd.geometry import Sphere, paint_ce
+    ces = [torch.zeros((20, 20, 20)) for _ in range(3)]
+    s = Sphere((10.0, 10.5, 10.5), 3.2, 4.0)
+    paint_ce(ces, [s], (20, 20, 20))
+    assert ces[0][10, 10, 10] == np.float32(0.125)       # Ex(10,10,10) at (10, 10.5, 10.5): centre
+    assert ces[0][13, 10, 10] == np.float32(0.125)       # distance 3 < 3.2
+    assert ces[0][14, 10, 10] == np.float32(0.5)         # distance 4
+    assert ces[1][10, 10, 10] == np.float32(0.125)       # Ey at (10.5, 10, 10.5): distance ~0.71
+    assert ces[2][11, 12, 10] == np.float32(0.125)       # Ez at (11.5, 12.5, 10): distance ~2.55
+    assert ces[2][13, 12, 10] == np.float32(0.5)         # Ez at (13.5, 12.5, 10): distance ~4.06
+    n_in = int((ces[0] == np.float32(0.125)).sum())
+    assert abs(n_in - 4 / 3 * np.pi * 3.2 ** 3) < 0.25 * 4 / 3 * np.pi * 3.2 ** 3
