@@ -22,6 +22,24 @@ class CpmlHalf(ctypes.Structure):
     _fields_ = [("npml", _I), ("b", _P), ("a", _P), ("psi1", _P * 6), ("psi2", _P * 6)]
 
 
+MAX_SRC = 4
+MAX_PROBE = 8
+
+
+class FusedDesc(ctypes.Structure):
+    """``mifdtd_fused_desc`` (include/mifdtd.h)."""
+    _fields_ = [
+        ("nx", _I), ("ny", _I), ("nz_pitch", _I), ("pad", _I),
+        ("e", _P * 6), ("e_alt", _P * 6), ("ce", _P * 3),
+        ("npml", _I), ("bE", _P), ("aE", _P), ("bH", _P), ("aH", _P),
+        ("psiE", _P), ("psiE_alt", _P), ("psiH", _P),
+        ("nsrc", _I), ("src_comp", _I * MAX_SRC), ("src_box", (_I * 6) * MAX_SRC),
+        ("src_sv", _P * MAX_SRC), ("src_svs", _F * MAX_SRC), ("src_overwrite", _I * MAX_SRC),
+        ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
+        ("rows_per_thread", _I), ("planes_per_chunk", _I),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "mifdtd_last_error": (ctypes.c_char_p, []),
@@ -41,6 +59,7 @@ _SIGS = {
     "mifdtd_cpml_h": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
+    "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
 }
 
 # Symbols declared in include/mifdtd.h that the library must export.

@@ -1,0 +1,65 @@
+"""Tile-shape selection for the fused step kernel.
+
+``fused_tiles`` returns (rows_per_thread T, planes_per_chunk CX) for a grid: a table of
+measured winners (filled by ``tune_fused``) keyed by (nx, ny, nz_pitch, npml), else a
+heuristic that keeps enough work-groups in flight (>= 4 per CU on 256 CUs) while
+keeping the warm-up plane overhead (1/CX) small.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
+_table = None
+N_CU = 256
+
+
+def _load():
+    global _table
+    if _table is None:
+        try:
+            with open(_TABLE_PATH) as fh:
+                _table = {tuple(json.loads(k)): tuple(v) for k, v in json.load(fh).items()}
+        except (OSError, ValueError):
+            _table = {}
+    return _table
+
+
+def heuristic(nx, ny, nzp, npml):
+    """T rows per wave (y halo overhead 1/T), CX planes per chunk (seed overhead 1/CX),
+    keeping >= 4 waves per SIMD of work (4096 waves on 256 CUs)."""
+    T = 8 if ny >= 64 else max(1, ny // 8)
+    nrb = -(-ny // T)
+    target = 4 * 4 * N_CU
+    cx = nx
+    while cx > 4 and nrb * (-(-nx // cx)) < target:
+        cx = -(-cx // 2)
+    return T, max(1, cx)
+
+
+def fused_tiles(nx, ny, nzp, npml):
+    t = _load().get((nx, ny, nzp, npml))
+    return t if t is not None else heuristic(nx, ny, nzp, npml)
+
+
+def tune_fused(sim, candidates=((4, 16), (4, 32), (8, 16), (8, 32), (8, 64), (16, 32), (16, 64)), steps=5, save=True):
+    """Time each (T, CX) on ``sim`` (a YeeSim with engine='fused') and record the best.
+    Runs real steps (the simulation advances by len(candidates) * (steps + 1) steps)."""
+    from .fused import FusedStepper
+    f = sim.fields
+    best = None
+    results = {}
+    for T, CX in candidates:
+        st = FusedStepper(sim, rows_per_thread=T, planes_per_chunk=CX)
+        st.run(1, None)
+        ms = st.run(steps, None, time_kernel=True)
+        results[(T, CX)] = ms
+        if best is None or ms < results[best]:
+            best = (T, CX)
+    key = (f.nx, f.ny, f.nz_pitch, sim.cpml.npml if sim.cpml else 0)
+    _load()[key] = best
+    if save:
+        with open(_TABLE_PATH, "w") as fh:
+            json.dump({json.dumps(list(k)): list(v) for k, v in _table.items()}, fh, indent=1)
+    return best, results

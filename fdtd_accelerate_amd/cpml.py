@@ -78,19 +78,52 @@ class Cpml:
         t = lambda x: torch.tensor(x, dtype=torch.float32, device=dev)
         self.bE, self.aE, self.bH, self.aH = t(bE), t(aE), t(bH), t(aH)
         nx, ny, nzp = fields.ns_pitch
-        self.psiE = {}
-        self.psiH = {}
+        # psi of all six faces packed in one buffer per half (psi1 then psi2 of each face,
+        # x faces, y faces, z faces); faces that are off keep their (zero) space
+        self._sizes = [npml * ny * nzp, nx * npml * nzp, nx * ny * npml]
+        total = 4 * sum(self._sizes)
+        self.psiE_buf = torch.zeros(total, dtype=torch.float32, device=dev)
+        self.psiH_buf = torch.zeros(total, dtype=torch.float32, device=dev)
+        self.psiE_alt_buf = None  # second psi-E buffer of the fused engine (allocated on demand)
+        self._build_views()
+        _lib.load()
+        fields.append_instance(self)
+
+    def face_offset(self, face, comp):
+        """Float offset of psi<comp+1> of face (2*axis+side) in a packed buffer."""
+        ax, sd = divmod(face, 2)
+        sX, sY, sZ = self._sizes
+        base = [0, 4 * sX, 4 * sX + 4 * sY][ax]
+        return base + (2 * sd + comp) * self._sizes[ax]
+
+    def _views(self, buf):
+        nx, ny, nzp = self.mainf.ns_pitch
+        out = {}
         for ax in range(3):
             for sd, c in enumerate("fb"):
                 if c in self.apply_direction[ax]:
-                    shp = psi_shape(ax, nx, ny, nzp, npml)
-                    z = lambda: torch.zeros(shp, dtype=torch.float32, device=dev)
-                    self.psiE[ax, sd] = (z(), z())
-                    self.psiH[ax, sd] = (z(), z())
+                    f = 2 * ax + sd
+                    n = self._sizes[ax]
+                    shp = psi_shape(ax, nx, ny, nzp, self.npml)
+                    o1, o2 = self.face_offset(f, 0), self.face_offset(f, 1)
+                    out[ax, sd] = (buf[o1:o1 + n].view(shp), buf[o2:o2 + n].view(shp))
+        return out
+
+    def _build_views(self):
+        self.psiE = self._views(self.psiE_buf)
+        self.psiH = self._views(self.psiH_buf)
         self._descE = self._desc(self.psiE, self.bE, self.aE)
         self._descH = self._desc(self.psiH, self.bH, self.aH)
-        _lib.load()
-        fields.append_instance(self)
+
+    def alt_psiE(self):
+        if self.psiE_alt_buf is None:
+            self.psiE_alt_buf = torch.zeros_like(self.psiE_buf)
+        return self.psiE_alt_buf
+
+    def swap_psiE(self):
+        """Make the alternate psi-E buffer current (after an odd number of fused steps)."""
+        self.psiE_buf, self.psiE_alt_buf = self.psiE_alt_buf, self.psiE_buf
+        self._build_views()
 
     def _desc(self, psi, b, a):
         d = _lib.CpmlHalf()
@@ -103,9 +136,10 @@ class Cpml:
         return d
 
     def reset(self):
-        for p1, p2 in list(self.psiE.values()) + list(self.psiH.values()):
-            p1.zero_()
-            p2.zero_()
+        self.psiE_buf.zero_()
+        self.psiH_buf.zero_()
+        if self.psiE_alt_buf is not None:
+            self.psiE_alt_buf.zero_()
 
     def update_e(self):
         f = self.mainf

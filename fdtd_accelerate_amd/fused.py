@@ -1,0 +1,124 @@
+"""Host driver of the fused single-pass step (``mifdtd_fused_run``).
+
+The fused kernel reads the state of step n from one buffer set and writes step n+1 to
+a second (ping-pong).  ``FusedStepper`` owns the second field set and the second psi-E
+buffer, builds the descriptor from the ``YeeSim`` objects (fields, CPML, sources,
+probes), runs K steps in one C call (the C loop launches K kernels back to back) and,
+after an odd K, swaps the buffers so that ``sim.fields`` / ``sim.cpml`` always hold
+the current state.  Sources advance their tstep exactly as the in-place path does.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .autotune import fused_tiles
+
+_COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
+
+
+class FusedStepper:
+    kernel_name = "k_fused"
+
+    def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None):
+        f = sim.fields
+        if f.ch_on:
+            raise ValueError("the fused engine uses CH = 0.5; use engine='kemp' for CH arrays")
+        cp = sim.cpml
+        if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
+            raise ValueError("the fused engine needs CPML on all six faces; use engine='kemp'")
+        if len(sim.sources) > _lib.MAX_SRC:
+            raise ValueError(f"the fused engine supports at most {_lib.MAX_SRC} sources")
+        if len(sim.probe_points) > _lib.MAX_PROBE:
+            raise ValueError(f"the fused engine supports at most {_lib.MAX_PROBE} probes")
+        self.sim = sim
+        self.alt = [torch.empty_like(t) for t in f.ehs]
+        T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0)
+        self.T = rows_per_thread or T
+        self.CX = planes_per_chunk or CX
+        self.bytes_per_cell = 60.0 if f.ce_on else 48.0
+        if cp is not None:
+            cp.alt_psiE()
+
+    def _desc(self):
+        sim = self.sim
+        f = sim.fields
+        d = _lib.FusedDesc()
+        d.nx, d.ny, d.nz_pitch, d.pad = f.nx, f.ny, f.nz_pitch, f.pad
+        for c in range(6):
+            d.e[c] = f.ehs[c].data_ptr()
+            d.e_alt[c] = self.alt[c].data_ptr()
+        if f.ce_on:
+            for c in range(3):
+                d.ce[c] = f.ces[c].data_ptr()
+        cp = sim.cpml
+        if cp is not None:
+            d.npml = cp.npml
+            d.bE, d.aE, d.bH, d.aH = (cp.bE.data_ptr(), cp.aE.data_ptr(), cp.bH.data_ptr(), cp.aH.data_ptr())
+            d.psiE = cp.psiE_buf.data_ptr()
+            d.psiE_alt = cp.alt_psiE().data_ptr()
+            d.psiH = cp.psiH_buf.data_ptr()
+        d.nsrc = len(sim.sources)
+        for n, s in enumerate(sim.sources):
+            d.src_comp[n] = _COMP[s.str_f]
+            box = (s.pt0[0], s.pt1[0], s.pt0[1], s.pt1[1], s.pt0[2], s.pt1[2])
+            for m in range(6):
+                d.src_box[n][m] = box[m]
+            d.src_sv[n] = None if s.sv is None else s.sv.data_ptr()
+            d.src_svs[n] = s.svs
+            d.src_overwrite[n] = int(s.is_overwrite)
+        d.nprobe = len(sim.probe_points)
+        for n, (c, (i, j, k)) in enumerate(sim.probe_points):
+            d.probe_comp[n] = c
+            d.probe_ijk[n][0], d.probe_ijk[n][1], d.probe_ijk[n][2] = i, j, k
+        d.rows_per_thread = self.T
+        d.planes_per_chunk = self.CX
+        return d
+
+    def _tvals(self, nsteps):
+        srcs = self.sim.sources
+        tv = np.zeros((max(nsteps, 1), max(len(srcs), 1)), np.float32)
+        for n, s in enumerate(srcs):
+            for k in range(nsteps):
+                tv[k, n] = np.float32(s.tfunc(s.tstep + k))
+        return np.ascontiguousarray(tv)
+
+    def _call(self, nsteps, buf):
+        sim = self.sim
+        f = sim.fields
+        if buf is None and sim.probe_points:
+            buf = torch.zeros((nsteps, len(sim.probe_points)), dtype=torch.float32, device=f.device)
+        d = self._desc()
+        tv = self._tvals(nsteps)
+        parity = ctypes.c_int(0)
+        _lib.call("mifdtd_fused_run", ctypes.byref(d), nsteps, tv.ctypes.data,
+                  None if buf is None else buf.data_ptr(), ctypes.byref(parity),
+                  _lib.stream_handle(f.stream))
+        for s in sim.sources:
+            s.tstep += nsteps
+        if parity.value:
+            f.ehs, self.alt = self.alt, f.ehs
+            if sim.cpml is not None:
+                sim.cpml.swap_psiE()
+
+    def run(self, nsteps, buf, time_kernel=False):
+        """Advance nsteps; ``buf`` is an optional device (nsteps, nprobe) probe buffer.
+        With ``time_kernel`` each launch is bracketed by HIP events on the fields'
+        stream and the average launch time (ms) is returned."""
+        if not time_kernel:
+            self._call(nsteps, buf)
+            return None
+        st = self.sim.fields.stream
+        evs = []
+        for k in range(nsteps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            self._call(1, None if buf is None else buf[k:k + 1])
+            b.record(st)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in evs) / max(nsteps, 1)

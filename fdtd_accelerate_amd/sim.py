@@ -52,39 +52,31 @@ class YeeSim:
         i, j, k = pt
         if not (0 <= i < f.nx and 0 <= j < f.ny and 0 <= k < f.nz):
             raise ValueError(f"probe point {pt} outside the grid {f.ns}")
-        self.probe_points.append((_NAMES.index(str_f), (i * f.ny + j) * f.nz_pitch + k))
+        self.probe_points.append((_NAMES.index(str_f), (int(i), int(j), int(k))))
         self._probe_off = None
 
     def _probe_offsets(self):
         if self._probe_off is None and self.probe_points:
-            self._probe_off = {}
-            for c in range(6):
-                offs = [o for cc, o in self.probe_points if cc == c]
-                if offs:
-                    self._probe_off[c] = torch.tensor(offs, dtype=torch.int64, device=self.fields.device)
+            f = self.fields
+            offs = [(i * f.ny + j) * f.nz_pitch + k for _, (i, j, k) in self.probe_points]
+            self._probe_off = torch.tensor(offs, dtype=torch.int64, device=f.device)
         return self._probe_off
 
     # -------------------------------------------------------------- stepping
     def _record(self, buf, t):
+        """Gather every probe (insertion order) into row t of the device buffer."""
         offs = self._probe_offsets()
-        if not offs:
-            return
         f = self.fields
-        col = 0
-        for c, o in offs.items():
-            n = o.numel()
-            out = buf[t, col:col + n]
-            _lib.call("mifdtd_gather", f.ehs[c].data_ptr(), o.data_ptr(), n, out.data_ptr(),
-                      _lib.stream_handle(f.stream))
-            col += n
+        for n, (c, _) in enumerate(self.probe_points):
+            _lib.call("mifdtd_gather", f.ehs[c].data_ptr(), offs[n:n + 1].data_ptr(), 1,
+                      buf[t, n:n + 1].data_ptr(), _lib.stream_handle(f.stream))
 
     def run(self, nsteps):
         """Advance ``nsteps`` steps; returns the probe series of these steps (host array
-        (nsteps, nprobes), column order grouped by component) or None."""
+        (nsteps, nprobes), columns in the order the probes were added) or None."""
         f = self.fields
-        offs = self._probe_offsets()
-        nprobe = sum(o.numel() for o in offs.values()) if offs else 0
-        buf = torch.zeros((nsteps, max(nprobe, 1)), dtype=torch.float32, device=f.device) if nprobe else None
+        nprobe = len(self.probe_points)
+        buf = torch.zeros((nsteps, nprobe), dtype=torch.float32, device=f.device) if nprobe else None
         if self.engine == "fused":
             from .fused import FusedStepper
             if self._fused is None:

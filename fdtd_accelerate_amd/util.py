@@ -23,7 +23,7 @@ def check_type(arg_name, arg, arg_type, element_type=None):
 
 def check_value(arg_name, arg, value):
     """Raise ValueError unless ``arg`` is one of ``value`` (``K7/util/common.py:40-52``)."""
-    vals = value if isinstance(value, (list, tuple)) else (value,)
+    vals = value if isinstance(value, (list, tuple, range, set, frozenset)) else (value,)
     if arg not in vals:
         r = repr(value)
         if isinstance(value, (list, tuple)) and len(r) > 40:

@@ -144,6 +144,53 @@ int mifdtd_source(float *f, int nx, int ny, int nz_pitch,
 int mifdtd_gather(const float *f, const long long *offsets, int count, float *out,
                   mifdtd_stream_t stream);
 
+
+/* ------------------------------------------------------------- fused step ----
+ * The whole time step (core E, CPML-E on all faces, E sources, core H, CPML-H, H
+ * sources, probes) in one kernel launch per step, streaming each array once:
+ * state of step n is read from e[] / psiE*[], step n+1 is written to e_alt[] /
+ * psiE*_alt[] (every cell, walls copied), and the two sets swap every step.  psiH is
+ * updated in place.  Same per-cell arithmetic as update_e/update_h + the CPML and
+ * source calls above (values bit-identical; masked zero lanes may differ in sign).
+ * No reference counterpart: it fuses the reference step loop of
+ * KU/cpu/test/020-dielectric-cpml.py:58-73 (E, CPML-E, source, H, CPML-H).
+ * Limits: CPML on all six faces or none, CH = 0.5 (no CH arrays),
+ * at most MIFDTD_FUSED_MAX_SRC sources and MIFDTD_FUSED_MAX_PROBE probes.
+ * Components: 0..5 = ex, ey, ez, hx, hy, hz.  Face index f = 2*axis + side.
+ * MIFDTD_PSI_PACKED layout of psiE / psiE_alt / psiH (floats): for f = 0..5, psi1 then
+ * psi2 of face f, each face_size(f) long: x faces npml*ny*nz_pitch, y faces
+ * nx*npml*nz_pitch, z faces nx*ny*npml (z last so x/y faces stay 16-byte aligned);
+ * 4*(sX+sY+sZ) floats per half.  The per-face psi arrays of mifdtd_cpml_e/h may be
+ * views into such a buffer. */
+#define MIFDTD_FUSED_MAX_SRC 4
+#define MIFDTD_FUSED_MAX_PROBE 8
+typedef struct {
+    int nx, ny, nz_pitch, pad;
+    float *e[6];          /* ex, ey, ez, hx, hy, hz at step n */
+    float *e_alt[6];      /* the other buffer set */
+    const float *ce[3];   /* cex, cey, cez or all NULL (0.5) */
+    int npml;             /* 0: no CPML; else all six faces */
+    const float *bE, *aE, *bH, *aH;
+    float *psiE, *psiE_alt, *psiH;  /* packed psi buffers (MIFDTD_PSI_PACKED layout) */
+    int nsrc;
+    int src_comp[MIFDTD_FUSED_MAX_SRC];
+    int src_box[MIFDTD_FUSED_MAX_SRC][6];     /* i0, i1, j0, j1, k0, k1 (inclusive) */
+    const float *src_sv[MIFDTD_FUSED_MAX_SRC]; /* C-order box array or NULL */
+    float src_svs[MIFDTD_FUSED_MAX_SRC];
+    int src_overwrite[MIFDTD_FUSED_MAX_SRC];
+    int nprobe;
+    int probe_comp[MIFDTD_FUSED_MAX_PROBE];
+    int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
+    int rows_per_thread;  /* y rows streamed by one wave, 1..64 */
+    int planes_per_chunk; /* x planes per wave sweep */
+} mifdtd_fused_desc;
+
+/* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
+ * of each source; probe_series: DEVICE array (nsteps, nprobe).  On return *parity is
+ * nsteps & 1: 1 means the current state is in e_alt[] / psiE*_alt[]. */
+int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
+                     float *probe_series, int *parity, mifdtd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,139 @@
+"""GPU parity of the fused single-pass step against the oracle and the in-place path.
+
+Bar: every value bit-identical (compared as bytes after mapping -0.0 to +0.0: the
+fused kernel may store +0 where the in-place path stores value*0 = -0 in a masked lane),
+for every tile shape, chunk length, odd/even step counts and grid edge cases."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from fdtd_accelerate_amd import _lib
+    _lib.load()
+
+
+def canon(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return (a + np.float32(0.0)).tobytes()
+
+
+def same_values(a, b):
+    return np.shape(a) == np.shape(b) and canon(a) == canon(b)
+
+
+def _sim(ns, npml, engine, T=None, CX=None, ce=True):
+    from fdtd_accelerate_amd.sim import YeeSim
+    fp = {}
+    if T:
+        fp["rows_per_thread"] = T
+    if CX:
+        fp["planes_per_chunk"] = CX
+    return YeeSim(*ns, npml=npml, coeff_use="e" if ce else "", engine=engine, fused_params=fp)
+
+
+def _fill(sim, seed):
+    rng = np.random.default_rng(seed)
+    f = sim.fields
+    nx, ny, nz = f.ns
+    f.set_ehs(*[rng.standard_normal((nx, ny, nz)).astype(np.float32) for _ in range(6)])
+    if f.ce_on:
+        f.set_ces(*[(0.5 / rng.uniform(1, 6, (nx, ny, nz))).astype(np.float32) for _ in range(3)])
+
+
+@pytest.mark.parametrize("ns,npml,T,CX,steps", [
+    ((24, 20, 32), 0, 1, 7, 3),
+    ((24, 20, 32), 0, 2, 64, 4),
+    ((17, 23, 45), 0, 4, 5, 5),
+    ((30, 26, 40), 4, 2, 8, 6),
+    ((30, 26, 40), 4, 8, 3, 5),
+    ((29, 31, 57), 5, 4, 16, 7),
+    ((40, 36, 160), 6, 2, 11, 4),
+    ((22, 22, 200), 3, 1, 1, 3),
+    ((12, 13, 1000), 4, 3, 5, 3),
+    ((26, 70, 300), 5, 16, 9, 2),
+])
+def test_fused_matches_inplace(ns, npml, T, CX, steps):
+    """Random fields (walls included) + CE + CPML + a point and a plane source + probes:
+    fused engine == in-place engine after `steps` steps, fields, psi and probe series."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", T, CX)]
+    nx, ny, nz = ns
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    for s in sims:
+        _fill(s, 42)
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_source("hy", (3, 1, 0), (3, ny - 2, nz - 1), tf, 0.25)
+        s.add_probe("ez", (nx // 2 + 1, ny // 2, nz // 2))
+        s.add_probe("hx", (1, 2, 3))
+        s.add_probe("hz", (nx - 1, ny - 1, nz - 1))
+    series = [s.run(steps) for s in sims]
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert same_values(a, b), (name, np.abs(a - b).max())
+    assert same_values(series[0], series[1])
+    if npml:
+        for key in sims[0].cpml.psiE:
+            for c in range(2):
+                assert same_values(sims[0].cpml.psiE[key][c].cpu().numpy(), sims[1].cpml.psiE[key][c].cpu().numpy()), ("psiE", key, c)
+                assert same_values(sims[0].cpml.psiH[key][c].cpu().numpy(), sims[1].cpml.psiH[key][c].cpu().numpy()), ("psiH", key, c)
+
+
+def test_fused_continues_across_calls_and_engines():
+    """Steps split over several run() calls (odd/even parity, buffer swaps) and an engine
+    switch mid-run equal one continuous run."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, npml = (26, 28, 36), 4
+    tf = gaussian_pulse(4.0, 2.0, 8.0)
+    a = _sim(ns, npml, "kemp")
+    b = _sim(ns, npml, "fused", 2, 6)
+    for s in (a, b):
+        _fill(s, 7)
+        s.add_source("ex", (13, 14, 18), (13, 14, 18), tf)
+    a.run(9)
+    b.run(3)
+    b.run(1)
+    b.engine = "kemp"
+    b.run(2)
+    b.engine = "fused"
+    b.run(3)
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
+
+
+def test_fused_config0_vs_oracle():
+    """Config 0 (128^3 vacuum, CPML 10, point source, probe) on the fused engine, 400 steps:
+    probe series and fields equal the float32 oracle."""
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    steps = 400
+    sim = YeeSim(128, 128, 128, npml=10, coeff_use="", engine="fused")
+    sim.add_source("ez", (64, 64, 64), (64, 64, 64), gaussian_pulse())
+    sim.add_probe("ez", (96, 64, 64))
+    series = sim.run(steps)[:, 0]
+    o = O.OracleSim(128, 128, 128, 0, 10, np.float32)
+    o.add_source("ez", (64, 64, 64), (64, 64, 64), O.pulse)
+    o.add_probe("ez", (96, 64, 64))
+    o.run(steps)
+    assert same_values(series, np.array(o.series, np.float32)[:, 0])
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert same_values(sim.field(name).cpu().numpy(), o.field(name)), name
+
+
+def test_fused_rejects_unsupported():
+    from fdtd_accelerate_amd.fused import FusedStepper
+    from fdtd_accelerate_amd.sim import YeeSim
+    s = YeeSim(24, 24, 24, npml=4, cpml_faces=("fb", "f", "fb"), engine="fused")
+    with pytest.raises(ValueError):
+        FusedStepper(s)
+    s2 = YeeSim(8, 8, 16, coeff_use="eh", engine="fused")
+    with pytest.raises(ValueError):
+        FusedStepper(s2)

@@ -93,7 +93,7 @@ def test_core_reference_digest(golden):
     assert hashlib.sha256(b"".join(g.tobytes() for g in got)).hexdigest() == str(golden["big_sha256"])
 
 
-@pytest.mark.parametrize("shape,pad", [((9, 11, 64), 0), ((12, 7, 48), 5), ((5, 33, 16), 15),
+@pytest.mark.parametrize("shape,pad", [((9, 11, 64), 0), ((12, 7, 48), 5), ((5, 33, 16), 12),
                                        ((31, 17, 96), 1), ((3, 3, 4), 1), ((64, 64, 128), 0)])
 def test_core_random_vs_oracle(shape, pad):
     L = _lib()
@@ -258,7 +258,7 @@ def test_incident_direct_one_step(str_f, pt0, pt1, is_array):
     from fdtd_accelerate_amd import Fields, GetFields, IncidentDirect, util
     nx, ny, nz = 40, 50, 60
     f = Fields(nx, ny, nz)
-    rng = np.random.default_rng(len(str_f) + sum(pt1))
+    rng = np.random.default_rng(len(str_f) + abs(sum(pt1)) + 7 * abs(sum(pt0)))
     p0 = util.convert_indices(f.ns, pt0)
     p1 = util.convert_indices(f.ns, pt1)
     if is_array:
