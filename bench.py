@@ -6,7 +6,9 @@ non-overlapping dielectric spheres (radius U[8,40], eps_r U[1,12], background 1)
 Gaussian-modulated pulse, Ez probe; synthetic data.  One step = E update + CPML-E +
 source + H update + CPML-H (+ probe gather).
 
-    python bench.py [--steps K] [--warmup W] [--engine fused|kemp] [--n 1024]
+    python bench.py [--steps K] [--warmup W] [--engine kemp|slab|fused] [--n 1024]
+    python bench.py --copy-calibration MiB   (plain device copy of known size, for
+                                              scaling the HBM byte counters)
 
 Prints one JSON line: Gcell-updates/s over K timed steps (inputs resident in HBM),
 the dominant kernel's roofline (algorithmic bytes / its average launch time, from
@@ -25,6 +27,11 @@ sys.path.insert(0, ROOT)
 METRIC = "Gcell-updates/s + % HBM peak, plain vs batched vs adjoint-grad, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel (GB), from two rocprofv3 --pmc runs
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a plain 1 GiB copy gives), keyed by
+# (engine, n, npml); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt.
+MEASURED_TRAFFIC_GB = {("kemp", 1024, 12): 64.69}
+
 
 def parse():
     p = argparse.ArgumentParser()
@@ -34,7 +41,9 @@ def parse():
     p.add_argument("--n", type=int, default=1024)
     p.add_argument("--npml", type=int, default=12)
     p.add_argument("--spheres", type=int, default=200)
-    p.add_argument("--engine", default="kemp", choices=("fused", "kemp"))
+    p.add_argument("--engine", default="kemp", choices=("fused", "kemp", "slab"))
+    p.add_argument("--slab-planes", type=int, default=None)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-n", type=int, default=256)
     return p.parse_args()
@@ -114,6 +123,40 @@ def time_fused(sim, steps, warmup):
     return ms / steps, wall, kern_ms, st.kernel_name, st.bytes_per_cell
 
 
+def time_slab(sim, steps, warmup, slab_planes=None):
+    """Slab engine (E pass + H pass per slab, CPML and sources fused); events bracket each
+    step's launches on the fields' stream, the step is the timed unit."""
+    import torch
+    from fdtd_accelerate_amd.fused import SlabStepper
+    st = SlabStepper(sim, slab_planes=slab_planes)
+    st.run(warmup, None)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(sim.fields.stream)
+    kern_ms = st.run(steps, None, time_kernel=True)
+    t1.record(sim.fields.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1)
+    return ms / steps, wall, kern_ms, f"k_slab_e+k_slab_h (S={st.S})", None
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of ``mib`` MiB, 5 times: the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE counters (known bytes) scale the counters of the step kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "launches": 5}))
+
+
 def cpu_baseline(args):
     """The oracle (float32, OpenMP) on a bounded sample: an n_s^3 Yee+CPML grid with
     the same npml and a random-sphere CE map, timed over a few steps on this host."""
@@ -149,17 +192,25 @@ def cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the slab-decomposition design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the package has no multi-GPU path "
+              "(sharding would split the grid into x slabs with one-plane halo exchange)", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     sim = build_sim(args)
-    runner = time_fused if args.engine == "fused" else time_kemp
-    ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
+    if args.engine == "slab":
+        ms, wall, kern_ms, kname, kbytes = time_slab(sim, args.steps, args.warmup, args.slab_planes)
+    else:
+        runner = time_fused if args.engine == "fused" else time_kemp
+        ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
     n = args.n
     cells = n ** 3
     gcells = cells / (ms * 1e-3) / 1e9
     step_bytes = 60.0 * cells + 32.0 * cpml_face_cells(n, args.npml)
+    if kbytes is None:  # whole-step unit: algorithmic bytes of the step
+        kbytes = step_bytes / cells
     achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
@@ -178,8 +229,9 @@ def main():
                    "engine": args.engine, "cells": cells},
         "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": kbytes, "avg_launch_ms": round(kern_ms, 4)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine, n, args.npml)), "traffic_unit": "GB per launch",
+                     "bytes_per_cell": round(kbytes, 3), "avg_launch_ms": round(kern_ms, 4)},
         "wall_s": round(wall, 3),
         "device": torch.cuda.get_device_name(0),
     }

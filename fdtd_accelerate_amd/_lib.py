@@ -60,6 +60,7 @@ _SIGS = {
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
+    "mifdtd_slab_run": (_I, [ctypes.POINTER(FusedDesc), _I, _I, _P, _P, _P]),
 }
 
 # Symbols declared in include/mifdtd.h that the library must export.

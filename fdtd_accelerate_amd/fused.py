@@ -1,4 +1,5 @@
-"""Host driver of the fused single-pass step (``mifdtd_fused_run``).
+"""Host drivers of the whole-step engines: the fused single-pass step
+(``mifdtd_fused_run``) and the slab-ordered step (``mifdtd_slab_run``).
 
 The fused kernel reads the state of step n from one buffer set and writes step n+1 to
 a second (ping-pong).  ``FusedStepper`` owns the second field set and the second psi-E
@@ -6,6 +7,9 @@ buffer, builds the descriptor from the ``YeeSim`` objects (fields, CPML, sources
 probes), runs K steps in one C call (the C loop launches K kernels back to back) and,
 after an odd K, swaps the buffers so that ``sim.fields`` / ``sim.cpml`` always hold
 the current state.  Sources advance their tstep exactly as the in-place path does.
+
+The slab step updates in place (no second buffer set): ``SlabStepper`` shares the
+descriptor and time-value logic and runs E and H passes slab by slab.
 """
 from __future__ import annotations
 
@@ -15,9 +19,23 @@ import numpy as np
 import torch
 
 from . import _lib
-from .autotune import fused_tiles
+from .autotune import fused_tiles, slab_planes
 
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
+
+
+def _check_whole_step(sim, engine):
+    """Limits shared by the whole-step engines (same descriptor)."""
+    f = sim.fields
+    if f.ch_on:
+        raise ValueError(f"the {engine} engine uses CH = 0.5; use engine='kemp' for CH arrays")
+    cp = sim.cpml
+    if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
+        raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
+    if len(sim.sources) > _lib.MAX_SRC:
+        raise ValueError(f"the {engine} engine supports at most {_lib.MAX_SRC} sources")
+    if len(sim.probe_points) > _lib.MAX_PROBE:
+        raise ValueError(f"the {engine} engine supports at most {_lib.MAX_PROBE} probes")
 
 
 class FusedStepper:
@@ -25,15 +43,8 @@ class FusedStepper:
 
     def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None):
         f = sim.fields
-        if f.ch_on:
-            raise ValueError("the fused engine uses CH = 0.5; use engine='kemp' for CH arrays")
+        _check_whole_step(sim, "fused")
         cp = sim.cpml
-        if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
-            raise ValueError("the fused engine needs CPML on all six faces; use engine='kemp'")
-        if len(sim.sources) > _lib.MAX_SRC:
-            raise ValueError(f"the fused engine supports at most {_lib.MAX_SRC} sources")
-        if len(sim.probe_points) > _lib.MAX_PROBE:
-            raise ValueError(f"the fused engine supports at most {_lib.MAX_PROBE} probes")
         self.sim = sim
         self.alt = [torch.empty_like(t) for t in f.ehs]
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0)
@@ -50,7 +61,7 @@ class FusedStepper:
         d.nx, d.ny, d.nz_pitch, d.pad = f.nx, f.ny, f.nz_pitch, f.pad
         for c in range(6):
             d.e[c] = f.ehs[c].data_ptr()
-            d.e_alt[c] = self.alt[c].data_ptr()
+            d.e_alt[c] = self.alt[c].data_ptr() if self.alt else None
         if f.ce_on:
             for c in range(3):
                 d.ce[c] = f.ces[c].data_ptr()
@@ -59,7 +70,7 @@ class FusedStepper:
             d.npml = cp.npml
             d.bE, d.aE, d.bH, d.aH = (cp.bE.data_ptr(), cp.aE.data_ptr(), cp.bH.data_ptr(), cp.aH.data_ptr())
             d.psiE = cp.psiE_buf.data_ptr()
-            d.psiE_alt = cp.alt_psiE().data_ptr()
+            d.psiE_alt = cp.alt_psiE().data_ptr() if self.alt else None
             d.psiH = cp.psiH_buf.data_ptr()
         d.nsrc = len(sim.sources)
         for n, s in enumerate(sim.sources):
@@ -122,3 +133,35 @@ class FusedStepper:
             evs.append((a, b))
         torch.cuda.synchronize()
         return sum(a.elapsed_time(b) for a, b in evs) / max(nsteps, 1)
+
+
+class SlabStepper(FusedStepper):
+    """In-place slab-ordered whole step: per slab of ``slab_planes`` x planes, an E pass
+    then an H pass, each with CPML, sources and probes fused in."""
+    kernel_name = "k_slab"
+
+    def __init__(self, sim, slab_planes=None):
+        f = sim.fields
+        _check_whole_step(sim, "slab")
+        self.sim = sim
+        self.alt = None
+        self.T = self.CX = 0
+        cp = sim.cpml
+        self.S = slab_planes or slab_planes_for(f, cp)
+        self.bytes_per_cell = 60.0 if f.ce_on else 48.0
+
+    def _call(self, nsteps, buf):
+        sim = self.sim
+        f = sim.fields
+        if buf is None and sim.probe_points:
+            buf = torch.zeros((nsteps, len(sim.probe_points)), dtype=torch.float32, device=f.device)
+        d = self._desc()
+        tv = self._tvals(nsteps)
+        _lib.call("mifdtd_slab_run", ctypes.byref(d), self.S, nsteps, tv.ctypes.data,
+                  None if buf is None else buf.data_ptr(), _lib.stream_handle(f.stream))
+        for s in sim.sources:
+            s.tstep += nsteps
+
+
+def slab_planes_for(f, cp):
+    return slab_planes(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0, f.ce_on)

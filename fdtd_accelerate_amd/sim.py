@@ -5,7 +5,8 @@
 ``KU/cpu/test/020-dielectric-cpml.py:58-73``) and records probe time series on the
 device (one gather per step into a device buffer, copied to the host once at the
 end).  ``engine='kemp'`` runs the in-place per-half-step kernels (the drop-in
-surface); ``engine='fused'`` runs the single-pass fused step (see ``fused.py``).
+surface); ``engine='fused'`` runs the single-pass fused step and ``engine='slab'``
+the in-place slab-ordered whole step (see ``fused.py``).
 """
 from __future__ import annotations
 
@@ -24,8 +25,8 @@ _NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
 class YeeSim:
     def __init__(self, nx, ny, nz, npml=0, coeff_use="e", cpml_faces=("fb", "fb", "fb"),
                  cpml_R=1e-6, cpml_m=4, segment_nbytes=64, engine="kemp", fused_params=None):
-        if engine not in ("kemp", "fused"):
-            raise ValueError(f"engine must be 'kemp' or 'fused' : {engine!r}")
+        if engine not in ("kemp", "fused", "slab"):
+            raise ValueError(f"engine must be 'kemp', 'fused' or 'slab' : {engine!r}")
         self.fields = Fields(nx, ny, nz, coeff_use=coeff_use, segment_nbytes=segment_nbytes)
         self.core = Core(self.fields)
         self.cpml = Cpml(self.fields, npml, cpml_faces, m=cpml_m, R=cpml_R) if npml > 0 else None
@@ -36,6 +37,7 @@ class YeeSim:
         self.engine = engine
         self.tstep = 0
         self._fused = None
+        self._slab = None
         self._fused_params = fused_params
 
     # -------------------------------------------------------------- set-up
@@ -82,6 +84,11 @@ class YeeSim:
             if self._fused is None:
                 self._fused = FusedStepper(self, **(self._fused_params or {}))
             self._fused.run(nsteps, buf)
+        elif self.engine == "slab":
+            from .fused import SlabStepper
+            if self._slab is None:
+                self._slab = SlabStepper(self, **(self._fused_params or {}))
+            self._slab.run(nsteps, buf)
         else:
             for t in range(nsteps):
                 f.update_e()

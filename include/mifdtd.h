@@ -191,6 +191,21 @@ typedef struct {
 int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
                      float *probe_series, int *parity, mifdtd_stream_t stream);
 
+/* ------------------------------------------------------------- slab step ----
+ * The same whole step, in place, as two launches per slab of slab_planes consecutive
+ * x planes: E pass (core E + CPML-E + E sources + E probes) then H pass (core H +
+ * CPML-H + H sources + H probes), slabs in increasing x.  A slab small enough for its
+ * E, H and CE planes to stay in the last-level cache between its two passes makes
+ * HBM see each array about once per step.  Results equal update_e/update_h + the CPML
+ * and source calls above bit for bit (masked lanes included).  Uses e[], ce[], npml,
+ * the CPML tables, psiE and psiH (both updated in place), sources and probes of the
+ * descriptor; e_alt, psiE_alt, rows_per_thread and planes_per_chunk are ignored.
+ * No reference counterpart (it reorders the step loop of
+ * KU/cpu/test/020-dielectric-cpml.py:58-73 by slab).  tvals: HOST (nsteps, nsrc);
+ * probe_series: DEVICE (nsteps, nprobe). */
+int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps,
+                    const float *tvals, float *probe_series, mifdtd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

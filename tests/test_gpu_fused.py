@@ -137,3 +137,94 @@ def test_fused_rejects_unsupported():
     s2 = YeeSim(8, 8, 16, coeff_use="eh", engine="fused")
     with pytest.raises(ValueError):
         FusedStepper(s2)
+
+
+def _slab(ns, npml, S, ce=True):
+    from fdtd_accelerate_amd.sim import YeeSim
+    return YeeSim(*ns, npml=npml, coeff_use="e" if ce else "", engine="slab", fused_params={"slab_planes": S})
+
+
+@pytest.mark.parametrize("ns,npml,S,steps,ce", [
+    ((24, 20, 32), 0, 1, 3, True),
+    ((24, 20, 32), 0, 5, 4, False),
+    ((30, 26, 40), 4, 1, 5, True),
+    ((30, 26, 40), 4, 7, 6, True),
+    ((29, 31, 57), 5, 4, 7, True),
+    ((40, 36, 160), 6, 3, 4, False),
+    ((12, 13, 1000), 4, 2, 3, True),
+    ((26, 70, 300), 5, 26, 2, True),
+    ((33, 64, 128), 8, 64, 3, True),
+])
+def test_slab_matches_inplace(ns, npml, S, steps, ce):
+    """Slab engine == in-place engine byte for byte (signed zeros included): fields, psi
+    and probe series, with a point, a plane (spatial array) and an overwrite source."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    sims = [_sim(ns, npml, "kemp", ce=ce), _slab(ns, npml, S, ce)]
+    nx, ny, nz = ns
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    sv = np.random.default_rng(3).standard_normal((ny - 2, nz)).astype(np.float32)
+    for s in sims:
+        _fill(s, 42)
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_source("hy", (3, 1, 0), (3, ny - 2, nz - 1), tf, sv)
+        s.add_source("ex", (nx - 1, ny - 1, 0), (nx - 1, ny - 1, nz - 1), tf, 0.25, True)
+        s.add_probe("ez", (nx // 2 + 1, ny // 2, nz // 2))
+        s.add_probe("hx", (1, 2, 3))
+        s.add_probe("hz", (nx - 1, ny - 1, nz - 1))
+        s.add_probe("ey", (0, 0, 0))
+    series = [s.run(steps) for s in sims]
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert a.tobytes() == b.tobytes(), (name, np.abs(a - b).max())
+    assert series[0].tobytes() == series[1].tobytes()
+    if npml:
+        for key in sims[0].cpml.psiE:
+            for c in range(2):
+                assert sims[0].cpml.psiE[key][c].cpu().numpy().tobytes() == sims[1].cpml.psiE[key][c].cpu().numpy().tobytes(), ("psiE", key, c)
+                assert sims[0].cpml.psiH[key][c].cpu().numpy().tobytes() == sims[1].cpml.psiH[key][c].cpu().numpy().tobytes(), ("psiH", key, c)
+
+
+def test_slab_config0_vs_oracle():
+    """Config 0 (128^3 vacuum, CPML 10, point source, probe) on the slab engine, 400
+    steps: probe series and fields byte-equal to the float32 oracle."""
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    steps = 400
+    sim = YeeSim(128, 128, 128, npml=10, coeff_use="", engine="slab")
+    sim.add_source("ez", (64, 64, 64), (64, 64, 64), gaussian_pulse())
+    sim.add_probe("ez", (96, 64, 64))
+    series = sim.run(steps)[:, 0]
+    o = O.OracleSim(128, 128, 128, 0, 10, np.float32)
+    o.add_source("ez", (64, 64, 64), (64, 64, 64), O.pulse)
+    o.add_probe("ez", (96, 64, 64))
+    o.run(steps)
+    assert series.tobytes() == np.array(o.series, np.float32)[:, 0].tobytes()
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert sim.field(name).cpu().numpy().tobytes() == o.field(name).tobytes(), name
+
+
+def test_slab_engine_switch():
+    """kemp -> slab -> fused -> slab mid-run equals one continuous in-place run."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, npml = (26, 28, 36), 4
+    tf = gaussian_pulse(4.0, 2.0, 8.0)
+    a = _sim(ns, npml, "kemp")
+    b = _slab(ns, npml, 5)
+    for s in (a, b):
+        _fill(s, 9)
+        s.add_source("ez", (13, 14, 18), (13, 14, 18), tf)
+        s.add_probe("ez", (14, 14, 18))
+    a.run(10)
+    b.engine = "kemp"
+    b.run(2)
+    b.engine = "slab"
+    b.run(3)
+    b.engine = "fused"
+    b._fused_params = {"rows_per_thread": 2, "planes_per_chunk": 6}
+    b.run(3)
+    b.engine = "slab"
+    b.run(2)
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
+    assert same_values(a.series(), b.series())
