@@ -7,6 +7,8 @@ Gaussian-modulated pulse, Ez probe; synthetic data.  One step = E update + CPML-
 source + H update + CPML-H (+ probe gather).
 
     python bench.py [--steps K] [--warmup W] [--engine kemp|slab|fused] [--n 1024]
+    python bench.py --workload s4 [--n 512]   (4th-order symplectic, 100 spheres, CPML 10,
+                                              Courant 0.9 x stability limit)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -44,6 +46,7 @@ def parse():
     p.add_argument("--engine", default="kemp", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    p.add_argument("--workload", default="yee", choices=("yee", "s4"))
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-n", type=int, default=256)
     return p.parse_args()
@@ -143,6 +146,54 @@ def time_slab(sim, steps, warmup, slab_planes=None):
     return ms / steps, wall, kern_ms, f"k_slab_e+k_slab_h (S={st.S})", None
 
 
+def run_s4(args):
+    """4th-order symplectic step: 512^3, 100 spheres (eps_r U[1,12], r U[8,40] scaled to
+    the grid), 10-cell CPML, S = 0.9 S_max, Ez point source at the centre."""
+    import torch
+    from fdtd_accelerate_amd import geometry
+    from fdtd_accelerate_amd.s4 import S4Sim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    n = args.n
+    npml = 10 if args.npml == 12 else args.npml
+    nsph = 100 if args.spheres == 200 else args.spheres
+    rmax = min(40.0, (n - 2 * npml) / 8)
+    spheres = geometry.random_spheres(2024, nsph, (min(8.0, rmax / 2), rmax), (1.0, 12.0), (npml,) * 3, (n - npml,) * 3)
+    sim = S4Sim(n, n, n, npml=npml)
+    sim.set_eps(geometry.eps_map_numpy((n, n, n), spheres))
+    c = n // 2
+    sim.add_source("ez", (c, c, c), (c, c, c), gaussian_pulse())
+    for _ in range(args.warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(sim.stream)
+    for k in range(args.steps):
+        sim.step(events=evs[k])
+    t1.record(sim.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1) / args.steps
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    cells = n ** 3
+    step_bytes = 336.0 * cells + 32.0 * 4 * 6 * npml * n * n
+    achieved = 48.0 * cells / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"4th-order symplectic FP32 {n}^3, {nsph} spheres, CPML {npml}, S=0.9*S_max={sim.S:.4f}, Ez point source",
+                   "cells": cells},
+        "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "k_s4_e", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": 48.0, "avg_launch_ms": round(kern_ms, 4)},
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, 5 times: the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters (known bytes) scale the counters of the step kernels."""
@@ -198,6 +249,11 @@ def main():
     import torch
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "s4":
+        if args.n == 1024:
+            args.n = 512
+        print(json.dumps(run_s4(args)), flush=True)
         return
     sim = build_sim(args)
     if args.engine == "slab":

@@ -61,6 +61,12 @@ _SIGS = {
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
     "mifdtd_slab_run": (_I, [ctypes.POINTER(FusedDesc), _I, _I, _P, _P, _P]),
+    "s4_update_h": (None, [_I] * 3 + [_F] + [_P] * 9),
+    "s4_update_e": (None, [_I] * 3 + [_F] + [_P] * 9),
+    "mifdtd_s4_update_h": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),
+    "mifdtd_s4_update_e": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),
+    "mifdtd_s4_cpml_e": (_I, [_I] * 3 + [ctypes.POINTER(CpmlHalf), _F] + [_P] * 9 + [_F, _P]),
+    "mifdtd_s4_cpml_h": (_I, [_I] * 3 + [ctypes.POINTER(CpmlHalf), _F] + [_P] * 9 + [_F, _P]),
 }
 
 # Symbols declared in include/mifdtd.h that the library must export.

@@ -1,0 +1,457 @@
+// 4th-order symplectic FDTD: the stage updates of S4/Python-C/dielectric.c (H :5-49,
+// E :52-96; the same arithmetic as the CUDA kernels of S4/PyCUDA/100-base.py:10-44) and
+// a CPML for it, on the reference's layout: (nx, ny, nz) Fortran order, x contiguous,
+// no padding.
+//
+// One thread per x-quad (4 consecutive x cells of one (j, k) row); nx must be a
+// multiple of 4.  The x-neighbours i-2 .. i+5 of a quad come from the quad itself plus
+// scalar loads of the neighbouring quads; the y/z neighbours +-1, +-2 rows/planes are
+// quad loads.  Per-cell arithmetic keeps the reference's evaluation order:
+//   H: h -= (cl*ch) * (27*(((a0 - a1) - b0) + b1) - (((a2 - a3) - b2) + b3))
+//   E: e += (dl*ce) * (27*(((a0 - a1) - b0) + b1) - (((a2 - a3) - b2) + b3))
+// so that with -ffp-contract=off the result equals the reference C bit for bit.
+// Guards (which cells each stage writes) follow dielectric.c:30-42,77-89 including its
+// i < nx-1, j < ny-1, k < nz-1 loop bounds.
+
+#include "common.hpp"
+
+namespace mifdtd {
+
+constexpr int S4B = 256;
+
+struct S4Geom {
+    int nx, ny, nz;
+    unsigned Qx;      // quads per x row
+    unsigned nquads;  // Qx * ny * nz
+};
+
+__device__ __forceinline__ Q4 ld4z(const float *p, bool on) { return on ? ld4(p) : splat4(0.f); }
+
+// d/dx(27,-1) of f about the quad: out[l] = 27*((f[i+1]-f[i]) - ...) is assembled by the
+// caller; here the 7 x-neighbours i0-2 .. i0+5 of the quad are gathered.
+struct XWin {
+    float v[8];  // v[m] = f[i0 - 2 + m], m = 0..7  (i0-2 .. i0+5)
+};
+
+__device__ __forceinline__ XWin xwin(const float *f, size_t id, const Q4 &q, bool has_prev, bool has_next)
+{
+    XWin w;
+    w.v[0] = has_prev ? f[id - 2] : 0.f;
+    w.v[1] = has_prev ? f[id - 1] : 0.f;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) w.v[2 + l] = q.v[l];
+    w.v[6] = has_next ? f[id + 4] : 0.f;
+    w.v[7] = has_next ? f[id + 5] : 0.f;
+    return w;
+}
+
+template <bool HC>
+__global__ __launch_bounds__(S4B) void k_s4_h(S4Geom g, float cl,
+                                              const float *__restrict__ ex, const float *__restrict__ ey,
+                                              const float *__restrict__ ez, float *__restrict__ hx,
+                                              float *__restrict__ hy, float *__restrict__ hz,
+                                              const float *__restrict__ cx, const float *__restrict__ cy,
+                                              const float *__restrict__ cz, float ch0)
+{
+    const unsigned f = blockIdx.x * S4B + threadIdx.x;
+    if (f >= g.nquads) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const unsigned row = f / g.Qx;
+    const int i0 = (int)(f - row * g.Qx) * 4;
+    const int k = (int)(row / (unsigned)ny);
+    const int j = (int)row - k * ny;
+    if (j > ny - 2 || k > nz - 2) return;  // loop bounds j < ny-1, k < nz-1
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const size_t id = ((size_t)k * ny + j) * nx + i0;
+    const bool rx = j > 1 && k > 1;  // hx rows
+    const bool ry = k > 1;           // hy rows (i > 1 per lane)
+    const bool rz = j > 1;           // hz rows (i > 1 per lane)
+    const bool hp = i0 > 0, hn = i0 + 4 < nx;
+    const float c27 = 27.f;
+
+    const Q4 ex0 = ld4(ex + id), ey0 = ld4(ey + id), ez0 = ld4(ez + id);
+    // hx: ez(j), ez(j-1), ey(k), ey(k-1), ez(j+1), ez(j-2), ey(k+1), ey(k-2)
+    const Q4 ezjm = ld4z(ez + id - sy, rx), ezjp = ld4z(ez + id + sy, rx), ezj2 = ld4z(ez + id - 2 * sy, rx);
+    const Q4 eykm = ld4z(ey + id - sz, rx), eykp = ld4z(ey + id + sz, rx), eyk2 = ld4z(ey + id - 2 * sz, rx);
+    // hy: ex(k), ex(k-1), ez(i), ez(i-1), ex(k+1), ex(k-2), ez(i+1), ez(i-2)
+    const Q4 exkm = ld4z(ex + id - sz, ry), exkp = ld4z(ex + id + sz, ry), exk2 = ld4z(ex + id - 2 * sz, ry);
+    // hz: ey(i), ey(i-1), ex(j), ex(j-1), ey(i+1), ey(i-2), ex(j+1), ex(j-2)
+    const Q4 exjm = ld4z(ex + id - sy, rz), exjp = ld4z(ex + id + sy, rz), exj2 = ld4z(ex + id - 2 * sy, rz);
+    const XWin wz = xwin(ez, id, ez0, hp && ry, hn && ry);
+    const XWin wy = xwin(ey, id, ey0, hp && rz, hn && rz);
+    Q4 h0 = ld4(hx + id), h1 = ld4(hy + id), h2 = ld4(hz + id);
+    const Q4 c0 = HC ? ld4(cx + id) : splat4(ch0);
+    const Q4 c1 = HC ? ld4(cy + id) : splat4(ch0);
+    const Q4 c2 = HC ? ld4(cz + id) : splat4(ch0);
+
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int i = i0 + l;
+        if (i > nx - 2) continue;  // loop bound i < nx-1
+        if (rx && j < ny - 1 && k < nz - 1)
+            h0.v[l] -= cl * c0.v[l] * (c27 * (ez0.v[l] - ezjm.v[l] - ey0.v[l] + eykm.v[l])
+                                       - (ezjp.v[l] - ezj2.v[l] - eykp.v[l] + eyk2.v[l]));
+        if (i > 1 && ry)
+            h1.v[l] -= cl * c1.v[l] * (c27 * (ex0.v[l] - exkm.v[l] - ez0.v[l] + wz.v[l + 1])
+                                       - (exkp.v[l] - exk2.v[l] - wz.v[l + 3] + wz.v[l]));
+        if (i > 1 && rz)
+            h2.v[l] -= cl * c2.v[l] * (c27 * (ey0.v[l] - wy.v[l + 1] - ex0.v[l] + exjm.v[l])
+                                       - (wy.v[l + 3] - wy.v[l] - exjp.v[l] + exj2.v[l]));
+    }
+    if (rx) st4(hx + id, h0);
+    if (ry) st4(hy + id, h1);
+    if (rz) st4(hz + id, h2);
+}
+
+template <bool HC>
+__global__ __launch_bounds__(S4B) void k_s4_e(S4Geom g, float dl,
+                                              float *__restrict__ ex, float *__restrict__ ey,
+                                              float *__restrict__ ez, const float *__restrict__ hx,
+                                              const float *__restrict__ hy, const float *__restrict__ hz,
+                                              const float *__restrict__ cx, const float *__restrict__ cy,
+                                              const float *__restrict__ cz, float ce0)
+{
+    const unsigned f = blockIdx.x * S4B + threadIdx.x;
+    if (f >= g.nquads) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const unsigned row = f / g.Qx;
+    const int i0 = (int)(f - row * g.Qx) * 4;
+    const int k = (int)(row / (unsigned)ny);
+    const int j = (int)row - k * ny;
+    if (j > ny - 2 || k > nz - 2) return;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const size_t id = ((size_t)k * ny + j) * nx + i0;
+    const bool rx = j > 0 && j < ny - 2 && k > 0 && k < nz - 2;  // ex rows
+    const bool ry = k > 0 && k < nz - 2;                         // ey rows (0 < i < nx-2 per lane)
+    const bool rz = j > 0 && j < ny - 2;                         // ez rows (0 < i < nx-2 per lane)
+    const bool hp = i0 > 0, hn = i0 + 4 < nx;
+    const float c27 = 27.f;
+
+    const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
+    // ex: hz(j+1), hz, hy(k+1), hy, hz(j+2), hz(j-1), hy(k+2), hy(k-1)
+    const Q4 hzjp = ld4z(hz + id + sy, rx), hzj2 = ld4z(hz + id + 2 * sy, rx), hzjm = ld4z(hz + id - sy, rx);
+    const Q4 hykp = ld4z(hy + id + sz, rx), hyk2 = ld4z(hy + id + 2 * sz, rx), hykm = ld4z(hy + id - sz, rx);
+    // ey: hx(k+1), hx, hz(i+1), hz, hx(k+2), hx(k-1), hz(i+2), hz(i-1)
+    const Q4 hxkp = ld4z(hx + id + sz, ry), hxk2 = ld4z(hx + id + 2 * sz, ry), hxkm = ld4z(hx + id - sz, ry);
+    // ez: hy(i+1), hy, hx(j+1), hx, hy(i+2), hy(i-1), hx(j+2), hx(j-1)
+    const Q4 hxjp = ld4z(hx + id + sy, rz), hxj2 = ld4z(hx + id + 2 * sy, rz), hxjm = ld4z(hx + id - sy, rz);
+    const XWin wz = xwin(hz, id, hz0, hp && ry, hn && ry);
+    const XWin wy = xwin(hy, id, hy0, hp && rz, hn && rz);
+    Q4 e0 = ld4(ex + id), e1 = ld4(ey + id), e2 = ld4(ez + id);
+    const Q4 c0 = HC ? ld4(cx + id) : splat4(ce0);
+    const Q4 c1 = HC ? ld4(cy + id) : splat4(ce0);
+    const Q4 c2 = HC ? ld4(cz + id) : splat4(ce0);
+
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int i = i0 + l;
+        if (i > nx - 2) continue;
+        if (rx)
+            e0.v[l] += dl * c0.v[l] * (c27 * (hzjp.v[l] - hz0.v[l] - hykp.v[l] + hy0.v[l])
+                                       - (hzj2.v[l] - hzjm.v[l] - hyk2.v[l] + hykm.v[l]));
+        if (i > 0 && i < nx - 2 && ry)
+            e1.v[l] += dl * c1.v[l] * (c27 * (hxkp.v[l] - hx0.v[l] - wz.v[l + 3] + hz0.v[l])
+                                       - (hxk2.v[l] - hxkm.v[l] - wz.v[l + 4] + wz.v[l + 1]));
+        if (i > 0 && i < nx - 2 && rz)
+            e2.v[l] += dl * c2.v[l] * (c27 * (wy.v[l + 3] - hy0.v[l] - hxjp.v[l] + hx0.v[l])
+                                       - (wy.v[l + 4] - wy.v[l + 1] - hxj2.v[l] + hxjm.v[l]));
+    }
+    if (rx) st4(ex + id, e0);
+    if (ry) st4(ey + id, e1);
+    if (rz) st4(ez + id, e2);
+}
+
+// ------------------------------------------------------------------ CPML
+// Thread per slab cell (both sides of one axis in one grid, side-major).  The face's
+// two tangential components and the driving fields are picked by AXIS; see the header
+// for the sign table.
+__device__ __forceinline__ bool s4_valid_e(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (i > nx - 2 || j > ny - 2 || k > nz - 2) return false;
+    if (c == 0) return j > 0 && j < ny - 2 && k > 0 && k < nz - 2;
+    if (c == 1) return i > 0 && i < nx - 2 && k > 0 && k < nz - 2;
+    return i > 0 && i < nx - 2 && j > 0 && j < ny - 2;
+}
+__device__ __forceinline__ bool s4_valid_h(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (i > nx - 2 || j > ny - 2 || k > nz - 2) return false;
+    if (c == 0) return j > 1 && k > 1;
+    if (c == 1) return i > 1 && k > 1;
+    return i > 1 && j > 1;
+}
+
+struct S4Face {
+    float *psi1[2], *psi2[2];
+};
+
+// Slab cell n of one side -> (i, j, k, psi index), psi layout x fastest:
+// x face (npml, ny, nz), y face (nx, npml, nz), z face (nx, ny, npml).
+template <int AXIS>
+__device__ __forceinline__ void s4_slab_cell(unsigned n, int nx, int ny, int npml, int q, int &i, int &j, int &k)
+{
+    if (AXIS == 0) {
+        const unsigned r = n / (unsigned)npml;  // j + ny*k
+        j = (int)(r % (unsigned)ny);
+        k = (int)(r / (unsigned)ny);
+        i = q;
+    } else if (AXIS == 1) {
+        i = (int)(n % (unsigned)nx);
+        const unsigned r = n / (unsigned)nx;  // p + npml*k
+        k = (int)(r / (unsigned)npml);
+        j = q;
+    } else {
+        i = (int)(n % (unsigned)nx);
+        j = (int)((n / (unsigned)nx) % (unsigned)ny);
+        k = q;
+    }
+}
+
+template <int AXIS, bool HC>
+__global__ __launch_bounds__(S4B) void k_s4_cpml_e(int nx, int ny, int nz, int npml, unsigned per_side, float w,
+                                                   float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
+                                                   const float *__restrict__ hx, const float *__restrict__ hy,
+                                                   const float *__restrict__ hz, const float *__restrict__ cx,
+                                                   const float *__restrict__ cy, const float *__restrict__ cz, float ce0,
+                                                   S4Face fs, const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    unsigned n = blockIdx.x * S4B + threadIdx.x;
+    if (n >= 2 * per_side) return;
+    const int side = n >= per_side;
+    if (side) n -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    const int n_e = AXIS == 0 ? nx : AXIS == 1 ? ny : nz;
+    int p;
+    if (AXIS == 0) p = (int)(n % (unsigned)npml);
+    else if (AXIS == 1) p = (int)((n / (unsigned)nx) % (unsigned)npml);
+    else p = (int)(n / ((unsigned)nx * (unsigned)ny));
+    const int q = side ? n_e - 1 - npml + p : p;
+    const int t = side ? npml + 1 + p : p + 1;
+    int i, j, k;
+    s4_slab_cell<AXIS>(n, nx, ny, npml, q, i, j, k);
+    const size_t id = ((size_t)k * ny + j) * nx + i;
+    const size_t s = AXIS == 0 ? 1 : AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
+    // x: psi1 -> ey from hz (-), psi2 -> ez from hy (+); y: psi1 -> ez from hx (-),
+    // psi2 -> ex from hz (+); z: psi1 -> ex from hy (-), psi2 -> ey from hx (+)
+    float *e1 = AXIS == 0 ? ey : AXIS == 1 ? ez : ex;
+    float *e2 = AXIS == 0 ? ez : AXIS == 1 ? ex : ey;
+    const float *g1 = AXIS == 0 ? hz : AXIS == 1 ? hx : hy;
+    const float *g2 = AXIS == 0 ? hy : AXIS == 1 ? hz : hx;
+    const float *c1 = AXIS == 0 ? cy : AXIS == 1 ? cz : cx;
+    const float *c2 = AXIS == 0 ? cz : AXIS == 1 ? cx : cy;
+    const int comp1 = AXIS == 0 ? 1 : AXIS == 1 ? 2 : 0, comp2 = AXIS == 0 ? 2 : AXIS == 1 ? 0 : 1;
+    const float b = tb[t], a = ta[t];
+    const float c27 = 27.f;
+    if (s4_valid_e(comp1, i, j, k, nx, ny, nz)) {
+        const float d = c27 * (g1[id + s] - g1[id]) - (g1[id + 2 * s] - g1[id - s]);
+        const float tt = b * psi1[n] + a * d;
+        psi1[n] = tt;
+        e1[id] -= w * (HC ? c1[id] : ce0) * tt;
+    }
+    if (s4_valid_e(comp2, i, j, k, nx, ny, nz)) {
+        const float d = c27 * (g2[id + s] - g2[id]) - (g2[id + 2 * s] - g2[id - s]);
+        const float tt = b * psi2[n] + a * d;
+        psi2[n] = tt;
+        e2[id] += w * (HC ? c2[id] : ce0) * tt;
+    }
+}
+
+template <int AXIS, bool HC>
+__global__ __launch_bounds__(S4B) void k_s4_cpml_h(int nx, int ny, int nz, int npml, unsigned per_side, float w,
+                                                   const float *__restrict__ ex, const float *__restrict__ ey,
+                                                   const float *__restrict__ ez, float *__restrict__ hx,
+                                                   float *__restrict__ hy, float *__restrict__ hz,
+                                                   const float *__restrict__ cx, const float *__restrict__ cy,
+                                                   const float *__restrict__ cz, float ch0, S4Face fs,
+                                                   const float *__restrict__ tb, const float *__restrict__ ta)
+{
+    unsigned n = blockIdx.x * S4B + threadIdx.x;
+    if (n >= 2 * per_side) return;
+    const int side = n >= per_side;
+    if (side) n -= per_side;
+    float *psi1 = fs.psi1[side], *psi2 = fs.psi2[side];
+    if (!psi1) return;
+    const int n_e = AXIS == 0 ? nx : AXIS == 1 ? ny : nz;
+    int p;
+    if (AXIS == 0) p = (int)(n % (unsigned)npml);
+    else if (AXIS == 1) p = (int)((n / (unsigned)nx) % (unsigned)npml);
+    else p = (int)(n / ((unsigned)nx * (unsigned)ny));
+    const int q = side ? n_e - npml + p : p;
+    const int t = side ? npml + 2 + p : p + 1;
+    int i, j, k;
+    s4_slab_cell<AXIS>(n, nx, ny, npml, q, i, j, k);
+    const size_t id = ((size_t)k * ny + j) * nx + i;
+    const size_t s = AXIS == 0 ? 1 : AXIS == 1 ? (size_t)nx : (size_t)nx * ny;
+    // x: psi1 -> hy from ez (+), psi2 -> hz from ey (-); y: psi1 -> hz from ex (+),
+    // psi2 -> hx from ez (-); z: psi1 -> hx from ey (+), psi2 -> hy from ex (-)
+    float *h1 = AXIS == 0 ? hy : AXIS == 1 ? hz : hx;
+    float *h2 = AXIS == 0 ? hz : AXIS == 1 ? hx : hy;
+    const float *g1 = AXIS == 0 ? ez : AXIS == 1 ? ex : ey;
+    const float *g2 = AXIS == 0 ? ey : AXIS == 1 ? ez : ex;
+    const float *c1 = AXIS == 0 ? cy : AXIS == 1 ? cz : cx;
+    const float *c2 = AXIS == 0 ? cz : AXIS == 1 ? cx : cy;
+    const int comp1 = AXIS == 0 ? 1 : AXIS == 1 ? 2 : 0, comp2 = AXIS == 0 ? 2 : AXIS == 1 ? 0 : 1;
+    const float b = tb[t], a = ta[t];
+    const float c27 = 27.f;
+    if (s4_valid_h(comp1, i, j, k, nx, ny, nz)) {
+        const float d = c27 * (g1[id] - g1[id - s]) - (g1[id + s] - g1[id - 2 * s]);
+        const float tt = b * psi1[n] + a * d;
+        psi1[n] = tt;
+        h1[id] += w * (HC ? c1[id] : ch0) * tt;
+    }
+    if (s4_valid_h(comp2, i, j, k, nx, ny, nz)) {
+        const float d = c27 * (g2[id] - g2[id - s]) - (g2[id + s] - g2[id - 2 * s]);
+        const float tt = b * psi2[n] + a * d;
+        psi2[n] = tt;
+        h2[id] -= w * (HC ? c2[id] : ch0) * tt;
+    }
+}
+
+static int s4_check(int nx, int ny, int nz)
+{
+    if (nx < 4 || ny < 4 || nz < 4) return fail(MIFDTD_EINVAL, "4th-order grid (%d,%d,%d) smaller than 4", nx, ny, nz);
+    if (nx % 4) return fail(MIFDTD_EALIGN, "4th-order nx %d is not a multiple of 4", nx);
+    if ((unsigned long long)nx * ny * nz / 4 >= (1ull << 31)) return fail(MIFDTD_ERANGE, "4th-order grid too large");
+    return 0;
+}
+
+static int s4_core(bool h_half, int nx, int ny, int nz, float w, float *ex, float *ey, float *ez, float *hx, float *hy,
+                   float *hz, const float *c0, const float *c1, const float *c2, float cconst, hipStream_t st)
+{
+    if (int rc = s4_check(nx, ny, nz)) return rc;
+    const float *ptrs[6] = {ex, ey, ez, hx, hy, hz};
+    for (const float *p : ptrs) {
+        if (!p) return fail(MIFDTD_ENULL, "field pointer is NULL");
+        if (!aligned16(p)) return fail(MIFDTD_EALIGN, "field pointer not 16-byte aligned");
+    }
+    const bool hc = c0 || c1 || c2;
+    if (hc && !(c0 && c1 && c2)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
+    if (hc && (!aligned16(c0) || !aligned16(c1) || !aligned16(c2)))
+        return fail(MIFDTD_EALIGN, "coefficient pointer not 16-byte aligned");
+    S4Geom g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.Qx = (unsigned)(nx / 4);
+    g.nquads = g.Qx * (unsigned)ny * (unsigned)nz;
+    const unsigned grid = (g.nquads + S4B - 1) / S4B;
+    if (h_half) {
+        if (hc) k_s4_h<true><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        else k_s4_h<false><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        MIFDTD_CHECK_LAUNCH("s4 update_h launch");
+    } else {
+        if (hc) k_s4_e<true><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        else k_s4_e<false><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        MIFDTD_CHECK_LAUNCH("s4 update_e launch");
+    }
+    return 0;
+}
+
+template <int AXIS>
+static int s4_cpml_axis(bool h_half, int nx, int ny, int nz, const mifdtd_cpml_half *cp, float w, float *ex, float *ey,
+                        float *ez, float *hx, float *hy, float *hz, const float *c0, const float *c1, const float *c2,
+                        float cconst, hipStream_t st)
+{
+    S4Face fs;
+    bool any = false;
+    for (int sd = 0; sd < 2; sd++) {
+        fs.psi1[sd] = cp->psi1[2 * AXIS + sd];
+        fs.psi2[sd] = cp->psi2[2 * AXIS + sd];
+        if ((fs.psi1[sd] == nullptr) != (fs.psi2[sd] == nullptr))
+            return fail(MIFDTD_ENULL, "psi1/psi2 of face %d must both be set or both NULL", 2 * AXIS + sd);
+        any = any || fs.psi1[sd];
+    }
+    if (!any) return 0;
+    const int npml = cp->npml;
+    const unsigned per_side = AXIS == 0 ? (unsigned)npml * ny * nz
+                            : AXIS == 1 ? (unsigned)nx * npml * nz
+                                        : (unsigned)nx * ny * npml;
+    const unsigned grid = (2 * per_side + S4B - 1) / S4B;
+    const bool hc = c0 != nullptr;
+    if (h_half) {
+        if (hc) k_s4_cpml_h<AXIS, true><<<grid, S4B, 0, st>>>(nx, ny, nz, npml, per_side, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, fs, cp->b, cp->a);
+        else k_s4_cpml_h<AXIS, false><<<grid, S4B, 0, st>>>(nx, ny, nz, npml, per_side, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, fs, cp->b, cp->a);
+    } else {
+        if (hc) k_s4_cpml_e<AXIS, true><<<grid, S4B, 0, st>>>(nx, ny, nz, npml, per_side, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, fs, cp->b, cp->a);
+        else k_s4_cpml_e<AXIS, false><<<grid, S4B, 0, st>>>(nx, ny, nz, npml, per_side, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, fs, cp->b, cp->a);
+    }
+    MIFDTD_CHECK_LAUNCH("s4 cpml launch");
+    return 0;
+}
+
+static int s4_cpml(bool h_half, int nx, int ny, int nz, const mifdtd_cpml_half *cp, float w, float *ex, float *ey,
+                   float *ez, float *hx, float *hy, float *hz, const float *c0, const float *c1, const float *c2,
+                   float cconst, hipStream_t st)
+{
+    if (int rc = s4_check(nx, ny, nz)) return rc;
+    if (!cp) return fail(MIFDTD_ENULL, "CPML descriptor is NULL");
+    const int npml = cp->npml;
+    if (npml < 1 || npml > MIFDTD_MAX_NPML) return fail(MIFDTD_EINVAL, "npml %d outside [1,%d]", npml, MIFDTD_MAX_NPML);
+    if (nx < 2 * npml + 4 || ny < 2 * npml + 4 || nz < 2 * npml + 4)
+        return fail(MIFDTD_EINVAL, "4th-order grid (%d,%d,%d) too small for npml %d", nx, ny, nz, npml);
+    if (!cp->b || !cp->a) return fail(MIFDTD_ENULL, "CPML tables are NULL");
+    if ((c0 || c1 || c2) && !(c0 && c1 && c2)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
+    if (int rc = s4_cpml_axis<0>(h_half, nx, ny, nz, cp, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, st)) return rc;
+    if (int rc = s4_cpml_axis<1>(h_half, nx, ny, nz, cp, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, st)) return rc;
+    return s4_cpml_axis<2>(h_half, nx, ny, nz, cp, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst, st);
+}
+
+}  // namespace mifdtd
+
+using namespace mifdtd;
+
+extern "C" {
+
+void s4_update_h(int nx, int ny, int nz, float cl, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+                 float *cx, float *cy, float *cz)
+{
+    if (!cx || !cy || !cz) {
+        fail(MIFDTD_ENULL, "s4_update_h needs the three coefficient arrays");
+        return;
+    }
+    s4_core(true, nx, ny, nz, cl, ex, ey, ez, hx, hy, hz, cx, cy, cz, 0.f, nullptr);
+}
+
+void s4_update_e(int nx, int ny, int nz, float dl, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+                 float *cx, float *cy, float *cz)
+{
+    if (!cx || !cy || !cz) {
+        fail(MIFDTD_ENULL, "s4_update_e needs the three coefficient arrays");
+        return;
+    }
+    s4_core(false, nx, ny, nz, dl, ex, ey, ez, hx, hy, hz, cx, cy, cz, 0.f, nullptr);
+}
+
+int mifdtd_s4_update_h(int nx, int ny, int nz, float cl, const float *ex, const float *ey, const float *ez, float *hx,
+                       float *hy, float *hz, const float *chx, const float *chy, const float *chz, float ch0,
+                       mifdtd_stream_t stream)
+{
+    return s4_core(true, nx, ny, nz, cl, const_cast<float *>(ex), const_cast<float *>(ey), const_cast<float *>(ez), hx,
+                   hy, hz, chx, chy, chz, ch0, to_stream(stream));
+}
+
+int mifdtd_s4_update_e(int nx, int ny, int nz, float dl, float *ex, float *ey, float *ez, const float *hx,
+                       const float *hy, const float *hz, const float *cex, const float *cey, const float *cez, float ce0,
+                       mifdtd_stream_t stream)
+{
+    return s4_core(false, nx, ny, nz, dl, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
+                   const_cast<float *>(hz), cex, cey, cez, ce0, to_stream(stream));
+}
+
+int mifdtd_s4_cpml_e(int nx, int ny, int nz, const mifdtd_cpml_half *cp, float dl, float *ex, float *ey, float *ez,
+                     const float *hx, const float *hy, const float *hz, const float *cex, const float *cey,
+                     const float *cez, float ce0, mifdtd_stream_t stream)
+{
+    return s4_cpml(false, nx, ny, nz, cp, dl, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
+                   const_cast<float *>(hz), cex, cey, cez, ce0, to_stream(stream));
+}
+
+int mifdtd_s4_cpml_h(int nx, int ny, int nz, const mifdtd_cpml_half *cp, float cl, const float *ex, const float *ey,
+                     const float *ez, float *hx, float *hy, float *hz, const float *chx, const float *chy,
+                     const float *chz, float ch0, mifdtd_stream_t stream)
+{
+    return s4_cpml(true, nx, ny, nz, cp, cl, const_cast<float *>(ex), const_cast<float *>(ey), const_cast<float *>(ez),
+                   hx, hy, hz, chx, chy, chz, ch0, to_stream(stream));
+}
+
+}  // extern "C"

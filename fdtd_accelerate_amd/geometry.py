@@ -74,3 +74,21 @@ def paint_ce_numpy(ns, nz_pitch, spheres, background=1.0):
     out = [torch.zeros((ns[0], ns[1], nz_pitch), dtype=torch.float32) for _ in range(3)]
     paint_ce(out, spheres, ns, background)
     return [t.numpy() for t in out]
+
+
+def eps_map_numpy(ns, spheres, background=1.0):
+    """eps_r sampled at the integer points (i, j, k) of an (nx, ny, nz) grid: the
+    single per-cell permittivity map of the 4th-order path (one coefficient array per
+    field, ``S4/PyCUDA/100-base.py:75,84-89``)."""
+    nx, ny, nz = ns
+    eps = np.full((nx, ny, nz), background, np.float32)
+    for s in spheres:
+        lo = [max(0, int(np.floor(s.center[a] - s.radius))) for a in range(3)]
+        hi = [min(n, int(np.ceil(s.center[a] + s.radius)) + 1) for a, n in enumerate(ns)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        ax = [np.arange(lo[a], hi[a], dtype=np.float64) - s.center[a] for a in range(3)]
+        d2 = ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2
+        box = eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        box[d2 <= s.radius ** 2] = np.float32(s.eps_r)
+    return eps

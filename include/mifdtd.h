@@ -206,6 +206,58 @@ int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
 int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps,
                     const float *tvals, float *probe_series, mifdtd_stream_t stream);
 
+/* ------------------------------------------------------- 4th-order symplectic ----
+ * Layout of the reference's 4th-order path: (nx, ny, nz) Fortran order (x contiguous,
+ * no padding), S4/PyCUDA/100-base.py:74-89; nx must be a multiple of 4.
+ * s4_update_h / s4_update_e replace the Python-C extension functions
+ * S4/Python-C/dielectric.c:5 (update_h, "iiifOOOOOOOOO": nx, ny, nz, cl, Ex..Hz, Cx..Cz)
+ * and :52 (update_e, same with dl), and the CUDA kernels S4/PyCUDA/100-base.py:10,28:
+ *   hx -= cl*chx*(27*(ez-ez[j-1]-ey+ey[k-1]) - (ez[j+1]-ez[j-2]-ey[k+1]+ey[k-2]))   2<=j<=ny-2, 2<=k<=nz-2
+ *   hy -= cl*chy*(27*(ex-ex[k-1]-ez+ez[i-1]) - (ex[k+1]-ex[k-2]-ez[i+1]+ez[i-2]))   2<=i<=nx-2, 2<=k<=nz-2
+ *   hz -= cl*chz*(27*(ey-ey[i-1]-ex+ex[j-1]) - (ey[i+1]-ey[i-2]-ex[j+1]+ex[j-2]))   2<=i<=nx-2, 2<=j<=ny-2
+ *   ex += dl*cex*(27*(hz[j+1]-hz-hy[k+1]+hy) - (hz[j+2]-hz[j-1]-hy[k+2]+hy[k-1]))   1<=j<=ny-3, 1<=k<=nz-3
+ *   ey += dl*cey*(27*(hx[k+1]-hx-hz[i+1]+hz) - (hx[k+2]-hx[k-1]-hz[i+2]+hz[i-1]))   1<=i<=nx-3, 1<=k<=nz-3
+ *   ez += dl*cez*(27*(hy[i+1]-hy-hx[j+1]+hx) - (hy[i+2]-hy[i-1]-hx[j+2]+hx[j-1]))   1<=i<=nx-3, 1<=j<=ny-3
+ * restricted to i < nx-1, j < ny-1, k < nz-1 (the loop bounds of dielectric.c:30-32).
+ * The coefficient arrays hold S/(24*eps_r) (S/24 in vacuum, 100-base.py:75). */
+void s4_update_h(int nx, int ny, int nz, float cl, float *ex, float *ey, float *ez,
+                 float *hx, float *hy, float *hz, float *cx, float *cy, float *cz);
+void s4_update_e(int nx, int ny, int nz, float dl, float *ex, float *ey, float *ez,
+                 float *hx, float *hy, float *hz, float *cx, float *cy, float *cz);
+
+/* Status-returning, stream-taking forms; NULL coefficient arrays mean the constant
+ * ch0 / ce0 (e.g. S/24 for mu_r = 1). */
+int mifdtd_s4_update_h(int nx, int ny, int nz, float cl,
+                       const float *ex, const float *ey, const float *ez,
+                       float *hx, float *hy, float *hz,
+                       const float *chx, const float *chy, const float *chz, float ch0,
+                       mifdtd_stream_t stream);
+int mifdtd_s4_update_e(int nx, int ny, int nz, float dl,
+                       float *ex, float *ey, float *ez,
+                       const float *hx, const float *hy, const float *hz,
+                       const float *cex, const float *cey, const float *cez, float ce0,
+                       mifdtd_stream_t stream);
+
+/* CPML for the 4th-order path (no reference counterpart: the reference's 4th-order
+ * path has no absorbing boundary).  Face, table and sign conventions of the Yee CPML
+ * above (kuFDTD), driven by the 4th-order difference of the stage update:
+ *   E stage: psi = b*psi + a*(27*(H[+1]-H) - (H[+2]-H[-1]));   E_1 -= dl*CE*psi1, E_2 += dl*CE*psi2
+ *   H stage: psi = b*psi + a*(27*(E-E[-1]) - (E[+1]-E[-2]));   H_1 += cl*CH*psi1, H_2 -= cl*CH*psi2
+ * b/a are the tables of that stage's time step (b = exp(-sigma*w*dt), a = b-1 for the
+ * stage weight w = dl or cl; negative weights give b > 1 and the product over a step
+ * is exp(-sigma*dt)).  psi layouts, x fastest: x face (npml, ny, nz), y face
+ * (nx, npml, nz), z face (nx, ny, npml).  Applied only where the stage update writes. */
+int mifdtd_s4_cpml_e(int nx, int ny, int nz, const mifdtd_cpml_half *cp, float dl,
+                     float *ex, float *ey, float *ez,
+                     const float *hx, const float *hy, const float *hz,
+                     const float *cex, const float *cey, const float *cez, float ce0,
+                     mifdtd_stream_t stream);
+int mifdtd_s4_cpml_h(int nx, int ny, int nz, const mifdtd_cpml_half *cp, float cl,
+                     const float *ex, const float *ey, const float *ez,
+                     float *hx, float *hy, float *hz,
+                     const float *chx, const float *chy, const float *chz, float ch0,
+                     mifdtd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

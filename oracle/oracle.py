@@ -54,7 +54,10 @@ def _load():
         getattr(lib, f"ora_src_{sfx}").argtypes = [_P] + [_I] * 9 + [real, _P, real, _I]
         getattr(lib, f"ora_s4_update_h_{sfx}").argtypes = [_I] * 3 + [real] + [_P] * 9
         getattr(lib, f"ora_s4_update_e_{sfx}").argtypes = [_I] * 3 + [real] + [_P] * 9
-        for n in ("update_e", "update_h", "cpml_e", "cpml_h", "src", "s4_update_h", "s4_update_e"):
+        getattr(lib, f"ora_s4_cpml_e_{sfx}").argtypes = [_I] * 6 + [real] + [_P] * 13
+        getattr(lib, f"ora_s4_cpml_h_{sfx}").argtypes = [_I] * 6 + [real] + [_P] * 13
+        for n in ("update_e", "update_h", "cpml_e", "cpml_h", "src", "s4_update_h", "s4_update_e",
+                  "s4_cpml_e", "s4_cpml_h"):
             getattr(lib, f"ora_{n}_{sfx}").restype = None
     return lib
 
@@ -241,6 +244,118 @@ def s4_update_e(ehs, ces, dl):
     assert ex.flags.f_contiguous
     getattr(lib, f"ora_s4_update_e_{_sfx(ex)}")(nx, ny, nz, float(ex.dtype.type(dl)),
                                                  *[_ptr(a) for a in ehs], *[_ptr(c) for c in ces])
+
+
+# 5-stage symplectic integrator of S4/PyCUDA/100-base.py:60-63 (H weights c, E weights
+# d; the step closes with H(c[4]) = H(c1), 100-base.py:138-141).
+S4_C = (0.17399689146541, -0.12038504121430, 0.89277629949778, -0.12038504121430, 0.17399689146541)
+S4_D = (0.62337932451322, -0.12337932451322, -0.12337932451322, 0.62337932451322, 0.0)
+
+
+def s4_psi_shape(axis, nx, ny, nz, npml):
+    """psi layout of the 4th-order CPML (x fastest): F-order shapes."""
+    return [(npml, ny, nz), (nx, npml, nz), (nx, ny, npml)][axis]
+
+
+def s4_cpml_e(axis, side, ehs, ces, psi1, psi2, b, a, w, npml):
+    ex = ehs[0]
+    nx, ny, nz = ex.shape
+    getattr(lib, f"ora_s4_cpml_e_{_sfx(ex)}")(axis, side, nx, ny, nz, npml, float(ex.dtype.type(w)),
+                                               *[_ptr(x) for x in ehs], *[_ptr(c) for c in ces],
+                                               _ptr(psi1), _ptr(psi2), _ptr(b), _ptr(a))
+
+
+def s4_cpml_h(axis, side, ehs, chs, psi1, psi2, b, a, w, npml):
+    ex = ehs[0]
+    nx, ny, nz = ex.shape
+    getattr(lib, f"ora_s4_cpml_h_{_sfx(ex)}")(axis, side, nx, ny, nz, npml, float(ex.dtype.type(w)),
+                                               *[_ptr(x) for x in ehs], *[_ptr(c) for c in chs],
+                                               _ptr(psi1), _ptr(psi2), _ptr(b), _ptr(a))
+
+
+class OracleS4Sim:
+    """CPU oracle of the 4th-order symplectic step (``S4/PyCUDA/100-base.py:124-141``):
+      for s in 0..3: H(c_s) -> CPML-H(c_s) -> E(d_s) -> CPML-E(d_s) -> E sources at t+s/5
+      then H(c_4) -> CPML-H(c_4); probes after the step.
+    Fields are (nx, ny, nz) Fortran-order arrays.  ``ces``/``chs`` hold S/(24 eps_r) /
+    S/24 (``100-base.py:75``); default vacuum.  CPML tables of stage s are those of
+    the time step w_s*S (``cpml_tables(dt=w_s*S)``, w_s in float64).  Sources follow the
+    reference's stage timing: after E stage s the source adds
+    float32(tfunc(tstep + s/5)) * spatial_value (``100-base.py:136``), tstep from 1.
+    """
+
+    def __init__(self, nx, ny, nz, S, npml=0, dtype=np.float32, ces=None, chs=None, cpml_R=1e-6, cpml_m=4):
+        self.nx, self.ny, self.nz, self.S, self.npml = nx, ny, nz, S, npml
+        self.dtype = dtype
+        self.ehs = [np.zeros((nx, ny, nz), dtype, order="F") for _ in range(6)]
+        c0 = dtype(S / 24.0)
+        self.ces = [np.asfortranarray(c, dtype=dtype) for c in ces] if ces is not None else \
+            [np.full((nx, ny, nz), c0, dtype, order="F") for _ in range(3)]
+        self.chs = [np.asfortranarray(c, dtype=dtype) for c in chs] if chs is not None else \
+            [np.full((nx, ny, nz), c0, dtype, order="F") for _ in range(3)]
+        self.cl = np.array(S4_C, dtype)
+        self.dl = np.array(S4_D, dtype)
+        self.sources = []
+        self.probes = []
+        self.series = []
+        self.tstep = 1
+        if npml:
+            self.tabE = [cpml_tables(npml, cpml_m, cpml_R, dt=S4_D[s] * S, dtype=dtype)[:2] for s in range(4)]
+            self.tabH = [cpml_tables(npml, cpml_m, cpml_R, dt=S4_C[s] * S, dtype=dtype)[2:] for s in range(5)]
+            self.psiE, self.psiH = {}, {}
+            for ax in range(3):
+                for sd in range(2):
+                    shp = s4_psi_shape(ax, nx, ny, nz, npml)
+                    self.psiE[ax, sd] = tuple(np.zeros(shp, dtype, order="F") for _ in range(2))
+                    self.psiH[ax, sd] = tuple(np.zeros(shp, dtype, order="F") for _ in range(2))
+
+    def field(self, name):
+        return self.ehs[("ex", "ey", "ez", "hx", "hy", "hz").index(name)]
+
+    def add_source(self, str_f, pt0, pt1, tfunc, svalue=1.0):
+        if str_f[0] != "e":
+            raise ValueError("4th-order sources drive E components")
+        self.sources.append((str_f, tuple(pt0), tuple(pt1), tfunc, svalue))
+
+    def add_probe(self, str_f, pt):
+        self.probes.append((str_f, tuple(pt)))
+
+    def _h(self, s):
+        s4_update_h(self.ehs, self.chs, self.cl[s])
+        if self.npml:
+            b, a = self.tabH[s]
+            for ax in range(3):
+                for sd in range(2):
+                    p1, p2 = self.psiH[ax, sd]
+                    s4_cpml_h(ax, sd, self.ehs, self.chs, p1, p2, b, a, self.cl[s], self.npml)
+
+    def _e(self, s):
+        s4_update_e(self.ehs, self.ces, self.dl[s])
+        if self.npml:
+            b, a = self.tabE[s]
+            for ax in range(3):
+                for sd in range(2):
+                    p1, p2 = self.psiE[ax, sd]
+                    s4_cpml_e(ax, sd, self.ehs, self.ces, p1, p2, b, a, self.dl[s], self.npml)
+        real = self.dtype
+        for str_f, p0, p1, tfunc, sv in self.sources:
+            f = self.field(str_f)
+            box = (slice(p0[0], p1[0] + 1), slice(p0[1], p1[1] + 1), slice(p0[2], p1[2] + 1))
+            tv = real(tfunc(self.tstep + s / 5.0))
+            f[box] += tv * (np.asarray(sv, real) if isinstance(sv, np.ndarray) else real(sv))
+
+    def step(self):
+        for s in range(4):
+            self._h(s)
+            self._e(s)
+        self._h(4)
+        self.tstep += 1
+        if self.probes:
+            self.series.append([self.field(n)[p] for n, p in self.probes])
+
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.step()
 
 
 # --------------------------------------------------------------------- full step

@@ -325,4 +325,130 @@ void SFX(ora_s4_update_e)(int nx, int ny, int nz, REAL dl,
                                              - (hy[id + 2] - hy[id - 1] - hx[id + 2 * sy] + hx[id - sy]));
             }
 }
+/* ---- 4th-order CPML (new: the reference's 4th-order path has no absorbing boundary).
+ * Same face/table/sign conventions as the Yee CPML of KU/cpu/core/cpml_non_kappa.c
+ * (x: psi1 -> ey from hz, psi2 -> ez from hy; y: psi1 -> ez from hx, psi2 -> ex from
+ * hz; z: psi1 -> ex from hy, psi2 -> ey from hx; E front plane p uses table entry p+1,
+ * back plane n-1-npml+p entry npml+1+p; H front p -> p+1, back n-npml+p -> npml+2+p),
+ * with the 4th-order difference of S4/Python-C/dielectric.c:82-89 / :35-42 as driver:
+ *   E half: D = 27*(f[+1]-f[0]) - (f[+2]-f[-1]),  H half: D = 27*(f[0]-f[-1]) - (f[+1]-f[-2])
+ *   psi = b*psi + a*D;   E -/+= w*CE*psi;   H +/-= w*CH*psi
+ * w is the stage weight (d_s for E, c_s for H) and b/a are the tables of that stage's
+ * time step (b = exp(-sigma*w*dt), a = b-1).  Applied only where the 4th-order core
+ * writes (guards of dielectric.c:34-41,81-88 and its i<nx-1, j<ny-1, k<nz-1 loops).
+ * psi layouts (x fastest): x face (npml, ny, nz), y face (nx, npml, nz), z face (nx, ny, npml). */
+static inline int SFX(s4_valid_e)(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (i > nx - 2 || j > ny - 2 || k > nz - 2) return 0;
+    if (c == 0) return j > 0 && j < ny - 2 && k > 0 && k < nz - 2;
+    if (c == 1) return i > 0 && i < nx - 2 && k > 0 && k < nz - 2;
+    return i > 0 && i < nx - 2 && j > 0 && j < ny - 2;
+}
+static inline int SFX(s4_valid_h)(int c, int i, int j, int k, int nx, int ny, int nz)
+{
+    if (i > nx - 2 || j > ny - 2 || k > nz - 2) return 0;
+    if (c == 0) return j > 1 && k > 1;
+    if (c == 1) return i > 1 && k > 1;
+    return i > 1 && j > 1;
+}
+
+void SFX(ora_s4_cpml_e)(int axis, int side, int nx, int ny, int nz, int npml, REAL w,
+                        REAL *ex, REAL *ey, REAL *ez,
+                        const REAL *hx, const REAL *hy, const REAL *hz,
+                        const REAL *cx, const REAL *cy, const REAL *cz,
+                        REAL *psi1, REAL *psi2, const REAL *b, const REAL *a)
+{
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const REAL c27 = (REAL)27;
+    const int n_e = axis == 0 ? nx : axis == 1 ? ny : nz;
+    const size_t s = axis == 0 ? 1 : axis == 1 ? sy : sz;
+    /* tangential pair of this face: (E1, H driving E1, C1, comp1), (E2, ...) */
+    REAL *e1 = axis == 0 ? ey : axis == 1 ? ez : ex;
+    REAL *e2 = axis == 0 ? ez : axis == 1 ? ex : ey;
+    const REAL *g1 = axis == 0 ? hz : axis == 1 ? hx : hy;
+    const REAL *g2 = axis == 0 ? hy : axis == 1 ? hz : hx;
+    const REAL *c1 = axis == 0 ? cy : axis == 1 ? cz : cx;
+    const REAL *c2 = axis == 0 ? cz : axis == 1 ? cx : cy;
+    const int comp1 = axis == 0 ? 1 : axis == 1 ? 2 : 0;
+    const int comp2 = axis == 0 ? 2 : axis == 1 ? 0 : 1;
+    for (int p = 0; p < npml; p++) {
+        const int q = side ? n_e - 1 - npml + p : p;
+        const int t = side ? npml + 1 + p : p + 1;
+        /* (u, v) run over the two tangential axes of the face */
+        const int nu = axis == 0 ? ny : nx, nv = axis == 2 ? ny : nz;
+        for (int v = 0; v < nv; v++)
+            for (int u = 0; u < nu; u++) {
+                    const int ii = axis == 0 ? q : u;
+                    const int jj = axis == 0 ? u : axis == 1 ? q : v;
+                    const int kk = axis == 2 ? q : v;
+                    const size_t id = FX(ii, jj, kk);
+                    const size_t pid = axis == 0 ? (size_t)p + (size_t)npml * ((size_t)jj + (size_t)ny * kk)
+                                     : axis == 1 ? (size_t)ii + (size_t)nx * ((size_t)p + (size_t)npml * kk)
+                                                 : (size_t)ii + (size_t)nx * ((size_t)jj + (size_t)ny * p);
+                    if (SFX(s4_valid_e)(comp1, ii, jj, kk, nx, ny, nz)) {
+                        const REAL d = c27 * (g1[id + s] - g1[id]) - (g1[id + 2 * s] - g1[id - s]);
+                        const REAL tt = b[t] * psi1[pid] + a[t] * d;
+                        psi1[pid] = tt;
+                        e1[id] -= w * c1[id] * tt;
+                    }
+                    if (SFX(s4_valid_e)(comp2, ii, jj, kk, nx, ny, nz)) {
+                        const REAL d = c27 * (g2[id + s] - g2[id]) - (g2[id + 2 * s] - g2[id - s]);
+                        const REAL tt = b[t] * psi2[pid] + a[t] * d;
+                        psi2[pid] = tt;
+                        e2[id] += w * c2[id] * tt;
+                    }
+                }
+    }
+}
+
+void SFX(ora_s4_cpml_h)(int axis, int side, int nx, int ny, int nz, int npml, REAL w,
+                        const REAL *ex, const REAL *ey, const REAL *ez,
+                        REAL *hx, REAL *hy, REAL *hz,
+                        const REAL *cx, const REAL *cy, const REAL *cz,
+                        REAL *psi1, REAL *psi2, const REAL *b, const REAL *a)
+{
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const REAL c27 = (REAL)27;
+    const int n_e = axis == 0 ? nx : axis == 1 ? ny : nz;
+    const size_t s = axis == 0 ? 1 : axis == 1 ? sy : sz;
+    /* x: psi1 -> hy from ez (+), psi2 -> hz from ey (-); y: psi1 -> hz from ex (+),
+     * psi2 -> hx from ez (-); z: psi1 -> hx from ey (+), psi2 -> hy from ex (-) */
+    REAL *h1 = axis == 0 ? hy : axis == 1 ? hz : hx;
+    REAL *h2 = axis == 0 ? hz : axis == 1 ? hx : hy;
+    const REAL *g1 = axis == 0 ? ez : axis == 1 ? ex : ey;
+    const REAL *g2 = axis == 0 ? ey : axis == 1 ? ez : ex;
+    const REAL *c1 = axis == 0 ? cy : axis == 1 ? cz : cx;
+    const REAL *c2 = axis == 0 ? cz : axis == 1 ? cx : cy;
+    const int comp1 = axis == 0 ? 1 : axis == 1 ? 2 : 0;
+    const int comp2 = axis == 0 ? 2 : axis == 1 ? 0 : 1;
+    for (int p = 0; p < npml; p++) {
+        const int q = side ? n_e - npml + p : p;
+        const int t = side ? npml + 2 + p : p + 1;
+        /* (u, v) run over the two tangential axes of the face */
+        const int nu = axis == 0 ? ny : nx, nv = axis == 2 ? ny : nz;
+        for (int v = 0; v < nv; v++)
+            for (int u = 0; u < nu; u++) {
+                    const int ii = axis == 0 ? q : u;
+                    const int jj = axis == 0 ? u : axis == 1 ? q : v;
+                    const int kk = axis == 2 ? q : v;
+                    const size_t id = FX(ii, jj, kk);
+                    const size_t pid = axis == 0 ? (size_t)p + (size_t)npml * ((size_t)jj + (size_t)ny * kk)
+                                     : axis == 1 ? (size_t)ii + (size_t)nx * ((size_t)p + (size_t)npml * kk)
+                                                 : (size_t)ii + (size_t)nx * ((size_t)jj + (size_t)ny * p);
+                    if (SFX(s4_valid_h)(comp1, ii, jj, kk, nx, ny, nz)) {
+                        const REAL d = c27 * (g1[id] - g1[id - s]) - (g1[id + s] - g1[id - 2 * s]);
+                        const REAL tt = b[t] * psi1[pid] + a[t] * d;
+                        psi1[pid] = tt;
+                        h1[id] += w * c1[id] * tt;
+                    }
+                    if (SFX(s4_valid_h)(comp2, ii, jj, kk, nx, ny, nz)) {
+                        const REAL d = c27 * (g2[id] - g2[id - s]) - (g2[id + s] - g2[id - 2 * s]);
+                        const REAL tt = b[t] * psi2[pid] + a[t] * d;
+                        psi2[pid] = tt;
+                        h2[id] -= w * c2[id] * tt;
+                    }
+                }
+    }
+}
 #undef FX
+

@@ -1,0 +1,211 @@
+"""Pins of the 4th-order symplectic oracle (S4/Python-C/dielectric.c, S4/PyCUDA/100-base.py).
+
+No reference test pins this path numerically, so the oracle is pinned by what the
+mathematics fixes: the integrator's order conditions and stability limit, exactness
+of the (27, -1)/24 difference on cubic polynomials (which fails on any index, sign or
+stagger mistake), an independent NumPy slice restatement of dielectric.c, and for the
+new CPML: the sigma = 0 no-op and absorption against a larger domain."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from fdtd_accelerate_amd.s4 import S4_C, S4_D, amplification, courant_limit, stability_theta
+
+
+def test_s4_coefficients_and_order():
+    assert math.isclose(sum(S4_C), 1.0, abs_tol=1e-13)
+    assert math.isclose(sum(S4_D), 1.0, abs_tol=1e-13)
+    assert S4_C == O.S4_C and S4_D == O.S4_D
+    # one-step phase error of the harmonic oscillator: O(theta^5) (4th-order method)
+    errs = []
+    for th in (0.2, 0.1):
+        ev = np.linalg.eigvals(amplification(th))
+        assert np.allclose(np.abs(ev), 1.0, atol=1e-12)  # symplectic: on the unit circle
+        errs.append(abs(abs(np.angle(ev[0])) - th))
+    assert 28.0 < errs[0] / errs[1] < 36.0
+    # stability limit quoted for this scheme: theta 3.0033, S_max 0.7431
+    assert abs(stability_theta() - 3.0033) < 1e-4
+    assert abs(courant_limit() - 0.7431) < 1e-4
+
+
+def _zeros(n, dtype=np.float64):
+    return [np.zeros(n, dtype, order="F") for _ in range(6)]
+
+
+def _cubic(rng):
+    c = rng.uniform(-1, 1, 4)
+    return (lambda x: ((c[0] * x + c[1]) * x + c[2]) * x + c[3],
+            lambda x: (3 * c[0] * x + 2 * c[1]) * x + c[2])
+
+
+def test_s4_e_stage_exact_on_cubics():
+    """E stage with dl = 1, CE = 1/24 adds exactly the 4th-order curl: for H fields that
+    are cubic along the differentiated axis the increment is the analytic derivative at
+    the E point (half a cell forward)."""
+    nx, ny, nz = 12, 11, 10
+    rng = np.random.default_rng(5)
+    ehs = _zeros((nx, ny, nz))
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    (p_hz_y, d_hz_y), (p_hy_z, d_hy_z) = _cubic(rng), _cubic(rng)
+    (p_hx_z, d_hx_z), (p_hz_x, d_hz_x) = _cubic(rng), _cubic(rng)
+    (p_hy_x, d_hy_x), (p_hx_y, d_hx_y) = _cubic(rng), _cubic(rng)
+    ehs[3][:] = p_hx_z(K) + p_hx_y(J)
+    ehs[4][:] = p_hy_z(K) + p_hy_x(I)
+    ehs[5][:] = p_hz_y(J) + p_hz_x(I)
+    ces = [np.full((nx, ny, nz), 1 / 24, order="F") for _ in range(3)]
+    O.s4_update_e(ehs, ces, 1.0)
+    ex, ey, ez = ehs[:3]
+    sl = np.s_[1:nx - 2, 1:ny - 2, 1:nz - 2]
+    np.testing.assert_allclose(ex[sl], (d_hz_y(J + 0.5) - d_hy_z(K + 0.5))[sl], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(ey[sl], (d_hx_z(K + 0.5) - d_hz_x(I + 0.5))[sl], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(ez[sl], (d_hy_x(I + 0.5) - d_hx_y(J + 0.5))[sl], rtol=0, atol=1e-11)
+    # guard cells are untouched
+    assert not ex[:, 0, :].any() and not ex[:, ny - 2:, :].any() and not ex[nx - 1, :, :].any()
+    assert not ey[0, :, :].any() and not ey[nx - 2:, :, :].any() and not ey[:, ny - 1, :].any()
+
+
+def test_s4_h_stage_exact_on_cubics():
+    """H stage with cl = 1, CH = 1/24: h -= 4th-order curl of E, the derivative taken
+    half a cell backward."""
+    nx, ny, nz = 12, 11, 10
+    rng = np.random.default_rng(6)
+    ehs = _zeros((nx, ny, nz))
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    (p_ez_y, d_ez_y), (p_ey_z, d_ey_z) = _cubic(rng), _cubic(rng)
+    (p_ex_z, d_ex_z), (p_ez_x, d_ez_x) = _cubic(rng), _cubic(rng)
+    (p_ey_x, d_ey_x), (p_ex_y, d_ex_y) = _cubic(rng), _cubic(rng)
+    ehs[0][:] = p_ex_z(K) + p_ex_y(J)
+    ehs[1][:] = p_ey_z(K) + p_ey_x(I)
+    ehs[2][:] = p_ez_y(J) + p_ez_x(I)
+    chs = [np.full((nx, ny, nz), 1 / 24, order="F") for _ in range(3)]
+    O.s4_update_h(ehs, chs, 1.0)
+    hx, hy, hz = ehs[3:]
+    sl = np.s_[2:nx - 1, 2:ny - 1, 2:nz - 1]
+    np.testing.assert_allclose(-hx[sl], (d_ez_y(J - 0.5) - d_ey_z(K - 0.5))[sl], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(-hy[sl], (d_ex_z(K - 0.5) - d_ez_x(I - 0.5))[sl], rtol=0, atol=1e-11)
+    np.testing.assert_allclose(-hz[sl], (d_ey_x(I - 0.5) - d_ex_y(J - 0.5))[sl], rtol=0, atol=1e-11)
+    assert not hx[:, :2, :].any() and not hx[:, ny - 1, :].any() and not hx[nx - 1, :, :].any()
+
+
+def _numpy_s4_h(ehs, chs, cl):
+    """Slice form of S4/Python-C/dielectric.c:30-42 (x fastest index first)."""
+    ex, ey, ez, hx, hy, hz = ehs
+    cx, cy, cz = chs
+    nx, ny, nz = ex.shape
+    f = ex.dtype.type
+    c27, cl = f(27), f(cl)
+    s = np.s_[0:nx - 1, 2:ny - 1, 2:nz - 1]
+    hx[s] -= cl * cx[s] * (c27 * (ez[s] - ez[0:nx - 1, 1:ny - 2, 2:nz - 1] - ey[s] + ey[0:nx - 1, 2:ny - 1, 1:nz - 2])
+                           - (ez[0:nx - 1, 3:ny, 2:nz - 1] - ez[0:nx - 1, 0:ny - 3, 2:nz - 1]
+                              - ey[0:nx - 1, 2:ny - 1, 3:nz] + ey[0:nx - 1, 2:ny - 1, 0:nz - 3]))
+    s = np.s_[2:nx - 1, 0:ny - 1, 2:nz - 1]
+    hy[s] -= cl * cy[s] * (c27 * (ex[s] - ex[2:nx - 1, 0:ny - 1, 1:nz - 2] - ez[s] + ez[1:nx - 2, 0:ny - 1, 2:nz - 1])
+                           - (ex[2:nx - 1, 0:ny - 1, 3:nz] - ex[2:nx - 1, 0:ny - 1, 0:nz - 3]
+                              - ez[3:nx, 0:ny - 1, 2:nz - 1] + ez[0:nx - 3, 0:ny - 1, 2:nz - 1]))
+    s = np.s_[2:nx - 1, 2:ny - 1, 0:nz - 1]
+    hz[s] -= cl * cz[s] * (c27 * (ey[s] - ey[1:nx - 2, 2:ny - 1, 0:nz - 1] - ex[s] + ex[2:nx - 1, 1:ny - 2, 0:nz - 1])
+                           - (ey[3:nx, 2:ny - 1, 0:nz - 1] - ey[0:nx - 3, 2:ny - 1, 0:nz - 1]
+                              - ex[2:nx - 1, 3:ny, 0:nz - 1] + ex[2:nx - 1, 0:ny - 3, 0:nz - 1]))
+
+
+def _numpy_s4_e(ehs, ces, dl):
+    """Slice form of S4/Python-C/dielectric.c:77-89."""
+    ex, ey, ez, hx, hy, hz = ehs
+    cx, cy, cz = ces
+    nx, ny, nz = ex.shape
+    f = ex.dtype.type
+    c27, dl = f(27), f(dl)
+    s = np.s_[0:nx - 1, 1:ny - 2, 1:nz - 2]
+    ex[s] += dl * cx[s] * (c27 * (hz[0:nx - 1, 2:ny - 1, 1:nz - 2] - hz[s] - hy[0:nx - 1, 1:ny - 2, 2:nz - 1] + hy[s])
+                           - (hz[0:nx - 1, 3:ny, 1:nz - 2] - hz[0:nx - 1, 0:ny - 3, 1:nz - 2]
+                              - hy[0:nx - 1, 1:ny - 2, 3:nz] + hy[0:nx - 1, 1:ny - 2, 0:nz - 3]))
+    s = np.s_[1:nx - 2, 0:ny - 1, 1:nz - 2]
+    ey[s] += dl * cy[s] * (c27 * (hx[1:nx - 2, 0:ny - 1, 2:nz - 1] - hx[s] - hz[2:nx - 1, 0:ny - 1, 1:nz - 2] + hz[s])
+                           - (hx[1:nx - 2, 0:ny - 1, 3:nz] - hx[1:nx - 2, 0:ny - 1, 0:nz - 3]
+                              - hz[3:nx, 0:ny - 1, 1:nz - 2] + hz[0:nx - 3, 0:ny - 1, 1:nz - 2]))
+    s = np.s_[1:nx - 2, 1:ny - 2, 0:nz - 1]
+    ez[s] += dl * cz[s] * (c27 * (hy[2:nx - 1, 1:ny - 2, 0:nz - 1] - hy[s] - hx[1:nx - 2, 2:ny - 1, 0:nz - 1] + hx[s])
+                           - (hy[3:nx, 1:ny - 2, 0:nz - 1] - hy[0:nx - 3, 1:ny - 2, 0:nz - 1]
+                              - hx[1:nx - 2, 3:ny, 0:nz - 1] + hx[1:nx - 2, 0:ny - 3, 0:nz - 1]))
+
+
+@pytest.mark.parametrize("shape", [(8, 9, 10), (13, 7, 6), (16, 16, 16)])
+def test_s4_c_oracle_equals_numpy_restatement(shape):
+    rng = np.random.default_rng(sum(shape))
+    a = [np.asfortranarray(rng.standard_normal(shape).astype(np.float32)) for _ in range(6)]
+    c = [np.asfortranarray(rng.uniform(0.01, 0.04, shape).astype(np.float32)) for _ in range(6)]
+    b = [x.copy(order="F") for x in a]
+    for s in range(4):
+        O.s4_update_h(a, c[3:], O.S4_C[s])
+        _numpy_s4_h(b, c[3:], np.float32(O.S4_C[s]))
+        O.s4_update_e(a, c[:3], O.S4_D[s])
+        _numpy_s4_e(b, c[:3], np.float32(O.S4_D[s]))
+    for x, y in zip(a, b):
+        assert x.tobytes() == y.tobytes()
+
+
+def _energy(o):
+    return sum(float(np.sum(f.astype(np.float64) ** 2)) for f in o.ehs)
+
+
+def test_s4_stability_edge():
+    """Random interior fields (the never-updated guard cells zero, i.e. a closed box) on a
+    24^3 grid: energy conserved up to S_max, exponential growth just above it."""
+    smax = courant_limit()
+    rng = np.random.default_rng(11)
+    init = [np.pad(rng.standard_normal((18, 18, 18)), 3) for _ in range(6)]
+    out = {}
+    for f in (0.98, 1.03):
+        o = O.OracleS4Sim(24, 24, 24, f * smax, dtype=np.float64)
+        for a, b in zip(o.ehs, init):
+            a[:] = b
+        o.run(5)
+        e0 = _energy(o)
+        o.run(150)
+        out[f] = _energy(o) / e0
+    assert 0.9 < out[0.98] < 1.1, out
+    assert out[1.03] > 1e6, out
+
+
+def test_s4_cpml_sigma_zero_is_noop():
+    """R = 1 gives sigma = 0 (b = 1, a = 0): psi stays 0 and the fields equal a run
+    without CPML."""
+    rng = np.random.default_rng(2)
+    init = [rng.standard_normal((20, 20, 20)).astype(np.float32) for _ in range(6)]
+    sims = [O.OracleS4Sim(20, 20, 20, 0.6, npml=0), O.OracleS4Sim(20, 20, 20, 0.6, npml=4, cpml_R=1.0)]
+    for o in sims:
+        for a, b in zip(o.ehs, init):
+            a[:] = b
+        o.run(4)
+    for a, b in zip(sims[0].ehs, sims[1].ehs):
+        np.testing.assert_array_equal(a, b)
+    for p in list(sims[1].psiE.values()) + list(sims[1].psiH.values()):
+        assert not p[0].any() and not p[1].any()
+
+
+def test_s4_cpml_absorbs():
+    """A pulse radiated in a 40^3 box with an 8-cell CPML matches the same source in an
+    88^3 box (whose walls are too far to be seen in the window) to within 3% of the
+    direct-field peak at a probe 4 cells from the absorbing layer."""
+    S = 0.9 * courant_limit()
+    pulse = lambda t: O.pulse(t, 24.0, 8.0, 12.0)
+    n, big, npml = 40, 88, 8
+    off = (big - n) // 2
+    c = n // 2
+    small = O.OracleS4Sim(n, n, n, S, npml=npml)
+    large = O.OracleS4Sim(big, big, big, S, npml=0)
+    small.add_source("ez", (c, c, c), (c, c, c), pulse)
+    large.add_source("ez", (c + off,) * 3, (c + off,) * 3, pulse)
+    p = n - npml - 4
+    small.add_probe("ez", (p, c, c))
+    large.add_probe("ez", (p + off, c + off, c + off))
+    steps = 90
+    small.run(steps)
+    large.run(steps)
+    a = np.array(small.series, np.float64)[:, 0]
+    b = np.array(large.series, np.float64)[:, 0]
+    peak = np.abs(b).max()
+    assert peak > 0
+    assert np.abs(a - b).max() < 0.03 * peak, (np.abs(a - b).max(), peak)
