@@ -9,6 +9,9 @@ source + H update + CPML-H (+ probe gather).
     python bench.py [--steps K] [--warmup W] [--engine kemp|slab|fused] [--n 1024]
     python bench.py --workload s4 [--n 512]   (4th-order symplectic, 100 spheres, CPML 10,
                                               Courant 0.9 x stability limit)
+    python bench.py --workload batch [--steps 300] [--tune]   (64 uneven Yee+CPML 8
+                       instances in one batched step loop vs the same instances run
+                       one after another)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -46,7 +49,10 @@ def parse():
     p.add_argument("--engine", default="kemp", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch"))
+    p.add_argument("--instances", type=int, default=64)
+    p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
+    p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-n", type=int, default=256)
     return p.parse_args()
@@ -194,6 +200,72 @@ def run_s4(args):
     }
 
 
+def run_batch(args):
+    """Batched plain Yee: 64 cubes with edges from {64,96,128,160,192}, own sphere maps,
+    CPML 8, seeded source; batched step loop vs sequential per-instance runs."""
+    import torch
+    from fdtd_accelerate_amd import batch as B
+    from fdtd_accelerate_amd.fused import SlabStepper
+    sims = B.make_config3(seed=3, count=args.instances, sizes=tuple(int(v) for v in args.sizes.split(",")))
+    cells = sum(s.fields.nx * s.fields.ny * s.fields.nz for s in sims)
+    bt = B.BatchedYee(sims)
+    tuned = None
+    if args.tune:
+        tuned = B.tune(bt)
+    bt.time_steps(max(args.warmup, 1))
+    wall0 = time.perf_counter()
+    ms = bt.time_steps(args.steps)
+    wall = time.perf_counter() - wall0
+    # sequential: each instance alone, (a) slab engine with its C step loop (2 launches
+    # per step), (b) the in-place engine driven step by step from Python
+    st = torch.cuda.current_stream()
+    steppers = [SlabStepper(s) for s in sims]
+    for sp in steppers:
+        sp.run(1, None)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for sp in steppers:
+        sp.run(args.steps, None)
+    b.record(st)
+    torch.cuda.synchronize()
+    seq_slab_ms = a.elapsed_time(b) / args.steps
+    ksteps = min(args.steps, 50)
+    for s in sims:
+        s.fields.update_e()
+        s.fields.update_h()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a.record(st)
+    for s in sims:
+        for _ in range(ksteps):
+            s.fields.update_e()
+            s.fields.update_h()
+    b.record(st)
+    torch.cuda.synchronize()
+    seq_kemp_ms = max(a.elapsed_time(b), (time.perf_counter() - t0) * 1e3) / ksteps
+    seq_best = min(seq_slab_ms, seq_kemp_ms)
+    sizes = sorted(s.fields.nx for s in sims)
+    return {
+        "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batched plain Yee FP32: {len(sims)} cubes, edges {dict((n, sizes.count(n)) for n in sorted(set(sizes)))}, "
+                               f"10-40 spheres each, CPML 8, seeded Ez source", "cells": cells,
+                   "use_graph": bt.use_graph, "remap_rows": bt.remap_rows},
+        "batched_vs_sequential": round(seq_best / ms, 3),
+        "sequential": {"slab_engine_ms_per_step": round(seq_slab_ms, 4),
+                       "inplace_engine_ms_per_step": round(seq_kemp_ms, 4),
+                       "inplace_steps_timed": ksteps},
+        "tuning": None if tuned is None else {"best": list(tuned[0]), "ms": {f"{k[0]}/{k[1]}": round(v, 4) for k, v in tuned[1].items()}},
+        "roofline": {"bound": "hbm", "kernel": "k_batch (E + H passes)", "achieved": round((84.0 * cells) / (ms * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round((84.0 * cells) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_cell": 84.0, "avg_launch_ms": round(ms / 2, 4)},
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, 5 times: the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters (known bytes) scale the counters of the step kernels."""
@@ -249,6 +321,11 @@ def main():
     import torch
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "batch":
+        if args.steps == 200:
+            args.steps = 300
+        print(json.dumps(run_batch(args)), flush=True)
         return
     if args.workload == "s4":
         if args.n == 1024:

@@ -40,6 +40,17 @@ class FusedDesc(ctypes.Structure):
     ]
 
 
+class InstanceDesc(ctypes.Structure):
+    """``mifdtd_instance`` (include/mifdtd.h)."""
+    _fields_ = [
+        ("nx", _I), ("ny", _I), ("nz_pitch", _I), ("pad", _I),
+        ("e", _P * 6), ("ce", _P * 3), ("npml", _I), ("psiE", _P), ("psiH", _P),
+        ("nsrc", _I), ("src_comp", _I * MAX_SRC), ("src_box", (_I * 6) * MAX_SRC),
+        ("src_sv", _P * MAX_SRC), ("src_svs", _F * MAX_SRC), ("src_overwrite", _I * MAX_SRC),
+        ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "mifdtd_last_error": (ctypes.c_char_p, []),
@@ -61,6 +72,10 @@ _SIGS = {
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
     "mifdtd_slab_run": (_I, [ctypes.POINTER(FusedDesc), _I, _I, _P, _P, _P]),
+    "mifdtd_batch_create": (_I, [ctypes.POINTER(InstanceDesc), _I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "mifdtd_batch_sizes": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint)]),
+    "mifdtd_batch_run": (_I, [_P, _I, _P, _P, _I, _I, _P]),
+    "mifdtd_batch_destroy": (_I, [_P]),
     "s4_update_h": (None, [_I] * 3 + [_F] + [_P] * 9),
     "s4_update_e": (None, [_I] * 3 + [_F] + [_P] * 9),
     "mifdtd_s4_update_h": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),

@@ -1,0 +1,274 @@
+// Batched whole-step engine: many independent Yee + CPML simulations of uneven sizes
+// advanced together, two launches per step for the whole batch (E pass of every
+// instance, then H pass of every instance).
+//
+// A launch's blocks are the concatenation of each instance's blocks (instance order);
+// block b finds its instance by binary search in the block-prefix table and runs the
+// same per-quad E/H update as the slab engine (core + CPML + sources + probes), so small
+// instances no longer leave the GPU idle between launches and a batch of N instances
+// costs 2 launches per step instead of 2N.  Per-instance descriptors live in device
+// memory (built once by mifdtd_batch_create); the per-step source values and probe
+// outputs are rows of device arrays indexed by the step.  Optionally the launches of a
+// run are captured once into a hipGraph and replayed as one graph launch.
+
+#include "step_quad.hpp"
+
+#include <vector>
+
+namespace mifdtd {
+
+struct BInst {
+    SGeom g;          // whole instance: i0 = 0, nquads = nx*ny*nz_pitch/4
+    SFields F;
+    SCpml cp;
+    SSrc se, sh;      // E / H sources with tval unset
+    int tv_e[kSMaxSrc], tv_h[kSMaxSrc];  // column of each source's value in a tvals row
+    SProbe pe, ph;    // probes with out unset; slot = column in a probe row
+};
+
+// Renumber instance-local block b so each XCD takes R consecutive blocks in turn.
+__device__ __forceinline__ unsigned xcd_rows(unsigned b, unsigned nb, unsigned R)
+{
+    if (R == 0) return b;
+    const unsigned span = 8u * R;
+    const unsigned full = nb / span * span;
+    if (b >= full) return b;
+    const unsigned x = b & 7u, sl = b >> 3;
+    return (sl / R) * span + x * R + sl % R;
+}
+
+template <bool HC, bool PML, bool EHALF>
+__global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, const unsigned *__restrict__ first_block,
+                                             int ninst, const float *__restrict__ tv_row, float *probe_row,
+                                             unsigned remap_rows)
+{
+    const unsigned b = blockIdx.x;
+    int lo = 0, hi = ninst;  // first_block[lo] <= b < first_block[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (first_block[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const BInst &I = inst[lo];
+    const unsigned nb = first_block[lo + 1] - first_block[lo];
+    const unsigned f = xcd_rows(b - first_block[lo], nb, remap_rows) * SB + threadIdx.x;
+    if (f >= I.g.nquads) return;
+    SSrc src = EHALF ? I.se : I.sh;
+#pragma unroll
+    for (int m = 0; m < kSMaxSrc; ++m)
+        if (m < src.n) src.tval[m] = tv_row[EHALF ? I.tv_e[m] : I.tv_h[m]];
+    SProbe prb = EHALF ? I.pe : I.ph;
+    prb.out = probe_row;
+    if (EHALF)
+        e_quad<HC, PML>(I.g, f, I.F.e[0], I.F.e[1], I.F.e[2], I.F.e[3], I.F.e[4], I.F.e[5], I.F.ce[0], I.F.ce[1],
+                        I.F.ce[2], I.cp, src, prb);
+    else
+        h_quad<PML>(I.g, f, I.F.e[0], I.F.e[1], I.F.e[2], I.F.e[3], I.F.e[4], I.F.e[5], I.cp, src, prb);
+}
+
+}  // namespace mifdtd
+
+using namespace mifdtd;
+
+struct mifdtd_batch {
+    int ninst = 0;
+    bool hc = false, pml = false;
+    unsigned total_blocks = 0;
+    int total_src = 0, total_probe = 0;
+    BInst *d_inst = nullptr;
+    unsigned *d_first = nullptr;
+};
+
+extern "C" {
+
+int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE, const float *aE, const float *bH,
+                        const float *aH, mifdtd_batch **out)
+{
+    if (!out) return fail(MIFDTD_ENULL, "out is NULL");
+    *out = nullptr;
+    if (!inst || ninst < 1) return fail(MIFDTD_EINVAL, "need at least one instance");
+    std::vector<BInst> h(ninst);
+    std::vector<unsigned> first(ninst + 1, 0);
+    const bool hc = inst[0].ce[0] != nullptr;
+    bool pml = false;
+    int tsrc = 0, tprobe = 0;
+    unsigned long long blocks = 0;
+    for (int n = 0; n < ninst; n++) {
+        const mifdtd_instance &d = inst[n];
+        const int nx = d.nx, ny = d.ny, nzp = d.nz_pitch, pad = d.pad;
+        if (int rc = check_grid(nx, ny, nzp, pad)) return fail(rc, "instance %d: %s", n, mifdtd_last_error());
+        for (int c = 0; c < 6; c++)
+            if (!d.e[c] || !aligned16(d.e[c])) return fail(MIFDTD_EALIGN, "instance %d: field %d NULL or misaligned", n, c);
+        if ((d.ce[0] != nullptr) != hc || (hc && (!d.ce[1] || !d.ce[2])))
+            return fail(MIFDTD_ENULL, "instance %d: CE arrays must be set on every instance or on none", n);
+        const int nz = nzp - pad;
+        BInst &B = h[n];
+        B = BInst{};
+        B.g.nx = nx;
+        B.g.ny = ny;
+        B.g.nzp = nzp;
+        B.g.nz = nz;
+        B.g.Q = (unsigned)(nzp / 4);
+        B.g.i0 = 0;
+        B.g.nquads = (unsigned)((unsigned long long)nx * ny * (nzp / 4));
+        B.g.remap = 0;
+        for (int c = 0; c < 6; c++) B.F.e[c] = d.e[c];
+        for (int c = 0; c < 3; c++) B.F.ce[c] = d.ce[c];
+        if (d.npml > 0) {
+            const int np_ = d.npml;
+            if (np_ > MIFDTD_MAX_NPML) return fail(MIFDTD_EINVAL, "instance %d: npml %d too large", n, np_);
+            if (nx < 2 * np_ + 2 || ny < 2 * np_ + 2 || nz < 2 * np_ + 2)
+                return fail(MIFDTD_EINVAL, "instance %d: grid too small for npml %d", n, np_);
+            if (!bE || !aE || !bH || !aH) return fail(MIFDTD_ENULL, "CPML tables are NULL");
+            if (!d.psiE || !d.psiH || !aligned16(d.psiE) || !aligned16(d.psiH))
+                return fail(MIFDTD_EALIGN, "instance %d: psi buffers NULL or misaligned", n);
+            pml = true;
+            B.cp.npml = np_;
+            B.cp.bE = bE;
+            B.cp.aE = aE;
+            B.cp.bH = bH;
+            B.cp.aH = aH;
+            B.cp.pe = d.psiE;
+            B.cp.ph = d.psiH;
+            B.cp.sX = (size_t)np_ * ny * nzp;
+            B.cp.sY = (size_t)nx * np_ * nzp;
+            B.cp.sZ = (size_t)nx * ny * np_;
+        }
+        if (d.nsrc < 0 || d.nsrc > MIFDTD_FUSED_MAX_SRC) return fail(MIFDTD_EINVAL, "instance %d: nsrc", n);
+        for (int s = 0; s < d.nsrc; s++) {
+            const int *bx = d.src_box[s];
+            const int c = d.src_comp[s];
+            if (c < 0 || c > 5 || bx[0] < 0 || bx[2] < 0 || bx[4] < 0 || bx[1] >= nx || bx[3] >= ny || bx[5] >= nzp ||
+                bx[0] > bx[1] || bx[2] > bx[3] || bx[4] > bx[5])
+                return fail(MIFDTD_EINVAL, "instance %d: source %d outside the grid", n, s);
+            SSrc &X = c < 3 ? B.se : B.sh;
+            int *tv = c < 3 ? B.tv_e : B.tv_h;
+            const int m = X.n++;
+            X.comp[m] = c;
+            X.i0[m] = bx[0];
+            X.i1[m] = bx[1];
+            X.j0[m] = bx[2];
+            X.j1[m] = bx[3];
+            X.k0[m] = bx[4];
+            X.k1[m] = bx[5];
+            X.sv[m] = d.src_sv[s];
+            X.svs[m] = d.src_svs[s];
+            X.ow[m] = d.src_overwrite[s];
+            tv[m] = tsrc + s;
+        }
+        tsrc += d.nsrc;
+        if (d.nprobe < 0 || d.nprobe > MIFDTD_FUSED_MAX_PROBE) return fail(MIFDTD_EINVAL, "instance %d: nprobe", n);
+        for (int q = 0; q < d.nprobe; q++) {
+            const int *p = d.probe_ijk[q];
+            const int c = d.probe_comp[q];
+            if (c < 0 || c > 5 || p[0] < 0 || p[0] >= nx || p[1] < 0 || p[1] >= ny || p[2] < 0 || p[2] >= nzp)
+                return fail(MIFDTD_EINVAL, "instance %d: probe %d outside the grid", n, q);
+            SProbe &P = c < 3 ? B.pe : B.ph;
+            const int m = P.n++;
+            P.comp[m] = c;
+            P.i[m] = p[0];
+            P.j[m] = p[1];
+            P.k[m] = p[2];
+            P.slot[m] = tprobe + q;
+        }
+        tprobe += d.nprobe;
+        first[n] = (unsigned)blocks;
+        blocks += (B.g.nquads + SB - 1) / SB;
+        if (blocks >= (1ull << 31)) return fail(MIFDTD_ERANGE, "batch exceeds 2^31 blocks");
+    }
+    first[ninst] = (unsigned)blocks;
+    auto *bt = new mifdtd_batch;
+    bt->ninst = ninst;
+    bt->hc = hc;
+    bt->pml = pml;
+    bt->total_blocks = (unsigned)blocks;
+    bt->total_src = tsrc;
+    bt->total_probe = tprobe;
+    int rc = hip_status(hipMalloc(&bt->d_inst, sizeof(BInst) * ninst), "batch descriptor alloc");
+    if (!rc) rc = hip_status(hipMalloc(&bt->d_first, sizeof(unsigned) * (ninst + 1)), "batch prefix alloc");
+    if (!rc) rc = hip_status(hipMemcpy(bt->d_inst, h.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice), "batch upload");
+    if (!rc)
+        rc = hip_status(hipMemcpy(bt->d_first, first.data(), sizeof(unsigned) * (ninst + 1), hipMemcpyHostToDevice),
+                        "batch upload");
+    if (rc) {
+        if (bt->d_inst) (void)hipFree(bt->d_inst);
+        if (bt->d_first) (void)hipFree(bt->d_first);
+        delete bt;
+        return rc;
+    }
+    *out = bt;
+    return 0;
+}
+
+int mifdtd_batch_sizes(const mifdtd_batch *b, int *total_src, int *total_probe, unsigned *total_blocks)
+{
+    if (!b) return fail(MIFDTD_ENULL, "batch is NULL");
+    if (total_src) *total_src = b->total_src;
+    if (total_probe) *total_probe = b->total_probe;
+    if (total_blocks) *total_blocks = b->total_blocks;
+    return 0;
+}
+
+static int batch_launches(mifdtd_batch *b, int nsteps, const float *tvals, float *probe_series, unsigned R, hipStream_t st)
+{
+    const unsigned grid = b->total_blocks;
+    for (int s = 0; s < nsteps; s++) {
+        const float *tv = b->total_src ? tvals + (size_t)s * b->total_src : nullptr;
+        float *pr = b->total_probe ? probe_series + (size_t)s * b->total_probe : nullptr;
+#define MIFDTD_BATCH_LAUNCH(HC, PML, EH) \
+    k_batch<HC, PML, EH><<<grid, SB, 0, st>>>(b->d_inst, b->d_first, b->ninst, tv, pr, R)
+        if (b->hc && b->pml) MIFDTD_BATCH_LAUNCH(true, true, true);
+        else if (b->hc) MIFDTD_BATCH_LAUNCH(true, false, true);
+        else if (b->pml) MIFDTD_BATCH_LAUNCH(false, true, true);
+        else MIFDTD_BATCH_LAUNCH(false, false, true);
+        MIFDTD_CHECK_LAUNCH("batch E launch");
+        if (b->pml) MIFDTD_BATCH_LAUNCH(false, true, false);
+        else MIFDTD_BATCH_LAUNCH(false, false, false);
+        MIFDTD_CHECK_LAUNCH("batch H launch");
+#undef MIFDTD_BATCH_LAUNCH
+    }
+    return 0;
+}
+
+int mifdtd_batch_run(mifdtd_batch *b, int nsteps, const float *tvals, float *probe_series, int use_graph,
+                     int remap_rows, mifdtd_stream_t stream)
+{
+    if (!b) return fail(MIFDTD_ENULL, "batch is NULL");
+    if (nsteps < 0) return fail(MIFDTD_EINVAL, "nsteps < 0");
+    if (b->total_src && !tvals) return fail(MIFDTD_ENULL, "tvals is NULL");
+    if (b->total_probe && !probe_series) return fail(MIFDTD_ENULL, "probe_series is NULL");
+    if (remap_rows < 0 || remap_rows > 4096) return fail(MIFDTD_EINVAL, "remap_rows %d outside [0,4096]", remap_rows);
+    if (nsteps == 0) return 0;
+    hipStream_t st = to_stream(stream);
+    if (!use_graph) return batch_launches(b, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
+    if (!st) return fail(MIFDTD_EINVAL, "graph capture needs a non-null stream");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = hip_status(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    const int lrc = batch_launches(b, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
+    rc = hip_status(hipStreamEndCapture(st, &graph), "end capture");
+    if (lrc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return lrc;
+    }
+    if (!rc) rc = hip_status(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    if (!rc) rc = hip_status(hipGraphLaunch(exec, st), "graph launch");
+    // the executable graph is released once its launch has completed
+    if (!rc) rc = hip_status(hipStreamSynchronize(st), "graph wait");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+int mifdtd_batch_destroy(mifdtd_batch *b)
+{
+    if (!b) return 0;
+    int rc = 0;
+    if (b->d_inst) rc = hip_status(hipFree(b->d_inst), "batch free");
+    if (b->d_first && !rc) rc = hip_status(hipFree(b->d_first), "batch free");
+    delete b;
+    return rc;
+}
+
+}  // extern "C"

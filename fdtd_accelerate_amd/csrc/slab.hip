@@ -1,527 +1,18 @@
 // Slab-ordered step: per time step, for each slab of consecutive x planes, one E pass
 // (core E + CPML-E + E sources + E probes) then one H pass (core H + CPML-H + H sources
-// + H probes) over that slab, in place.  A slab is sized so that the E, H (and CE) data
-// it touches stays resident in the 256 MB Infinity Cache between its two passes: the
-// H pass then re-reads E_new and H_old from the cache and HBM sees each array once per
-// step (60 B/cell + CPML), while both kernels keep the simple one-quad-per-thread,
-// high-occupancy streaming form.
+// + H probes) over that slab, in place; by default the slab is the whole grid (two
+// launches per step).  Smaller slabs keep a slab's E/H planes in the last-level cache
+// between its two passes, which on MI355X was measured to gain less than the extra
+// launches cost.
 //
 // Ordering is the reference's (E of the slab before H of the slab; slabs in increasing
 // x): H of plane i needs E_new of planes i and i-1 (done by this or the previous slab's
 // E pass) and E of plane i needs H_old of planes i and i+1 (not yet touched: the slab's
-// H pass runs after its E pass, the next slab's later).  Per-cell arithmetic equals the
-// in-place update_e/update_h + CPML + source path, bit for bit (masked lanes: value*0).
+// H pass runs after its E pass, the next slab's later).
 
-#include "common.hpp"
+#include "step_quad.hpp"
 
 namespace mifdtd {
-
-constexpr int SB = 256;
-constexpr int kSMaxSrc = 4;
-constexpr int kSMaxProbe = 8;
-
-struct SCpml {
-    int npml;
-    const float *bE, *aE, *bH, *aH;
-    float *pe, *ph;          // packed psi E / psi H
-    size_t sX, sY, sZ;
-    __device__ __forceinline__ size_t off(int f, int c) const
-    {
-        const int ax = f >> 1, sd = f & 1;
-        if (ax == 0) return (size_t)(2 * sd + c) * sX;
-        if (ax == 1) return 4 * sX + (size_t)(2 * sd + c) * sY;
-        return 4 * sX + 4 * sY + (size_t)(2 * sd + c) * sZ;
-    }
-};
-
-struct SSrc {
-    int n;
-    int comp[kSMaxSrc], i0[kSMaxSrc], i1[kSMaxSrc], j0[kSMaxSrc], j1[kSMaxSrc], k0[kSMaxSrc], k1[kSMaxSrc], ow[kSMaxSrc];
-    float tval[kSMaxSrc], svs[kSMaxSrc];
-    const float *sv[kSMaxSrc];
-};
-
-struct SProbe {
-    int n;
-    int comp[kSMaxProbe], i[kSMaxProbe], j[kSMaxProbe], k[kSMaxProbe], slot[kSMaxProbe];
-    float *out;  // row of the probe series for this step; probe n writes out[slot[n]]
-};
-
-struct SGeom {
-    int nx, ny, nzp, nz;
-    unsigned Q;        // quads per row
-    int i0;            // first plane of the slab
-    unsigned nquads;   // quads in the slab
-    int remap;         // deal the slab's blocks to XCDs in contiguous runs
-};
-
-// Blocks are dealt round-robin to the 8 XCDs; renumber them so that each XCD runs a
-// contiguous run of rows and the j-1 / j+1 rows a block reads sit in its own L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
-{
-    if (nb & 7u) return b;
-    return (b & 7u) * (nb >> 3) + (b >> 3);
-}
-
-__device__ __forceinline__ int sslab_e(int q, int n_e, int npml, int &side, int &p)
-{
-    side = 0;
-    p = 0;
-    if (q < npml) { p = q; return p + 1; }
-    if (q >= n_e - 1 - npml && q < n_e - 1) { side = 1; p = q - (n_e - 1 - npml); return npml + 1 + p; }
-    return -1;
-}
-__device__ __forceinline__ int sslab_h(int q, int n_e, int npml, int &side, int &p)
-{
-    side = 0;
-    p = 0;
-    if (q < npml) { p = q; return p + 1; }
-    if (q >= n_e - npml && q < n_e) { side = 1; p = q - (n_e - npml); return npml + 2 + p; }
-    return -1;
-}
-
-__device__ __forceinline__ void sapply_src(const SSrc &s, int c, int i, int j, int k0, Q4 &v)
-{
-#pragma unroll
-    for (int n = 0; n < kSMaxSrc; ++n) {
-        if (n < s.n && s.comp[n] == c && i >= s.i0[n] && i <= s.i1[n] && j >= s.j0[n] && j <= s.j1[n] &&
-            k0 + 3 >= s.k0[n] && k0 <= s.k1[n]) {
-            const int lj = s.j1[n] - s.j0[n] + 1, lk = s.k1[n] - s.k0[n] + 1;
-#pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                const int k = k0 + l;
-                if (k >= s.k0[n] && k <= s.k1[n]) {
-                    const float w = s.sv[n] ? s.sv[n][((size_t)(i - s.i0[n]) * lj + (j - s.j0[n])) * lk + (k - s.k0[n])]
-                                            : s.svs[n];
-                    const float add = s.tval[n] * w;
-                    v.v[l] = s.ow[n] ? add : v.v[l] + add;
-                }
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int k0, const Q4 &v)
-{
-#pragma unroll
-    for (int n = 0; n < kSMaxProbe; ++n) {
-        if (n < p.n && p.comp[n] == c && p.i[n] == i && p.j[n] == j) {
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (p.k[n] == k0 + l) p.out[p.slot[n]] = v.v[l];
-        }
-    }
-}
-
-// Both passes are written load-all / compute / store-all: every global load of a quad
-// (fields, neighbours, CE, psi) is issued before the first store, so a wave pays one
-// memory round trip even on CPML cells (psi pointers may alias the fields as far as the
-// compiler knows, so a load placed after a field store would wait for it).
-
-// E pass on quad f of the slab described by g.
-template <bool HC, bool PML>
-__device__ __forceinline__ void e_quad(
-    const SGeom &g, unsigned f, float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
-    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
-    const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
-    const SCpml &cp, const SSrc &src, const SProbe &prb)
-{
-    const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
-    const unsigned row = f / g.Q;
-    const int k0 = (int)(f - row * g.Q) * 4;
-    const int di = (int)(row / (unsigned)ny);
-    const int j = (int)row - di * ny;
-    const int i = g.i0 + di;
-    const size_t id = ((size_t)i * ny + j) * nzp + k0;
-    const size_t sx = (size_t)ny * nzp;
-    const bool jy = j < ny - 1, ix = i < nx - 1;
-    const Q4 z4 = splat4(0.f);
-
-    // ---------------------------------------------------------------- loads
-    const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
-    Q4 e0 = ld4(ex + id), e1 = ld4(ey + id), e2 = ld4(ez + id);
-    const Q4 c0 = HC ? ld4(cex + id) : splat4(0.5f);
-    const Q4 c1 = HC ? ld4(cey + id) : splat4(0.5f);
-    const Q4 c2 = HC ? ld4(cez + id) : splat4(0.5f);
-    const Q4 hzU = jy ? ld4(hz + id + nzp) : z4;   // hz(j+1)
-    const Q4 hxU = jy ? ld4(hx + id + nzp) : z4;   // hx(j+1)
-    const float hyn = jy ? hy[id + 4] : 0.f;       // hy(k0+4): next quad / next row start
-    const Q4 hzN = ix ? ld4(hz + id + sx) : z4;    // hz(i+1)
-    const Q4 hyN = ix ? ld4(hy + id + sx) : z4;    // hy(i+1)
-    const float hxn = ix ? hx[id + 4] : 0.f;       // hx(k0+4)
-
-    const int npml = PML ? cp.npml : 0;
-    int xs = 0, xp = 0, ys = 0, yp = 0, xt = -1, yt = -1;
-    Q4 px1 = z4, px2 = z4, py1 = z4, py2 = z4;
-    float *ppx1 = nullptr, *ppx2 = nullptr, *ppy1 = nullptr, *ppy2 = nullptr;
-    float pz1[4] = {0.f, 0.f, 0.f, 0.f}, pz2[4] = {0.f, 0.f, 0.f, 0.f};
-    int zt[4] = {-1, -1, -1, -1};
-    bool anyz = false;
-    if (PML) {
-        xt = sslab_e(i, nx, npml, xs, xp);  // x slab planes are all < nx-1
-        if (xt > 0) {
-            const size_t o = ((size_t)xp * ny + j) * nzp + k0;
-            ppx1 = cp.pe + cp.off(xs, 0) + o;
-            ppx2 = cp.pe + cp.off(xs, 1) + o;
-            px1 = ld4(ppx1);
-            px2 = ld4(ppx2);
-        }
-        yt = sslab_e(j, ny, npml, ys, yp);  // y slab rows are all < ny-1
-        if (yt > 0) {
-            const size_t o = ((size_t)i * npml + yp) * nzp + k0;
-            ppy1 = cp.pe + cp.off(2 + ys, 0) + o;
-            ppy2 = cp.pe + cp.off(2 + ys, 1) + o;
-            py1 = ld4(ppy1);
-            py2 = ld4(ppy2);
-        }
-        if (k0 < npml || k0 + 3 >= nz - 1 - npml) {
-            const size_t o = ((size_t)i * ny + j) * npml;
-#pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                int zs, zp;
-                zt[l] = sslab_e(k0 + l, nz, npml, zs, zp);
-                if (zt[l] > 0) {
-                    const float *q = cp.pe + cp.off(4 + zs, 0) + o + zp;  // psi2 sits sZ after psi1
-                    pz1[l] = q[0];
-                    pz2[l] = q[cp.sZ];
-                    anyz = true;
-                }
-            }
-        }
-    }
-
-    // -------------------------------------------------------------- compute
-    // ex: += cex*((hz[j+1]-hz) - (hy[k+1]-hy)) ; CPML + y psi2 (hz), - z psi1 (hy)
-    if (jy) {
-        float y1[4];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) y1[l] = l < 3 ? hy0.v[l + 1] : hyn;
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const float v = e0.v[l] + c0.v[l] * ((hzU.v[l] - hz0.v[l]) - (y1[l] - hy0.v[l]));
-            e0.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
-        }
-        if (PML) {
-            if (yt > 0) {
-                const float b = cp.bE[yt], a = cp.aE[yt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l)
-                    if (k0 + l < nz - 1) {
-                        const float tt = b * py2.v[l] + a * (hzU.v[l] - hz0.v[l]);
-                        py2.v[l] = tt;
-                        e0.v[l] = e0.v[l] + c0.v[l] * tt;
-                    }
-            }
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    const float tt = cp.bE[zt[l]] * pz1[l] + cp.aE[zt[l]] * (y1[l] - hy0.v[l]);
-                    pz1[l] = tt;
-                    e0.v[l] = e0.v[l] - c0.v[l] * tt;
-                }
-        }
-    }
-    // ey: += cey*((hx[k+1]-hx) - (hz[i+1]-hz)) ; CPML - x psi1 (hz), + z psi2 (hx)
-    if (ix) {
-        float x1[4];
-#pragma unroll
-        for (int l = 0; l < 4; ++l) x1[l] = l < 3 ? hx0.v[l + 1] : hxn;
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const float v = e1.v[l] + c1.v[l] * ((x1[l] - hx0.v[l]) - (hzN.v[l] - hz0.v[l]));
-            e1.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
-        }
-        if (PML) {
-            if (xt > 0) {
-                const float b = cp.bE[xt], a = cp.aE[xt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l)
-                    if (k0 + l < nz - 1) {
-                        const float tt = b * px1.v[l] + a * (hzN.v[l] - hz0.v[l]);
-                        px1.v[l] = tt;
-                        e1.v[l] = e1.v[l] - c1.v[l] * tt;
-                    }
-            }
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    const float tt = cp.bE[zt[l]] * pz2[l] + cp.aE[zt[l]] * (x1[l] - hx0.v[l]);
-                    pz2[l] = tt;
-                    e1.v[l] = e1.v[l] + c1.v[l] * tt;
-                }
-        }
-    }
-    // ez: += cez*((hy[i+1]-hy) - (hx[j+1]-hx)) ; CPML + x psi2 (hy), - y psi1 (hx)
-    if (ix && jy) {
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const float v = e2.v[l] + c2.v[l] * ((hyN.v[l] - hy0.v[l]) - (hxU.v[l] - hx0.v[l]));
-            e2.v[l] = (k0 + l < nz) ? v : v * 0.f;
-        }
-        if (PML) {
-            if (xt > 0) {
-                const float b = cp.bE[xt], a = cp.aE[xt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l)
-                    if (k0 + l < nz) {
-                        const float tt = b * px2.v[l] + a * (hyN.v[l] - hy0.v[l]);
-                        px2.v[l] = tt;
-                        e2.v[l] = e2.v[l] + c2.v[l] * tt;
-                    }
-            }
-            if (yt > 0) {
-                const float b = cp.bE[yt], a = cp.aE[yt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l)
-                    if (k0 + l < nz) {
-                        const float tt = b * py1.v[l] + a * (hxU.v[l] - hx0.v[l]);
-                        py1.v[l] = tt;
-                        e2.v[l] = e2.v[l] - c2.v[l] * tt;
-                    }
-            }
-        }
-    }
-    sapply_src(src, 0, i, j, k0, e0);
-    sapply_src(src, 1, i, j, k0, e1);
-    sapply_src(src, 2, i, j, k0, e2);
-
-    // --------------------------------------------------------------- stores
-    // (unchanged lanes and wall rows store back the value they loaded)
-    st4(ex + id, e0);
-    st4(ey + id, e1);
-    st4(ez + id, e2);
-    if (PML) {
-        if (xt > 0) {
-            st4(ppx1, px1);
-            st4(ppx2, px2);
-        }
-        if (yt > 0) {
-            st4(ppy1, py1);
-            st4(ppy2, py2);
-        }
-        if (anyz) {
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    int zs, zp;
-                    sslab_e(k0 + l, nz, npml, zs, zp);
-                    float *q = cp.pe + cp.off(4 + zs, 0) + ((size_t)i * ny + j) * npml + zp;
-                    if (jy) q[0] = pz1[l];
-                    if (ix) q[cp.sZ] = pz2[l];
-                }
-        }
-    }
-    sprobe(prb, 0, i, j, k0, e0);
-    sprobe(prb, 1, i, j, k0, e1);
-    sprobe(prb, 2, i, j, k0, e2);
-}
-
-// H pass on quad f of the slab described by g.
-template <bool PML>
-__device__ __forceinline__ void h_quad(
-    const SGeom &g, unsigned f, const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
-    float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
-    const SCpml &cp, const SSrc &src, const SProbe &prb)
-{
-    const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
-    (void)nx;
-    const unsigned row = f / g.Q;
-    const int k0 = (int)(f - row * g.Q) * 4;
-    const int di = (int)(row / (unsigned)ny);
-    const int j = (int)row - di * ny;
-    const int i = g.i0 + di;
-    const size_t id = ((size_t)i * ny + j) * nzp + k0;
-    const size_t sx = (size_t)ny * nzp;
-    const bool jp = j > 0, ip = i > 0;
-    const Q4 z4 = splat4(0.f);
-
-    // ---------------------------------------------------------------- loads
-    const Q4 ex0 = ld4(ex + id), ey0 = ld4(ey + id), ez0 = ld4(ez + id);
-    Q4 h0 = ld4(hx + id), h1 = ld4(hy + id), h2 = ld4(hz + id);
-    const Q4 ezD = jp ? ld4(ez + id - nzp) : z4;   // ez(j-1)
-    const Q4 exD = jp ? ld4(ex + id - nzp) : z4;   // ex(j-1)
-    const float eyp = jp ? ey[id - 1] : 0.f;       // ey(k0-1): previous quad / previous row end
-    const Q4 ezB = ip ? ld4(ez + id - sx) : z4;    // ez(i-1)
-    const Q4 eyB = ip ? ld4(ey + id - sx) : z4;    // ey(i-1)
-    const float exp_ = ip ? ex[id - 1] : 0.f;      // ex(k0-1)
-
-    const int npml = PML ? cp.npml : 0;
-    int xs = 0, xp = 0, ys = 0, yp = 0, xt = -1, yt = -1;
-    Q4 px1 = z4, px2 = z4, py1 = z4, py2 = z4;
-    float *ppx1 = nullptr, *ppx2 = nullptr, *ppy1 = nullptr, *ppy2 = nullptr;
-    float pz1[4] = {0.f, 0.f, 0.f, 0.f}, pz2[4] = {0.f, 0.f, 0.f, 0.f};
-    int zt[4] = {-1, -1, -1, -1};
-    bool anyz = false;
-    if (PML) {
-        xt = ip ? sslab_h(i, nx, npml, xs, xp) : -1;  // plane 0 is the wall
-        if (xt > 0) {
-            const size_t o = ((size_t)xp * ny + j) * nzp + k0;
-            ppx1 = cp.ph + cp.off(xs, 0) + o;
-            ppx2 = cp.ph + cp.off(xs, 1) + o;
-            px1 = ld4(ppx1);
-            px2 = ld4(ppx2);
-        }
-        yt = jp ? sslab_h(j, ny, npml, ys, yp) : -1;  // row 0 is the wall
-        if (yt > 0) {
-            const size_t o = ((size_t)i * npml + yp) * nzp + k0;
-            ppy1 = cp.ph + cp.off(2 + ys, 0) + o;
-            ppy2 = cp.ph + cp.off(2 + ys, 1) + o;
-            py1 = ld4(ppy1);
-            py2 = ld4(ppy2);
-        }
-        if (k0 < npml || k0 + 3 >= nz - npml) {
-            const size_t o = ((size_t)i * ny + j) * npml;
-#pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                int zs, zp;
-                const int k = k0 + l;
-                zt[l] = k > 0 ? sslab_h(k, nz, npml, zs, zp) : -1;  // k = 0 is the wall
-                if (zt[l] > 0) {
-                    const float *q = cp.ph + cp.off(4 + zs, 0) + o + zp;
-                    pz1[l] = q[0];
-                    pz2[l] = q[cp.sZ];
-                    anyz = true;
-                }
-            }
-        }
-    }
-
-    // -------------------------------------------------------------- compute
-    float ym[4], xm[4];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-        ym[l] = l > 0 ? ey0.v[l - 1] : eyp;
-        xm[l] = l > 0 ? ex0.v[l - 1] : exp_;
-    }
-    // hx: j>0 ; -= 0.5*((ez - ez[j-1]) - (ey - ey[k-1])) ; CPML - y psi2 (ez), + z psi1 (ey)
-    if (jp) {
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const float v = h0.v[l] - 0.5f * ((ez0.v[l] - ezD.v[l]) - (ey0.v[l] - ym[l]));
-            const int k = k0 + l;
-            h0.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
-        }
-        if (PML) {
-            if (yt > 0) {
-                const float b = cp.bH[yt], a = cp.aH[yt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l) {
-                    const int k = k0 + l;
-                    if (k > 0 && k < nz) {
-                        const float tt = b * py2.v[l] + a * (ez0.v[l] - ezD.v[l]);
-                        py2.v[l] = tt;
-                        h0.v[l] = h0.v[l] - 0.5f * tt;
-                    }
-                }
-            }
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    const float tt = cp.bH[zt[l]] * pz1[l] + cp.aH[zt[l]] * (ey0.v[l] - ym[l]);
-                    pz1[l] = tt;
-                    h0.v[l] = h0.v[l] + 0.5f * tt;
-                }
-        }
-    }
-    if (ip) {
-        // hy: -= 0.5*((ex - ex[k-1]) - (ez - ez[i-1])) ; CPML + x psi1 (ez), - z psi2 (ex)
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const float v = h1.v[l] - 0.5f * ((ex0.v[l] - xm[l]) - (ez0.v[l] - ezB.v[l]));
-            const int k = k0 + l;
-            h1.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
-        }
-        if (PML) {
-            if (xt > 0) {
-                const float b = cp.bH[xt], a = cp.aH[xt];
-#pragma unroll
-                for (int l = 0; l < 4; ++l) {
-                    const int k = k0 + l;
-                    if (k > 0 && k < nz) {
-                        const float tt = b * px1.v[l] + a * (ez0.v[l] - ezB.v[l]);
-                        px1.v[l] = tt;
-                        h1.v[l] = h1.v[l] + 0.5f * tt;
-                    }
-                }
-            }
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    const float tt = cp.bH[zt[l]] * pz2[l] + cp.aH[zt[l]] * (ex0.v[l] - xm[l]);
-                    pz2[l] = tt;
-                    h1.v[l] = h1.v[l] - 0.5f * tt;
-                }
-        }
-        // hz: j>0 ; -= 0.5*((ey - ey[i-1]) - (ex - ex[j-1])) ; CPML - x psi2 (ey), + y psi1 (ex)
-        if (jp) {
-#pragma unroll
-            for (int l = 0; l < 4; ++l) {
-                const float v = h2.v[l] - 0.5f * ((ey0.v[l] - eyB.v[l]) - (ex0.v[l] - exD.v[l]));
-                h2.v[l] = (k0 + l < nz) ? v : v * 0.f;
-            }
-            if (PML) {
-                if (xt > 0) {
-                    const float b = cp.bH[xt], a = cp.aH[xt];
-#pragma unroll
-                    for (int l = 0; l < 4; ++l)
-                        if (k0 + l < nz) {
-                            const float tt = b * px2.v[l] + a * (ey0.v[l] - eyB.v[l]);
-                            px2.v[l] = tt;
-                            h2.v[l] = h2.v[l] - 0.5f * tt;
-                        }
-                }
-                if (yt > 0) {
-                    const float b = cp.bH[yt], a = cp.aH[yt];
-#pragma unroll
-                    for (int l = 0; l < 4; ++l)
-                        if (k0 + l < nz) {
-                            const float tt = b * py1.v[l] + a * (ex0.v[l] - exD.v[l]);
-                            py1.v[l] = tt;
-                            h2.v[l] = h2.v[l] + 0.5f * tt;
-                        }
-                }
-            }
-        }
-    }
-    sapply_src(src, 3, i, j, k0, h0);
-    sapply_src(src, 4, i, j, k0, h1);
-    sapply_src(src, 5, i, j, k0, h2);
-
-    // --------------------------------------------------------------- stores
-    st4(hx + id, h0);
-    st4(hy + id, h1);
-    st4(hz + id, h2);
-    if (PML) {
-        if (xt > 0) {
-            st4(ppx1, px1);
-            st4(ppx2, px2);
-        }
-        if (yt > 0) {
-            st4(ppy1, py1);
-            st4(ppy2, py2);
-        }
-        if (anyz) {
-            const size_t o = ((size_t)i * ny + j) * npml;
-#pragma unroll
-            for (int l = 0; l < 4; ++l)
-                if (zt[l] > 0) {
-                    int zs, zp;
-                    sslab_h(k0 + l, nz, npml, zs, zp);
-                    float *q = cp.ph + cp.off(4 + zs, 0) + o + zp;
-                    if (jp) q[0] = pz1[l];
-                    if (ip) q[cp.sZ] = pz2[l];
-                }
-        }
-    }
-    sprobe(prb, 3, i, j, k0, h0);
-    sprobe(prb, 4, i, j, k0, h1);
-    sprobe(prb, 5, i, j, k0, h2);
-}
-
-struct SFields {
-    float *e[6];
-    const float *ce[3];
-};
 
 template <bool HC, bool PML>
 __global__ __launch_bounds__(SB) void k_slab_e(SGeom g, SFields F, SCpml cp, SSrc src, SProbe prb)

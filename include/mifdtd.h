@@ -206,6 +206,53 @@ int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
 int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps,
                     const float *tvals, float *probe_series, mifdtd_stream_t stream);
 
+/* ---------------------------------------------------------------- batched ----
+ * Many independent Yee (+CPML) simulations of uneven sizes advanced together: every
+ * step is two launches for the whole batch (E pass of all instances, then H pass), each
+ * block of a launch belonging to one instance.  Per instance the per-cell arithmetic is
+ * that of update_e/update_h + CPML + sources (as mifdtd_slab_run).  No reference
+ * counterpart: the reference runs one simulation per process (K7/main.py:296-326).
+ * Each instance: KEMP-layout fields (own nx, ny, nz_pitch, pad), CE arrays on every
+ * instance or on none, npml 0 or the batch's npml (tables shared), packed psi buffers
+ * (MIFDTD_PSI_PACKED, updated in place), up to MIFDTD_FUSED_MAX_SRC sources and
+ * MIFDTD_FUSED_MAX_PROBE probes.  Source n of instance m reads column
+ * (sources of instances < m) + n of the tvals row of the step; probes likewise fill the
+ * columns of a probe_series row. */
+typedef struct {
+    int nx, ny, nz_pitch, pad;
+    float *e[6];
+    const float *ce[3];
+    int npml;
+    float *psiE, *psiH;
+    int nsrc;
+    int src_comp[MIFDTD_FUSED_MAX_SRC];
+    int src_box[MIFDTD_FUSED_MAX_SRC][6];
+    const float *src_sv[MIFDTD_FUSED_MAX_SRC];
+    float src_svs[MIFDTD_FUSED_MAX_SRC];
+    int src_overwrite[MIFDTD_FUSED_MAX_SRC];
+    int nprobe;
+    int probe_comp[MIFDTD_FUSED_MAX_PROBE];
+    int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
+} mifdtd_instance;
+
+typedef struct mifdtd_batch mifdtd_batch; /* opaque; holds device copies of the descriptors */
+
+/* Validate the instances and upload their descriptors (allocates device memory). */
+int mifdtd_batch_create(const mifdtd_instance *inst, int ninst,
+                        const float *bE, const float *aE, const float *bH, const float *aH,
+                        mifdtd_batch **out);
+/* Columns of a tvals row and of a probe row; blocks of one launch. */
+int mifdtd_batch_sizes(const mifdtd_batch *b, int *total_src, int *total_probe,
+                       unsigned *total_blocks);
+/* Run nsteps steps.  tvals: DEVICE (nsteps, total_src) float32 source values;
+ * probe_series: DEVICE (nsteps, total_probe).  use_graph != 0 captures the run's
+ * launches into one hipGraph and launches it (needs a non-null stream; the call then
+ * returns after the graph completed).  remap_rows > 0 deals each instance's blocks to
+ * the 8 XCDs in runs of remap_rows consecutive blocks. */
+int mifdtd_batch_run(mifdtd_batch *b, int nsteps, const float *tvals, float *probe_series,
+                     int use_graph, int remap_rows, mifdtd_stream_t stream);
+int mifdtd_batch_destroy(mifdtd_batch *b);
+
 /* ------------------------------------------------------- 4th-order symplectic ----
  * Layout of the reference's 4th-order path: (nx, ny, nz) Fortran order (x contiguous,
  * no padding), S4/PyCUDA/100-base.py:74-89; nx must be a multiple of 4.
