@@ -12,6 +12,9 @@ source + H update + CPML-H (+ probe gather).
     python bench.py --workload batch [--steps 300] [--tune]   (64 uneven Yee+CPML 8
                        instances in one batched step loop vs the same instances run
                        one after another)
+    python bench.py --workload adjoint   (160^3 design problem: forward 800 steps with
+                       checkpoints every 50, adjoint gradient on the 64^3 box; reports
+                       backward / forward time)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -49,7 +52,7 @@ def parse():
     p.add_argument("--engine", default="kemp", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint"))
     p.add_argument("--instances", type=int, default=64)
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
@@ -266,6 +269,43 @@ def run_batch(args):
     }
 
 
+def run_adjoint(args):
+    """Config 4: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
+    64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
+    import numpy as np
+    import torch
+    from fdtd_accelerate_amd.adjoint import DesignProblem
+    p = DesignProblem()
+    eps = np.random.default_rng(4).uniform(1.0, 4.0, p.box[3:]).astype(np.float32)
+    p.forward(eps)  # warm-up (allocations, first launches)
+    p.backward()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    c = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    a.record(p.stream)
+    J = p.forward(eps)
+    b.record(p.stream)
+    g = p.backward()
+    c.record(p.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    fwd, bwd = a.elapsed_time(b), b.elapsed_time(c)
+    n = p.n[0] * p.n[1] * p.n[2]
+    return {
+        "metric": METRIC, "value": round(n * p.nsteps / (fwd * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": p.nsteps, "warmup": 1, "ms_per_step": round(fwd / p.nsteps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "adjoint gradient: 160^3 Yee, CPML 10, 64^3 design box eps_r U[1,4], Ez plane source x=20, "
+                               "J = sum of |E|^2 on x=140 over steps 601-800, checkpoints every 50", "cells": n},
+        "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3),
+        "J": J, "grad_norm": float(g.norm()),
+        "roofline": None,
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, 5 times: the reference kernel whose
     FETCH_SIZE / WRITE_SIZE counters (known bytes) scale the counters of the step kernels."""
@@ -321,6 +361,9 @@ def main():
     import torch
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
     if args.workload == "batch":
         if args.steps == 200:

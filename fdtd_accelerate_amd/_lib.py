@@ -51,6 +51,11 @@ class InstanceDesc(ctypes.Structure):
     ]
 
 
+class DesignBox(ctypes.Structure):
+    """``mifdtd_design_box``."""
+    _fields_ = [("i0", _I), ("j0", _I), ("k0", _I), ("di", _I), ("dj", _I), ("dk", _I)]
+
+
 # name -> (restype, argtypes)
 _SIGS = {
     "mifdtd_last_error": (ctypes.c_char_p, []),
@@ -76,6 +81,9 @@ _SIGS = {
     "mifdtd_batch_sizes": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint)]),
     "mifdtd_batch_run": (_I, [_P, _I, _P, _P, _I, _I, _P]),
     "mifdtd_batch_destroy": (_I, [_P]),
+    "mifdtd_adjoint_step": (_I, [_I] * 4 + [ctypes.POINTER(_P * 6), ctypes.POINTER(_P * 3), _I, _P, _P,
+                                            _P, _P, _P, _P, _P, _I, ctypes.POINTER(DesignBox), _P, _P, _P]),
+    "mifdtd_design_curl": (_I, [_I] * 4 + [_P] * 3 + [ctypes.POINTER(DesignBox), _P, _P]),
     "s4_update_h": (None, [_I] * 3 + [_F] + [_P] * 9),
     "s4_update_e": (None, [_I] * 3 + [_F] + [_P] * 9),
     "mifdtd_s4_update_h": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),

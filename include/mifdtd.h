@@ -253,6 +253,36 @@ int mifdtd_batch_run(mifdtd_batch *b, int nsteps, const float *tvals, float *pro
                      int use_graph, int remap_rows, mifdtd_stream_t stream);
 int mifdtd_batch_destroy(mifdtd_batch *b);
 
+/* ---------------------------------------------------------------- adjoint ----
+ * Discrete adjoint of one in-place Yee + CPML step (update_e, mifdtd_cpml_e, sources,
+ * update_h, mifdtd_cpml_h with CH = 0.5 and the packed psi layout), for gradients with
+ * respect to the CE coefficients of a design box.  No reference counterpart (the
+ * reference has no gradient path).  lam[0..5]: adjoint fields (same layout as the
+ * fields; on entry the adjoint of the state after step n, on return that of the state
+ * before it); lpsiE / lpsiH: adjoint psi (packed layout).  eplane (or NULL): the
+ * forward E^n on the monitor plane x = iplane, (3, ny, nz_pitch), whose |E|^2 the
+ * objective sums at this step (adds 2*E^n to the adjoint).  curl: (3, di, dj, dk) curl
+ * of H^{n-1} on the box as update_e forms it (mifdtd_design_curl, taken before step n);
+ * grad (or NULL): (3, di, dj, dk) double accumulator of dJ/dCE_c.
+ * lpsiE is carried one step behind: a call leaves the b_E scaling of its psi-E adjoint
+ * to the next call (two launches per step), so after the last step lpsiE is not yet
+ * the adjoint of the initial psi-E (the gradient and lE, lH, lpsiH are complete). */
+typedef struct {
+    int i0, j0, k0, di, dj, dk;
+} mifdtd_design_box;
+
+int mifdtd_adjoint_step(int nx, int ny, int nz_pitch, int pad,
+                        float *const lam[6], const float *const ce[3], int npml,
+                        float *lpsiE, float *lpsiH,
+                        const float *bE, const float *aE, const float *bH, const float *aH,
+                        const float *eplane, int iplane, const mifdtd_design_box *box,
+                        const float *curl, double *grad, mifdtd_stream_t stream);
+/* out (3, di, dj, dk): (hz[j+1]-hz)-(hy[k+1]-hy), (hx[k+1]-hx)-(hz[i+1]-hz),
+ * (hy[i+1]-hy)-(hx[j+1]-hx) on the box (0 where update_e does not write). */
+int mifdtd_design_curl(int nx, int ny, int nz_pitch, int pad,
+                       const float *hx, const float *hy, const float *hz,
+                       const mifdtd_design_box *box, float *out, mifdtd_stream_t stream);
+
 /* ------------------------------------------------------- 4th-order symplectic ----
  * Layout of the reference's 4th-order path: (nx, ny, nz) Fortran order (x contiguous,
  * no padding), S4/PyCUDA/100-base.py:74-89; nx must be a multiple of 4.

@@ -1,0 +1,118 @@
+"""Writes tests/golden/adjoint_config4.npz: FP64 reference data for the adjoint gradient
+of the config-4 design problem (run here, on the CPU, with the oracle):
+
+  160^3 grid, CPML 10, 64^3 design box at the centre with eps_r ~ U[1,4] (float32,
+  numpy.random.default_rng(EPS_SEED)), Ez soft source on the plane x = 20 over the
+  non-CPML interior driven by the config-0 pulse, objective J = sum over steps 601..800
+  of sum over the plane x = 140 of |E|^2.
+
+* central finite differences of the float64 C oracle (OracleSim) at 32 seeded voxels,
+  with the step h chosen by a sweep on three voxels (the h whose estimate changes least
+  against h/3);
+* the float64 gradient on the whole box from torch.autograd through the float64
+  restatement oracle/torch_ref.py (checkpointed).
+
+    python tests/golden/make_adjoint_golden.py [--quick]
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+from oracle.torch_ref import TorchYee, objective_grad  # noqa: E402
+
+EPS_SEED, VOX_SEED = 4, 44
+
+
+def config(quick):
+    if quick:
+        return dict(n=48, npml=6, box=(16, 16, 16, 16, 16, 16), src=8, mon=38, nsteps=200, window=60, nvox=6)
+    return dict(n=160, npml=10, box=(48, 48, 48, 64, 64, 64), src=20, mon=140, nsteps=800, window=200, nvox=32)
+
+
+def design_eps(box):
+    return np.random.default_rng(EPS_SEED).uniform(1.0, 4.0, box[3:]).astype(np.float32)
+
+
+def voxels(box, count):
+    r = np.random.default_rng(VOX_SEED)
+    return np.stack([r.integers(0, box[3 + a], count) for a in range(3)], 1)
+
+
+def oracle_J(cfg, eps64):
+    n, npml, box = cfg["n"], cfg["npml"], cfg["box"]
+    i0, j0, k0, di, dj, dk = box
+    ces = [np.full((n, n, n), 0.5) for _ in range(3)]
+    for c in ces:
+        c[i0:i0 + di, j0:j0 + dj, k0:k0 + dk] = 0.5 / eps64
+    o = O.OracleSim(n, n, n, 0, npml, np.float64, ces=ces)
+    o.add_source("ez", (cfg["src"], npml, npml), (cfg["src"], n - npml - 1, n - npml - 1), O.pulse)
+    n0 = cfg["nsteps"] - cfg["window"] + 1
+    J = 0.0
+    for s in range(1, cfg["nsteps"] + 1):
+        o.step()
+        if s >= n0:
+            m = cfg["mon"]
+            J += float((o.ehs[0][m] ** 2).sum() + (o.ehs[1][m] ** 2).sum() + (o.ehs[2][m] ** 2).sum())
+    return J
+
+
+def fd(cfg, eps64, v, h):
+    e1 = eps64.copy()
+    e1[tuple(v)] += h
+    e2 = eps64.copy()
+    e2[tuple(v)] -= h
+    return (oracle_J(cfg, e1) - oracle_J(cfg, e2)) / (2 * h)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=os.path.join(HERE, "adjoint_config4.npz"))
+    a = ap.parse_args()
+    cfg = config(a.quick)
+    torch.set_num_threads(os.cpu_count() or 8)
+    eps32 = design_eps(cfg["box"])
+    eps64 = eps32.astype(np.float64)
+    vox = voxels(cfg["box"], cfg["nvox"])
+    t0 = time.time()
+    J = oracle_J(cfg, eps64)
+    print(f"J64 {J:.12g}  ({time.time() - t0:.1f} s per run)", flush=True)
+    hs = [3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4, 3e-5]
+    sweep = np.zeros((3, len(hs)))
+    for a_, v in enumerate(vox[:3]):
+        for b_, h in enumerate(hs):
+            sweep[a_, b_] = fd(cfg, eps64, v, h)
+        print("sweep", v, sweep[a_], flush=True)
+    change = np.abs(np.diff(sweep, axis=1)) / np.abs(sweep[:, 1:])
+    h = hs[int(np.argmin(change.max(0))) + 1]
+    print("h", h, flush=True)
+    fds = np.array([fd(cfg, eps64, v, h) for v in vox])
+    print("fd done", flush=True)
+    n, npml, box = cfg["n"], cfg["npml"], cfg["box"]
+    m = TorchYee(n, n, n, 0, npml)
+    ceb = [torch.full((n, n, n), 0.5, dtype=torch.float64) for _ in range(3)]
+    mask = torch.zeros((n, n, n), dtype=torch.float64)
+    mask[cfg["src"], npml:n - npml, npml:n - npml] = 1
+    n0 = cfg["nsteps"] - cfg["window"] + 1
+    Jt, g = objective_grad(m, ceb, box, torch.tensor(eps64), cfg["nsteps"], n0, cfg["mon"],
+                           lambda s: [(2, O.pulse(s) * mask)], seg=2 if not a.quick else 10)
+    print(f"autograd J {Jt:.12g} (oracle {J:.12g})", flush=True)
+    g = g.numpy()
+    rel = np.abs(g[tuple(vox.T)] - fds) / np.abs(fds)
+    print("autograd vs fd max rel", rel.max(), flush=True)
+    np.savez_compressed(a.out, config=np.array(repr(cfg)), eps_seed=EPS_SEED, vox_seed=VOX_SEED,
+                        eps_sum=float(eps64.sum()), voxels=vox, h=h, hs=np.array(hs), sweep=sweep, fd=fds,
+                        J64=J, grad_ref=g.astype(np.float32))
+    print("wrote", a.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+"""GPU adjoint gradient (hand-written adjoint kernels, checkpoint and recompute) against
+the FP64 reference gradient (autograd through oracle/torch_ref.py) and, at config 4,
+against the committed FP64 central differences at 32 seeded voxels."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle.torch_ref import TorchYee, objective_grad
+from tests.conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from fdtd_accelerate_amd import _lib
+    _lib.load()
+
+
+TF = staticmethod(lambda t: O.pulse(t, 25.0, 8.0, 12.0))
+
+
+def _problem(ckpt=20):
+    from fdtd_accelerate_amd.adjoint import DesignProblem
+    return DesignProblem(n=(32, 30, 28), npml=5, box=(10, 9, 8, 8, 7, 6), src_plane=6, monitor=24,
+                         nsteps=150, window=50, ckpt=ckpt, tfunc=lambda t: O.pulse(t, 25.0, 8.0, 12.0))
+
+
+def _reference(p, eps):
+    f = p.sim.fields
+    m = TorchYee(f.nx, f.ny, f.nz, f.pad, p.npml)
+    ceb = [torch.full((f.nx, f.ny, f.nz_pitch), 0.5, dtype=torch.float64) for _ in range(3)]
+    mask = torch.zeros((f.nx, f.ny, f.nz_pitch), dtype=torch.float64)
+    mask[6, p.npml:f.ny - p.npml, p.npml:f.nz - p.npml] = 1
+    tf = lambda t: O.pulse(t, 25.0, 8.0, 12.0)
+    return objective_grad(m, ceb, p.box, torch.tensor(eps.astype(np.float64)), p.nsteps, p.n0, p.monitor,
+                          lambda n: [(2, tf(n) * mask)])
+
+
+def test_adjoint_matches_fp64_reference():
+    p = _problem()
+    eps = np.random.default_rng(9).uniform(1, 4, p.box[3:]).astype(np.float32)
+    J = p.forward(eps)
+    g = p.backward().cpu().numpy()
+    Jr, gr = _reference(p, eps)
+    gr = gr.numpy()
+    assert abs(J - Jr) <= 1e-5 * abs(Jr)
+    rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+    assert rel < 2e-5, rel
+
+
+def test_checkpoint_interval_does_not_change_gradient():
+    eps = np.random.default_rng(10).uniform(1, 4, (8, 7, 6)).astype(np.float32)
+    out = []
+    for ck in (7, 50, 150):
+        p = _problem(ck)
+        p.forward(eps)
+        out.append(p.backward().cpu().numpy())
+    assert out[0].tobytes() == out[1].tobytes() == out[2].tobytes()
+
+
+def test_autograd_function():
+    from fdtd_accelerate_amd.adjoint import FdtdObjective
+    p = _problem()
+    eps = torch.tensor(np.random.default_rng(11).uniform(1, 4, p.box[3:]).astype(np.float32), device="cuda",
+                       requires_grad=True)
+    J = FdtdObjective.apply(eps, p)
+    (2.0 * J).backward()
+    g = eps.grad.cpu().numpy()
+    p2 = _problem()
+    p2.forward(eps.detach())
+    g2 = p2.backward().cpu().numpy()
+    np.testing.assert_allclose(g, 2.0 * g2, rtol=1e-6)
+
+
+_GOLD = os.path.join(ROOT, "tests", "golden", "adjoint_config4.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(_GOLD), reason="config-4 FP64 reference data not generated")
+def test_config4_adjoint_vs_fd_and_fp64_gradient():
+    """Config 4 (160^3, CPML 10, 64^3 design box, 800 steps, checkpoints every 50):
+    adjoint gradient vs FP64 central differences at 32 voxels (max rel < 1e-3) and vs the
+    FP64 reference gradient over the box (relative L2 < 1e-4)."""
+    from fdtd_accelerate_amd.adjoint import DesignProblem
+    d = np.load(_GOLD)
+    box = (48, 48, 48, 64, 64, 64)
+    eps = np.random.default_rng(int(d["eps_seed"])).uniform(1.0, 4.0, box[3:]).astype(np.float32)
+    assert abs(float(eps.astype(np.float64).sum()) - float(d["eps_sum"])) < 1e-6
+    p = DesignProblem()
+    J = p.forward(eps)
+    g = p.backward().cpu().numpy()
+    assert abs(J - float(d["J64"])) <= 1e-5 * float(d["J64"])
+    vox = d["voxels"]
+    fd = d["fd"]
+    rel = np.abs(g[tuple(vox.T)] - fd) / np.abs(fd)
+    assert rel.max() < 1e-3, rel.max()
+    ref = d["grad_ref"].astype(np.float64)
+    assert np.linalg.norm(g - ref) / np.linalg.norm(ref) < 1e-4
