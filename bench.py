@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel (GB), from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a plain 1 GiB copy gives), keyed by
-# (engine, n, npml); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt.
-MEASURED_TRAFFIC_GB = {("kemp", 1024, 12): 64.69}
+# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt.
+MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69}
 
 
 def parse():
@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-ce-lut", action="store_true",
+                   help="kemp engine: read CE as float arrays instead of uint8 indices + table")
     p.add_argument("--cpu-sample-n", type=int, default=256)
     return p.parse_args()
 
@@ -81,6 +83,8 @@ def build_sim(args):
     c = n // 2
     sim.add_source("ez", (c, c, c), (c, c, c), gaussian_pulse())
     sim.add_probe("ez", (c + n // 4, c, c))
+    if args.engine == "kemp" and not args.no_ce_lut:
+        sim.fields.compress_ce()
     torch.cuda.synchronize()
     return sim
 
@@ -114,6 +118,8 @@ def time_kemp(sim, steps, warmup):
     wall = time.perf_counter() - wall0
     ms = t0.elapsed_time(t1)
     kern_ms = sum(a.elapsed_time(b) for a, b in evs) / steps
+    if f.ce_mat is not None:
+        return ms / steps, wall, kern_ms, "update_e (CE as uint8 index + table)", 39.0
     return ms / steps, wall, kern_ms, "update_e", 48.0
 
 
@@ -345,7 +351,7 @@ def cpu_baseline(args):
         s.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el > 10.0 or steps >= 200:
+        if el > 10.0 or steps >= 1000:
             break
     gcells = ns ** 3 * steps / el / 1e9
     return {"value": round(gcells, 4), "unit": "Gcell/s", "cores": cores, "kind": "port",
@@ -383,6 +389,7 @@ def main():
         ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
     n = args.n
     cells = n ** 3
+    ce_storage = "uint8 index + table" if getattr(sim.fields, "ce_mat", None) is not None else "float32"
     gcells = cells / (ms * 1e-3) / 1e9
     step_bytes = 60.0 * cells + 32.0 * cpml_face_cells(n, args.npml)
     if kbytes is None:  # whole-step unit: algorithmic bytes of the step
@@ -402,11 +409,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point source",
-                   "engine": args.engine, "cells": cells},
+                   "engine": args.engine, "cells": cells,
+                   "ce_storage": ce_storage},
         "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine, n, args.npml)), "traffic_unit": "GB per launch",
+                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine, n, args.npml, ce_storage)), "traffic_unit": "GB per launch",
                      "bytes_per_cell": round(kbytes, 3), "avg_launch_ms": round(kern_ms, 4)},
         "wall_s": round(wall, 3),
         "device": torch.cuda.get_device_name(0),

@@ -65,6 +65,7 @@ _SIGS = {
     "update_h": (None, [_I] * 3 + [_P] * 9),
     "mifdtd_update_e": (_I, [_I] * 4 + [_P] * 9 + [_P]),
     "mifdtd_update_h": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "mifdtd_update_e_lut": (_I, [_I] * 4 + [_P] * 10 + [_P]),
     "update_x_e": (None, [_I] * 5 + [_P] * 10),
     "update_x_h": (None, [_I] * 5 + [_P] * 8),
     "update_y_e": (None, [_I] * 5 + [_P] * 10),

@@ -17,6 +17,10 @@ class Core:
 
     def update_e(self):
         f = self.mainf
+        if f.ce_on and f.ce_mat is not None:
+            _lib.call("mifdtd_update_e_lut", f.nx, f.ny, f.nz_pitch, f.pad, *f.ptrs(f.ehs),
+                      *[m.data_ptr() for m in f.ce_mat], f.ce_lut.data_ptr(), _lib.stream_handle(f.stream))
+            return
         ces = f.ces if f.ce_on else (None, None, None)
         _lib.call("mifdtd_update_e", f.nx, f.ny, f.nz_pitch, f.pad, *f.ptrs(f.ehs), *f.ptrs(ces),
                   _lib.stream_handle(f.stream))

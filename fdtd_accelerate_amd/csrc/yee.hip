@@ -116,6 +116,84 @@ __global__ __launch_bounds__(kBlock) void k_update_e(
     }
 }
 
+// update_e with the CE arrays held as per-component uint8 material indices into a
+// 3 x 256 float table (the exact float32 CE values; staged in LDS): 39 instead of 48
+// bytes per cell, same arithmetic and masks as k_update_e<true>.
+__device__ __forceinline__ Q4 lut4(const float *tab, const unsigned char *m)
+{
+    const uchar4 t = *reinterpret_cast<const uchar4 *>(m);
+    Q4 q;
+    q.v[0] = tab[t.x];
+    q.v[1] = tab[t.y];
+    q.v[2] = tab[t.z];
+    q.v[3] = tab[t.w];
+    return q;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_e_lut(
+    int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
+    float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
+    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+    const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my, const unsigned char *__restrict__ mz,
+    const float *__restrict__ lut)
+{
+    __shared__ float tab[3 * 256];
+    for (int t = threadIdx.x; t < 3 * 256; t += kBlock) tab[t] = lut[t];
+    __syncthreads();
+    const unsigned f = blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nquads) return;
+    const unsigned row = f / Q;
+    const int k0 = (int)(f - row * Q) * 4;
+    const int i = (int)(row / (unsigned)ny);
+    const int j = (int)row - i * ny;
+    const size_t id = (size_t)row * nzp + k0;
+    const size_t sx = (size_t)ny * nzp;
+
+    const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
+
+    if (j < ny - 1) {
+        const Q4 e = ld4(ex + id);
+        const Q4 c = lut4(tab, mx + id);
+        const Q4 h1 = ld4(hz + id + nzp);
+        const float hyn = hy[id + 4];
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float h2 = l < 3 ? hy0.v[l + 1] : hyn;
+            const float v = e.v[l] + c.v[l] * ((h1.v[l] - hz0.v[l]) - (h2 - hy0.v[l]));
+            r.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
+        }
+        st4(ex + id, r);
+    }
+    if (i < nx - 1) {
+        const Q4 e = ld4(ey + id);
+        const Q4 c = lut4(tab + 256, my + id);
+        const Q4 h2 = ld4(hz + id + sx);
+        const float hxn = hx[id + 4];
+        Q4 r;
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            const float h1 = l < 3 ? hx0.v[l + 1] : hxn;
+            const float v = e.v[l] + c.v[l] * ((h1 - hx0.v[l]) - (h2.v[l] - hz0.v[l]));
+            r.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
+        }
+        st4(ey + id, r);
+        if (j < ny - 1) {
+            const Q4 e3 = ld4(ez + id);
+            const Q4 c3 = lut4(tab + 512, mz + id);
+            const Q4 g1 = ld4(hy + id + sx);
+            const Q4 g2 = ld4(hx + id + nzp);
+            Q4 r3;
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                const float v = e3.v[l] + c3.v[l] * ((g1.v[l] - hy0.v[l]) - (g2.v[l] - hx0.v[l]));
+                r3.v[l] = (k0 + l < nz) ? v : v * 0.f;
+            }
+            st4(ez + id, r3);
+        }
+    }
+}
+
 template <bool HAS_C>
 __global__ __launch_bounds__(kBlock) void k_update_h(
     int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
@@ -601,6 +679,29 @@ int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey,
 {
     return launch_core(true, nx, ny, nz_pitch, pad, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
                        const_cast<float *>(hz), cex, cey, cez, to_stream(stream));
+}
+
+int mifdtd_update_e_lut(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey, float *ez, const float *hx,
+                        const float *hy, const float *hz, const unsigned char *mat_x, const unsigned char *mat_y,
+                        const unsigned char *mat_z, const float *lut, mifdtd_stream_t stream)
+{
+    if (int rc = check_grid(nx, ny, nz_pitch, pad)) return rc;
+    const float *ptrs[6] = {ex, ey, ez, hx, hy, hz};
+    for (const float *p : ptrs) {
+        if (!p) return fail(MIFDTD_ENULL, "field pointer is NULL");
+        if (!aligned16(p)) return fail(MIFDTD_EALIGN, "field pointer %p is not 16-byte aligned", (const void *)p);
+    }
+    if (!mat_x || !mat_y || !mat_z || !lut) return fail(MIFDTD_ENULL, "material index or table is NULL");
+    const unsigned char *mats[3] = {mat_x, mat_y, mat_z};
+    for (const unsigned char *m : mats)
+        if (reinterpret_cast<uintptr_t>(m) & 3u) return fail(MIFDTD_EALIGN, "material index array not 4-byte aligned");
+    const unsigned Q = (unsigned)(nz_pitch / 4);
+    const unsigned nquads = (unsigned)nx * ny * Q;
+    const unsigned grid = (nquads + kBlock - 1) / kBlock;
+    k_update_e_lut<<<grid, kBlock, 0, to_stream(stream)>>>(nx, ny, nz_pitch, nz_pitch - pad, nquads, Q, ex, ey, ez, hx,
+                                                           hy, hz, mat_x, mat_y, mat_z, lut);
+    MIFDTD_CHECK_LAUNCH("update_e_lut launch");
+    return 0;
 }
 
 int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad, const float *ex, const float *ey, const float *ez,

@@ -68,6 +68,8 @@ class Fields:
         self.ehs = [zeros() for _ in range(6)]
         self.ces = [self._half() for _ in range(3)] if self.ce_on else None
         self.chs = [self._half() for _ in range(3)] if self.ch_on else None
+        self.ce_mat = None  # per-component uint8 indices into ce_lut (compress_ce)
+        self.ce_lut = None
         self.instance_list = []
 
     def _half(self):
@@ -117,6 +119,29 @@ class Fields:
         if not self.ce_on:
             raise AttributeError("The Fields instance has no ce arrays. You should add 'e' in the option 'coeff_use'")
         self._fill(self.ces, [cex, cey, cez], "ce")
+        self.ce_mat = self.ce_lut = None
+
+    def compress_ce(self):
+        """Let the core E update read the CE arrays as uint8 material indices into a
+        3 x 256 table of their exact values (39 instead of 48 bytes per cell; results
+        unchanged).  Possible when each component has at most 256 distinct values;
+        returns whether it was done.  The float arrays stay (the CPML kernels and host
+        access use them); call again after changing CE."""
+        if not self.ce_on:
+            return False
+        lut = torch.zeros((3, 256), dtype=torch.float32, device=self.device)
+        mats = []
+        with torch.cuda.stream(self.stream):
+            for c, ce in enumerate(self.ces):
+                vals, inv = torch.unique(ce, return_inverse=True)
+                if vals.numel() > 256:
+                    self.ce_mat = self.ce_lut = None
+                    return False
+                lut[c, :vals.numel()] = vals
+                mats.append(inv.to(torch.uint8).contiguous())
+                del inv
+        self.ce_mat, self.ce_lut = mats, lut
+        return True
 
     def set_chs(self, chx, chy, chz):
         if not self.ch_on:

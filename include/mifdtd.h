@@ -71,6 +71,18 @@ int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad,
                     const float *chx, const float *chy, const float *chz,
                     mifdtd_stream_t stream);
 
+/* update_e with the CE arrays given as per-component uint8 material indices
+ * (mat_c, same shape as the fields, 4-byte aligned) into lut (DEVICE, 3 x 256 float32:
+ * row c holds the distinct CE values of component c).  Same results as mifdtd_update_e
+ * with the expanded arrays, 39 instead of 48 bytes per cell.  No reference
+ * counterpart (a storage form of the CE arrays of K7/cpu/fields.py:112-122 for media
+ * with at most 256 distinct values per component). */
+int mifdtd_update_e_lut(int nx, int ny, int nz_pitch, int pad,
+                        float *ex, float *ey, float *ez,
+                        const float *hx, const float *hy, const float *hz,
+                        const unsigned char *mat_x, const unsigned char *mat_y,
+                        const unsigned char *mat_z, const float *lut, mifdtd_stream_t stream);
+
 /* ------------------------------------------------------------------ CPML ----
  * Replaces kuFDTD's non-kappa CPML C extension, KU/cpu/core/cpml_non_kappa.c:14
  * (update_x_e), :91 (update_x_h), :156 (update_y_e), :227 (update_y_h),

@@ -326,3 +326,27 @@ def test_config0_128cube_1000_steps():
     assert err < 1e-4 and ferr < 1e-4
     # the pulse reached the probe and the timing matches the source (peak after step 100)
     assert np.abs(series).max() > 1e-4 and np.argmax(np.abs(series)) > 100
+
+
+def test_update_e_lut_equals_float_coefficients():
+    """CE held as uint8 material indices + exact value table: same bytes as the float
+    CE path over several steps (sphere media, CPML, source)."""
+    from fdtd_accelerate_amd import geometry
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    n, npml = 48, 6
+    sims = [YeeSim(n, n, 40, npml=npml, coeff_use="e") for _ in range(2)]
+    sph = geometry.random_spheres(5, 8, (3.0, 6.0), (1.0, 12.0), (npml,) * 3, (n - npml, n - npml, 40 - npml))
+    for s in sims:
+        geometry.paint_ce(s.fields.ces, sph, (n, n, 40))
+        s.add_source("ez", (24, 24, 20), (24, 24, 20), gaussian_pulse(10.0, 4.0, 8.0))
+    assert sims[1].fields.compress_ce()
+    assert sims[1].fields.ce_lut[0, 1] != 0
+    for s in sims:
+        s.run(30)
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert sims[0].field(name).cpu().numpy().tobytes() == sims[1].field(name).cpu().numpy().tobytes(), name
+    # more than 256 distinct values: refused
+    s3 = YeeSim(16, 16, 16, coeff_use="e")
+    s3.fields.set_ces(*[np.linspace(0.05, 0.5, 16 ** 3, dtype=np.float32).reshape(16, 16, 16)] * 3)
+    assert not s3.fields.compress_ce()
