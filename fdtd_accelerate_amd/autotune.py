@@ -33,13 +33,13 @@ def _load():
 
 
 def heuristic(nx, ny, nzp, npml):
-    """T rows per wave (y halo overhead 1/T), CX planes per chunk (seed overhead 1/CX),
-    keeping >= 4 waves per SIMD of work (4096 waves on 256 CUs)."""
-    T = 8 if ny >= 64 else max(1, ny // 8)
+    """T owned rows per block (y halo overhead 1/T; at most 7: the block has T+1 waves),
+    CX planes per chunk (seed-plane overhead 1/CX), keeping >= 1024 blocks in flight
+    (4 per CU on 256 CUs)."""
+    T = min(7, max(1, ny // 8))
     nrb = -(-ny // T)
-    target = 4 * 4 * N_CU
-    cx = nx
-    while cx > 4 and nrb * (-(-nx // cx)) < target:
+    cx = min(nx, 64)
+    while cx > 8 and nrb * (-(-nx // cx)) < 1024:
         cx = -(-cx // 2)
     return T, max(1, cx)
 
@@ -49,7 +49,7 @@ def fused_tiles(nx, ny, nzp, npml):
     return t if t is not None else heuristic(nx, ny, nzp, npml)
 
 
-def tune_fused(sim, candidates=((4, 16), (4, 32), (8, 16), (8, 32), (8, 64), (16, 32), (16, 64)), steps=5, save=True):
+def tune_fused(sim, candidates=((3, 32), (3, 64), (5, 32), (5, 64), (7, 32), (7, 64), (7, 128)), steps=5, save=True):
     """Time each (T, CX) on ``sim`` (a YeeSim with engine='fused') and record the best.
     Runs real steps (the simulation advances by len(candidates) * (steps + 1) steps)."""
     from .fused import FusedStepper

@@ -28,8 +28,7 @@
 
 namespace mifdtd {
 
-constexpr int FB = 256;          // threads per work-group (4 independent waves)
-constexpr int kMaxT = 64;        // rows per wave (bound of the per-wave LDS carry)
+constexpr int kMaxRows = 7;      // owned rows per block (a block has rows+1 waves)
 constexpr int kMaxSrc = 4;
 constexpr int kMaxProbe = 8;
 
@@ -328,11 +327,178 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const FC
     }
 }
 
-// SEED = true: E_new (ey, ez) of each chunk's preceding plane i0-1 is computed and
-// stored into the step n+1 buffers (the owner chunk stores the same values), so the
-// main sweep reads E_new(i-1) from memory on every plane; no psi is stored.
-template <bool HC, bool PML, bool SEED>
-__global__ __launch_bounds__(FB) void k_fused(
+// H_new of row j (0 <= j < ny) of plane i for this lane's quad: hc = H(i,j) of step n,
+// exn/eyn/ezn = E_new(i,j), exb/ezb = E_new(i,j-1) (ex, ez), eyp/ezp = E_new(i-1,j)
+// (ey, ez), exm/eym = E_new(i,j) at k0-1 (ex, ey).  Updates psi-H in place.
+template <bool PML>
+__device__ __forceinline__ void compute_H(const FGeom &g, const FCpml &cp, const FSrc &src, int i, int j, int m,
+                                          int l4, bool act, const Q4 hc[3], const Q4 &exn, const Q4 &eyn,
+                                          const Q4 &ezn, const Q4 &exb, const Q4 &ezb, const Q4 &eyp, const Q4 &ezp,
+                                          float exm, float eym, Q4 &hxo, Q4 &hyo, Q4 &hzo)
+{
+    const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
+    const int k0 = m * 256 + l4;
+    const int npml = PML ? cp.npml : 0;
+    int hxs = 0, hxp = 0;
+    const int hxt = PML ? slab_h(i, nx, npml, hxs, hxp) : -1;
+    (void)hxs;
+    (void)hxp;
+        int ys = 0, yp = 0;
+        const int yt = PML ? slab_h(j, ny, npml, ys, yp) : -1;
+        const bool zslab = PML && (k0 < npml || k0 + 3 >= nz - npml);
+        const size_t zb = PML ? ((size_t)i * ny + j) * npml : 0;
+        // hx: j>0 ; -= 0.5*((ez - ez[j-1]) - (ey - ey[k-1]))
+        hxo = hc[0];
+        if (act && j > 0) {
+            const Q4 &z0 = ezn, &zm = ezb, &y0 = eyn;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const float ymv = l > 0 ? y0.v[l - 1] : eym;
+                const float v = hxo.v[l] - 0.5f * ((z0.v[l] - zm.v[l]) - (y0.v[l] - ymv));
+                const int k = k0 + l;
+                hxo.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
+            }
+            if (PML) {
+                if (yt > 0) {  // y face psi2: hx -= 0.5*psi, from ez(j)-ez(j-1)
+                    const float b = cp.bH[yt], a = cp.aH[yt];
+                    float *ps = cp.h + cp.off(2 + ys, 1) + ((size_t)i * npml + yp) * nzp + (size_t)m * 256 + l4;
+                    Q4 p4 = ld4(ps);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        const int k = k0 + l;
+                        if (k > 0 && k < nz) {
+                            const float tt = b * p4.v[l] + a * (z0.v[l] - zm.v[l]);
+                            p4.v[l] = tt;
+                            hxo.v[l] = hxo.v[l] - 0.5f * tt;
+                        }
+                    }
+                    st4(ps, p4);
+                }
+                if (zslab) {
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {  // z face psi1: hx += 0.5*psi, from ey(k)-ey(k-1)
+                        const int k = k0 + l;
+                        int zs, zp;
+                        const int zt = slab_h(k, nz, npml, zs, zp);
+                        if (zt > 0 && k > 0) {
+                            float *ps = cp.h + cp.off(4 + zs, 0) + zb + zp;
+                            const float ymv = l > 0 ? y0.v[l - 1] : eym;
+                            const float tt = cp.bH[zt] * *ps + cp.aH[zt] * (y0.v[l] - ymv);
+                            *ps = tt;
+                            hxo.v[l] = hxo.v[l] + 0.5f * tt;
+                        }
+                    }
+                }
+            }
+        }
+        if (act) apply_src(src, 3, i, j, k0, hxo);
+        // hy: i>0 ; -= 0.5*((ex - ex[k-1]) - (ez - ez[i-1]))
+        hyo = hc[1];
+        if (act && i > 0) {
+            const Q4 &x0 = exn, &z0 = ezn, &zp_ = ezp;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const float xmv = l > 0 ? x0.v[l - 1] : exm;
+                const float v = hyo.v[l] - 0.5f * ((x0.v[l] - xmv) - (z0.v[l] - zp_.v[l]));
+                const int k = k0 + l;
+                hyo.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
+            }
+            if (PML) {
+                if (hxt > 0) {  // x face psi1: hy += 0.5*psi, from ez(i)-ez(i-1)
+                    const float b = cp.bH[hxt], a = cp.aH[hxt];
+                    float *ps = cp.h + cp.off(hxs, 0) + ((size_t)hxp * ny + j) * nzp + (size_t)m * 256 + l4;
+                    Q4 p4 = ld4(ps);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        const int k = k0 + l;
+                        if (k > 0 && k < nz) {
+                            const float tt = b * p4.v[l] + a * (z0.v[l] - zp_.v[l]);
+                            p4.v[l] = tt;
+                            hyo.v[l] = hyo.v[l] + 0.5f * tt;
+                        }
+                    }
+                    st4(ps, p4);
+                }
+                if (zslab) {
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {  // z face psi2: hy -= 0.5*psi, from ex(k)-ex(k-1)
+                        const int k = k0 + l;
+                        int zs, zp;
+                        const int zt = slab_h(k, nz, npml, zs, zp);
+                        if (zt > 0 && k > 0) {
+                            float *ps = cp.h + cp.off(4 + zs, 1) + zb + zp;
+                            const float xmv = l > 0 ? x0.v[l - 1] : exm;
+                            const float tt = cp.bH[zt] * *ps + cp.aH[zt] * (x0.v[l] - xmv);
+                            *ps = tt;
+                            hyo.v[l] = hyo.v[l] - 0.5f * tt;
+                        }
+                    }
+                }
+            }
+        }
+        if (act) apply_src(src, 4, i, j, k0, hyo);
+        // hz: i>0 && j>0 ; -= 0.5*((ey - ey[i-1]) - (ex - ex[j-1]))
+        hzo = hc[2];
+        if (act && i > 0 && j > 0) {
+            const Q4 &y0 = eyn, &yp_ = eyp, &x0 = exn, &xm = exb;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const float v = hzo.v[l] - 0.5f * ((y0.v[l] - yp_.v[l]) - (x0.v[l] - xm.v[l]));
+                hzo.v[l] = (k0 + l < nz) ? v : v * 0.f;
+            }
+            if (PML) {
+                if (hxt > 0) {  // x face psi2: hz -= 0.5*psi, from ey(i)-ey(i-1)
+                    const float b = cp.bH[hxt], a = cp.aH[hxt];
+                    float *ps = cp.h + cp.off(hxs, 1) + ((size_t)hxp * ny + j) * nzp + (size_t)m * 256 + l4;
+                    Q4 p4 = ld4(ps);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        if (k0 + l < nz) {
+                            const float tt = b * p4.v[l] + a * (y0.v[l] - yp_.v[l]);
+                            p4.v[l] = tt;
+                            hzo.v[l] = hzo.v[l] - 0.5f * tt;
+                        }
+                    }
+                    st4(ps, p4);
+                }
+                if (yt > 0) {  // y face psi1: hz += 0.5*psi, from ex(j)-ex(j-1)
+                    const float b = cp.bH[yt], a = cp.aH[yt];
+                    float *ps = cp.h + cp.off(2 + ys, 0) + ((size_t)i * npml + yp) * nzp + (size_t)m * 256 + l4;
+                    Q4 p4 = ld4(ps);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) {
+                        if (k0 + l < nz) {
+                            const float tt = b * p4.v[l] + a * (x0.v[l] - xm.v[l]);
+                            p4.v[l] = tt;
+                            hzo.v[l] = hzo.v[l] + 0.5f * tt;
+                        }
+                    }
+                    st4(ps, p4);
+                }
+            }
+        }
+        if (act) apply_src(src, 5, i, j, k0, hzo);
+}
+
+// One step as one sweep (single pass, ping-pong buffers).  A block of T+1 waves owns
+// rows j0 .. j0+T-1 of an x chunk [i0, i1): wave w takes row j0-1+w (w = 0 is the y
+// halo row below the block, whose E_new only feeds the block's first row).  For each
+// z segment m (256 floats: 64 lanes x one quad) the block marches the planes of the
+// chunk, starting with the seed plane i0-1 (E_new only, kept in registers as
+// E_new(i-1)):
+//   1. every wave computes E_new(i, j) of its row (core + CPML-E + E sources);
+//   2. it publishes ex, ez of its row in LDS (double-buffered: the buffer alternates
+//      with every exchanged plane, across segments too) for the row above, and lane 63 leaves ex, ey at k = 256m+255 for lane 0 of segment m+1
+//      (per plane and row, in LDS);
+//   3. one barrier;
+//   4. the owned rows compute H_new(i, j) (core + CPML-H + H sources) from E_new(i, j),
+//      E_new(i, j-1) (LDS), E_new(i-1, j) (registers) and the k-1 values (lane shuffles,
+//      lane 0 from the carry) and store E_new, H_new and the probes.
+// H(i, j), H(i, j+1) and H(i+1, j) are read from the step-n buffers (the rows a block
+// shares between waves and planes are L2 hits).  Nothing a wave reads is written in
+// the same launch except the LDS exchange, which the barrier and the parity orders.
+template <bool HC, bool PML>
+__global__ __launch_bounds__(512) void k_fused(
     FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
@@ -346,246 +512,108 @@ __global__ __launch_bounds__(FB) void k_fused(
     cp.eIn = psiE_in;
     cp.eOut = psiE_out;
     cp.h = psiH;
-    // per-wave carry of lane 63's E_new element 3 (ex, ey) for each row, read by lane 0
-    // in the next z segment; double-buffered by segment parity so the read of segment
-    // m-1's value never meets segment m's write; used by one wave only
-    __shared__ float carry[FB / 64][2][2][kMaxT];
+    extern __shared__ float lds[];
+    const int T = g.T;
+    float *xbuf = lds;                              // [2][T+1][2][256]: ex, ez of each row
+    float *cbuf = lds + 2 * (T + 1) * 2 * 256;      // [cx+1][T+1][2]: ex, ey at k = 256m+255
 
-    // XCD-aware block remap (8 XCDs, round-robin dispatch): consecutive logical blocks
-    // (neighbouring row blocks of one chunk) land on the same XCD.
+    // XCD-aware block order: consecutive logical blocks (neighbouring row blocks of a
+    // chunk, which share halo rows) land on the same XCD.
     int bid = blockIdx.x;
     {
         const int nb = g.nblocks, nb8 = nb & ~7;
         if (bid < nb8) bid = (bid & 7) * (nb8 >> 3) + (bid >> 3);
     }
-    const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wave = bid * (FB / 64) + wl;
-    if (wave >= g.nwaves) return;  // no barriers in this kernel: idle waves just leave
+    const int rb = bid % g.nrb;
+    const int chunk = bid / g.nrb;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int l4 = 4 * lane;
     const bool last_lane = lane == 63;
-    const int chunk = SEED ? 1 + wave / g.nrb : wave / g.nrb;
-    const int rb = SEED ? wave - (chunk - 1) * g.nrb : wave - chunk * g.nrb;
-    const int j0 = rb * g.T;
-    const int nrows = min(g.T, g.ny - j0);
-    const int i0 = SEED ? chunk * g.cx - 1 : chunk * g.cx;
-    const int i1 = SEED ? i0 + 1 : min(g.nx, i0 + g.cx);
-    const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
-    const size_t sx = (size_t)ny * nzp;
-    const int npml = PML ? cp.npml : 0;
+    const int j0 = rb * T;
+    const int j = j0 - 1 + w;
+    const bool live = j >= 0 && j < g.ny;
+    const bool own = w >= 1 && live;
+    const int i0 = chunk * g.cx;
+    const int i1 = min(g.nx, i0 + g.cx);
+    const int ib = i0 > 0 ? i0 - 1 : i0;
 
-    for (int i = i0; i < i1; ++i) {
-        int hxs = 0, hxp = 0;
-        const int hxt = PML ? slab_h(i, nx, npml, hxs, hxp) : -1;
-        for (int m = 0; m < g.nseg; ++m) {
-            const int k0 = m * 256 + l4;
-            const bool act = k0 < nzp;
-            const bool has_next_seg = (m + 1) * 256 < nzp;
-            // rows: r = -1 is the halo row below the block (E only; not needed to seed)
-            const int rfirst = SEED ? 0 : -1;
-            Q4 hc[3], hu[3];
-            load_H(c, g, i, j0 + rfirst, m, l4, act, hc);
-            Q4 exb = zero4(), ezb = zero4();  // E_new of the row below (ex, ez)
-            for (int r = rfirst; r < nrows; ++r) {
-                const int j = j0 + r;
+    int it = 0;  // exchanged planes so far: the LDS parity alternates across segments too
+    for (int m = 0; m < g.nseg; ++m) {
+        const int k0 = m * 256 + l4;
+        const bool act = k0 < g.nzp;
+        const bool has_next_seg = (m + 1) * 256 < g.nzp;
+        Q4 eyp = zero4(), ezp = zero4();  // E_new(i-1, j)
+        for (int i = ib; i < i1; ++i) {
+            const bool seed = i < i0;
+            Q4 hc[3] = {zero4(), zero4(), zero4()};
+            Q4 exn = zero4(), eyn = zero4(), ezn = zero4();
+            if (live && (own || !seed)) {
+                Q4 hu[3];
+                load_H(c, g, i, j, m, l4, act, hc);
                 load_H(c, g, i, j + 1, m, l4, act, hu);
-                Q4 exn = zero4(), eyn = zero4(), ezn = zero4();
-                const bool own = r >= 0;
-                if (j >= 0)
-                    compute_E<HC, PML>(g, c, cp, src, i, j, m, l4, act, last_lane, has_next_seg, hc, hu, own,
-                                       own && !SEED, exn, eyn, ezn);
+                compute_E<HC, PML>(g, c, cp, src, i, j, m, l4, act, last_lane, has_next_seg, hc, hu, own,
+                                   own && !seed, exn, eyn, ezn);
+            }
+            if (seed) {  // uniform over the block: no exchange, no barrier
+                eyp = eyn;
+                ezp = ezn;
+                continue;
+            }
+            const int par = it++ & 1;
+            float *xs = xbuf + (size_t)((par * (T + 1) + w) * 2) * 256;
+            if (act) {
+                *reinterpret_cast<float4 *>(xs + l4) = make_float4(exn.v[0], exn.v[1], exn.v[2], exn.v[3]);
+                *reinterpret_cast<float4 *>(xs + 256 + l4) = make_float4(ezn.v[0], ezn.v[1], ezn.v[2], ezn.v[3]);
+            }
+            // k-1 neighbours: lanes 1..63 from the lane below; lane 0 from lane 63 of the
+            // previous segment at this plane (this wave wrote it there); segment 0: k-1 < 0
+            // feeds masked lanes only
+            float *cs = cbuf + (size_t)(((i - ib) * (T + 1) + w) * 2);
+            float exm = __shfl_up(exn.v[3], 1);
+            float eym = __shfl_up(eyn.v[3], 1);
+            if (lane == 0) {
+                exm = m > 0 ? cs[0] : 0.f;
+                eym = m > 0 ? cs[1] : 0.f;
+            }
+            __syncthreads();
+            if (last_lane) {
+                cs[0] = exn.v[3];
+                cs[1] = eyn.v[3];
+            }
+            if (own) {
+                const float *xb = xbuf + (size_t)((par * (T + 1) + (w - 1)) * 2) * 256;
+                const Q4 exb = act ? Q4{{xb[l4], xb[l4 + 1], xb[l4 + 2], xb[l4 + 3]}} : zero4();
+                const Q4 ezb = act ? Q4{{xb[256 + l4], xb[256 + l4 + 1], xb[256 + l4 + 2], xb[256 + l4 + 3]}} : zero4();
+                Q4 hxo, hyo, hzo;
+                compute_H<PML>(g, cp, src, i, j, m, l4, act, hc, exn, eyn, ezn, exb, ezb, eyp, ezp, exm, eym, hxo, hyo,
+                               hzo);
                 const size_t rbase = row_base(g, i, j, m);
-                if (SEED) {
-                    if (act) {
-                        st4(oey + rbase + l4, eyn);
-                        st4(oez + rbase + l4, ezn);
-                    }
-                } else if (own) {
-                    // E_new(i-1, j): from the step n+1 buffers (this wave's previous plane,
-                    // or the seed pass on a chunk's first plane)
-                    Q4 eyp = zero4(), ezp = zero4();
-                    if (i > 0 && act) {
-                        eyp = ld4(oey + rbase - sx + l4);
-                        ezp = ld4(oez + rbase - sx + l4);
-                    }
-                    // k-1 neighbours of E_new(i,j); lane 0 takes lane 63's value of the
-                    // previous segment (segment 0: k-1 < 0 only feeds masked lanes)
-                    float exm = __shfl_up(exn.v[3], 1);
-                    float eym = __shfl_up(eyn.v[3], 1);
-                    if (lane == 0) {
-                        exm = m > 0 ? carry[wl][(m - 1) & 1][0][r] : 0.f;
-                        eym = m > 0 ? carry[wl][(m - 1) & 1][1][r] : 0.f;
-                    }
-                    if (last_lane) {
-                        carry[wl][m & 1][0][r] = exn.v[3];
-                        carry[wl][m & 1][1][r] = eyn.v[3];
-                    }
-
-                    int ys = 0, yp = 0;
-                    const int yt = PML ? slab_h(j, ny, npml, ys, yp) : -1;
-                    const bool zslab = PML && (k0 < npml || k0 + 3 >= nz - npml);
-                    const size_t zb = PML ? ((size_t)i * ny + j) * npml : 0;
-                    // hx: j>0 ; -= 0.5*((ez - ez[j-1]) - (ey - ey[k-1]))
-                    Q4 hxo = hc[0];
-                    if (act && j > 0) {
-                        const Q4 &z0 = ezn, &zm = ezb, &y0 = eyn;
+                if (act) {
+                    st4(oex + rbase + l4, exn);
+                    st4(oey + rbase + l4, eyn);
+                    st4(oez + rbase + l4, ezn);
+                    st4(ohx + rbase + l4, hxo);
+                    st4(ohy + rbase + l4, hyo);
+                    st4(ohz + rbase + l4, hzo);
+                }
+#pragma unroll
+                for (int n = 0; n < kMaxProbe; ++n) {
+                    if (n < prb.n && act && prb.i[n] == i && prb.j[n] == j) {
+                        const int cc = prb.comp[n];
 #pragma unroll
                         for (int l = 0; l < 4; ++l) {
-                            const float ymv = l > 0 ? y0.v[l - 1] : eym;
-                            const float v = hxo.v[l] - 0.5f * ((z0.v[l] - zm.v[l]) - (y0.v[l] - ymv));
-                            const int k = k0 + l;
-                            hxo.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
-                        }
-                        if (PML) {
-                            if (yt > 0) {  // y face psi2: hx -= 0.5*psi, from ez(j)-ez(j-1)
-                                const float b = cp.bH[yt], a = cp.aH[yt];
-                                float *ps = cp.h + cp.off(2 + ys, 1) + ((size_t)i * npml + yp) * nzp + (size_t)m * 256 + l4;
-                                Q4 p4 = ld4(ps);
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {
-                                    const int k = k0 + l;
-                                    if (k > 0 && k < nz) {
-                                        const float tt = b * p4.v[l] + a * (z0.v[l] - zm.v[l]);
-                                        p4.v[l] = tt;
-                                        hxo.v[l] = hxo.v[l] - 0.5f * tt;
-                                    }
-                                }
-                                st4(ps, p4);
-                            }
-                            if (zslab) {
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {  // z face psi1: hx += 0.5*psi, from ey(k)-ey(k-1)
-                                    const int k = k0 + l;
-                                    int zs, zp;
-                                    const int zt = slab_h(k, nz, npml, zs, zp);
-                                    if (zt > 0 && k > 0) {
-                                        float *ps = cp.h + cp.off(4 + zs, 0) + zb + zp;
-                                        const float ymv = l > 0 ? y0.v[l - 1] : eym;
-                                        const float tt = cp.bH[zt] * *ps + cp.aH[zt] * (y0.v[l] - ymv);
-                                        *ps = tt;
-                                        hxo.v[l] = hxo.v[l] + 0.5f * tt;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if (act) apply_src(src, 3, i, j, k0, hxo);
-                    // hy: i>0 ; -= 0.5*((ex - ex[k-1]) - (ez - ez[i-1]))
-                    Q4 hyo = hc[1];
-                    if (act && i > 0) {
-                        const Q4 &x0 = exn, &z0 = ezn, &zp_ = ezp;
-#pragma unroll
-                        for (int l = 0; l < 4; ++l) {
-                            const float xmv = l > 0 ? x0.v[l - 1] : exm;
-                            const float v = hyo.v[l] - 0.5f * ((x0.v[l] - xmv) - (z0.v[l] - zp_.v[l]));
-                            const int k = k0 + l;
-                            hyo.v[l] = (k > 0 && k < nz) ? v : v * 0.f;
-                        }
-                        if (PML) {
-                            if (hxt > 0) {  // x face psi1: hy += 0.5*psi, from ez(i)-ez(i-1)
-                                const float b = cp.bH[hxt], a = cp.aH[hxt];
-                                float *ps = cp.h + cp.off(hxs, 0) + ((size_t)hxp * ny + j) * nzp + (size_t)m * 256 + l4;
-                                Q4 p4 = ld4(ps);
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {
-                                    const int k = k0 + l;
-                                    if (k > 0 && k < nz) {
-                                        const float tt = b * p4.v[l] + a * (z0.v[l] - zp_.v[l]);
-                                        p4.v[l] = tt;
-                                        hyo.v[l] = hyo.v[l] + 0.5f * tt;
-                                    }
-                                }
-                                st4(ps, p4);
-                            }
-                            if (zslab) {
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {  // z face psi2: hy -= 0.5*psi, from ex(k)-ex(k-1)
-                                    const int k = k0 + l;
-                                    int zs, zp;
-                                    const int zt = slab_h(k, nz, npml, zs, zp);
-                                    if (zt > 0 && k > 0) {
-                                        float *ps = cp.h + cp.off(4 + zs, 1) + zb + zp;
-                                        const float xmv = l > 0 ? x0.v[l - 1] : exm;
-                                        const float tt = cp.bH[zt] * *ps + cp.aH[zt] * (x0.v[l] - xmv);
-                                        *ps = tt;
-                                        hyo.v[l] = hyo.v[l] - 0.5f * tt;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if (act) apply_src(src, 4, i, j, k0, hyo);
-                    // hz: i>0 && j>0 ; -= 0.5*((ey - ey[i-1]) - (ex - ex[j-1]))
-                    Q4 hzo = hc[2];
-                    if (act && i > 0 && j > 0) {
-                        const Q4 &y0 = eyn, &yp_ = eyp, &x0 = exn, &xm = exb;
-#pragma unroll
-                        for (int l = 0; l < 4; ++l) {
-                            const float v = hzo.v[l] - 0.5f * ((y0.v[l] - yp_.v[l]) - (x0.v[l] - xm.v[l]));
-                            hzo.v[l] = (k0 + l < nz) ? v : v * 0.f;
-                        }
-                        if (PML) {
-                            if (hxt > 0) {  // x face psi2: hz -= 0.5*psi, from ey(i)-ey(i-1)
-                                const float b = cp.bH[hxt], a = cp.aH[hxt];
-                                float *ps = cp.h + cp.off(hxs, 1) + ((size_t)hxp * ny + j) * nzp + (size_t)m * 256 + l4;
-                                Q4 p4 = ld4(ps);
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {
-                                    if (k0 + l < nz) {
-                                        const float tt = b * p4.v[l] + a * (y0.v[l] - yp_.v[l]);
-                                        p4.v[l] = tt;
-                                        hzo.v[l] = hzo.v[l] - 0.5f * tt;
-                                    }
-                                }
-                                st4(ps, p4);
-                            }
-                            if (yt > 0) {  // y face psi1: hz += 0.5*psi, from ex(j)-ex(j-1)
-                                const float b = cp.bH[yt], a = cp.aH[yt];
-                                float *ps = cp.h + cp.off(2 + ys, 0) + ((size_t)i * npml + yp) * nzp + (size_t)m * 256 + l4;
-                                Q4 p4 = ld4(ps);
-#pragma unroll
-                                for (int l = 0; l < 4; ++l) {
-                                    if (k0 + l < nz) {
-                                        const float tt = b * p4.v[l] + a * (x0.v[l] - xm.v[l]);
-                                        p4.v[l] = tt;
-                                        hzo.v[l] = hzo.v[l] + 0.5f * tt;
-                                    }
-                                }
-                                st4(ps, p4);
-                            }
-                        }
-                    }
-                    if (act) apply_src(src, 5, i, j, k0, hzo);
-                    if (act) {
-                        st4(oex + rbase + l4, exn);
-                        st4(oey + rbase + l4, eyn);
-                        st4(oez + rbase + l4, ezn);
-                        st4(ohx + rbase + l4, hxo);
-                        st4(ohy + rbase + l4, hyo);
-                        st4(ohz + rbase + l4, hzo);
-                    }
-                    // probes (values after the whole step)
-#pragma unroll
-                    for (int n = 0; n < kMaxProbe; ++n) {
-                        if (n < prb.n && act && prb.i[n] == i && prb.j[n] == j) {
-                            const int cc = prb.comp[n];
-#pragma unroll
-                            for (int l = 0; l < 4; ++l) {
-                                if (prb.k[n] == k0 + l) {
-                                    const float v = cc == 0 ? exn.v[l] : cc == 1 ? eyn.v[l] : cc == 2 ? ezn.v[l]
-                                                  : cc == 3 ? hxo.v[l] : cc == 4 ? hyo.v[l] : hzo.v[l];
-                                    prb.out[n] = v;
-                                }
+                            if (prb.k[n] == k0 + l) {
+                                const float v = cc == 0 ? exn.v[l] : cc == 1 ? eyn.v[l] : cc == 2 ? ezn.v[l]
+                                              : cc == 3 ? hxo.v[l] : cc == 4 ? hyo.v[l] : hzo.v[l];
+                                prb.out[n] = v;
                             }
                         }
                     }
                 }
-                // slide up one row
-                exb = exn;
-                ezb = ezn;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) hc[q] = hu[q];
             }
+            eyp = eyn;
+            ezp = ezn;
         }
     }
 }
@@ -604,7 +632,7 @@ int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
     const int nx = d->nx, ny = d->ny, nzp = d->nz_pitch, pad = d->pad;
     if (int rc = check_grid(nx, ny, nzp, pad)) return rc;
     const int T = d->rows_per_thread;
-    if (T < 1 || T > kMaxT) return fail(MIFDTD_EINVAL, "rows_per_thread must be in [1,%d]", kMaxT);
+    if (T < 1 || T > kMaxRows) return fail(MIFDTD_EINVAL, "rows_per_thread (rows per block) must be in [1,%d]", kMaxRows);
     const int CX = d->planes_per_chunk;
     if (CX < 1) return fail(MIFDTD_EINVAL, "planes_per_chunk must be >= 1");
     if (nsteps < 0) return fail(MIFDTD_EINVAL, "nsteps < 0");
@@ -631,10 +659,12 @@ int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
     g.cx = CX;
     g.nchunks = (nx + CX - 1) / CX;
     g.nseg = (nzp + 255) / 256;
-    const long long nw = (long long)g.nrb * g.nchunks;
-    if (nw >= (1ll << 31) - 64) return fail(MIFDTD_ERANGE, "too many waves");
-    g.nwaves = (int)nw;
-    g.nblocks = (int)((nw + FB / 64 - 1) / (FB / 64));
+    const long long nbl = (long long)g.nrb * g.nchunks;
+    if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
+    g.nblocks = (int)nbl;
+    g.nwaves = g.nblocks * (T + 1);
+    const size_t lds_bytes = sizeof(float) * ((size_t)2 * (T + 1) * 2 * 256 + (size_t)(CX + 1) * (T + 1) * 2);
+    if (lds_bytes > 64 * 1024) return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS (> 64 KiB)", T, CX, lds_bytes);
 
     FCpml cp{};
     const bool pml = d->npml > 0;
@@ -682,28 +712,21 @@ int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
     float *out[6];
     for (int q = 0; q < 6; q++) { in[q] = d->e[q]; out[q] = d->e_alt[q]; }
     float *pin = d->psiE, *pout = d->psiE_alt;
-    FGeom gs = g;  // seed pass: one plane per chunk boundary
-    gs.nwaves = g.nrb * (g.nchunks - 1);
-    gs.nblocks = (gs.nwaves + FB / 64 - 1) / (FB / 64);
 
     hipStream_t st = to_stream(stream);
+    const unsigned nthr = 64u * (unsigned)(T + 1);
     for (int s = 0; s < nsteps; s++) {
         for (int n = 0; n < src.n; n++) src.tval[n] = tvals[(size_t)s * src.n + n];
         prb.out = d->nprobe ? probe_series + (size_t)s * d->nprobe : nullptr;
-#define MIFDTD_FUSED_ARGS(G)                                                                               \
-    G, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2], out[3], out[4], out[5], d->ce[0], \
-        d->ce[1], d->ce[2], pin, pout, d->psiH, cp, src, prb
-#define MIFDTD_FUSED_LAUNCH(HCV, PMLV)                                                                            \
-    do {                                                                                                        \
-        if (gs.nwaves > 0) k_fused<HCV, PMLV, true><<<gs.nblocks, FB, 0, st>>>(MIFDTD_FUSED_ARGS(gs));          \
-        k_fused<HCV, PMLV, false><<<g.nblocks, FB, 0, st>>>(MIFDTD_FUSED_ARGS(g));                              \
-    } while (0)
+#define MIFDTD_FUSED_LAUNCH(HCV, PMLV)                                                                          \
+    k_fused<HCV, PMLV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], \
+                                                          out[1], out[2], out[3], out[4], out[5], d->ce[0],     \
+                                                          d->ce[1], d->ce[2], pin, pout, d->psiH, cp, src, prb)
         if (hc && pml) MIFDTD_FUSED_LAUNCH(true, true);
         else if (hc) MIFDTD_FUSED_LAUNCH(true, false);
         else if (pml) MIFDTD_FUSED_LAUNCH(false, true);
         else MIFDTD_FUSED_LAUNCH(false, false);
 #undef MIFDTD_FUSED_LAUNCH
-#undef MIFDTD_FUSED_ARGS
         MIFDTD_CHECK_LAUNCH("fused step launch");
         for (int q = 0; q < 6; q++) { const float *x = in[q]; in[q] = out[q]; out[q] = const_cast<float *>(x); }
         float *tp = pin;

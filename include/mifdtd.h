@@ -159,7 +159,9 @@ int mifdtd_gather(const float *f, const long long *offsets, int count, float *ou
 
 /* ------------------------------------------------------------- fused step ----
  * The whole time step (core E, CPML-E on all faces, E sources, core H, CPML-H, H
- * sources, probes) in one kernel launch per step, streaming each array once:
+ * sources, probes) in one kernel launch per step, streaming each array once (a
+ * work-group of rows_per_thread+1 waves owns that many y rows of an x chunk and
+ * recomputes E of the row below; rows_per_thread * planes_per_chunk bounds its LDS):
  * state of step n is read from e[] / psiE*[], step n+1 is written to e_alt[] /
  * psiE*_alt[] (every cell, walls copied), and the two sets swap every step.  psiH is
  * updated in place.  Same per-cell arithmetic as update_e/update_h + the CPML and
@@ -193,8 +195,8 @@ typedef struct {
     int nprobe;
     int probe_comp[MIFDTD_FUSED_MAX_PROBE];
     int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
-    int rows_per_thread;  /* y rows streamed by one wave, 1..64 */
-    int planes_per_chunk; /* x planes per wave sweep */
+    int rows_per_thread;  /* fused: y rows owned by one work-group, 1..7 (T+1 waves) */
+    int planes_per_chunk; /* fused: x planes marched by one work-group */
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values

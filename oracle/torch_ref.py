@@ -139,26 +139,31 @@ class TorchYee:
         return E + H + newpsi
 
 
-def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources, seg=10, want_grad=True):
+def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources, seg=10, outer=None,
+                   want_grad=True):
     """J and dJ/deps (float64) for the design box ``box`` = (i0, j0, k0, di, dj, dk).
 
     ce_background: 3 float64 CE tensors; on the box, CE = 0.5/eps on all components.
-    sources: function(step n) -> list of (component, additive tensor).  Steps are run
-    in checkpointed segments of ``seg`` steps."""
+    sources: function(step n) -> list of (component, additive tensor).
+    Memory: without ``outer`` the steps run in checkpointed segments of ``seg`` steps
+    (one saved state per segment).  With ``outer`` the forward keeps a state every
+    ``outer`` steps only and the reverse sweep recomputes each outer segment (with
+    ``seg``-step checkpoints inside), carrying the adjoint of the state between
+    segments: the gradient is the same, the peak memory ~ (outer/seg + 1) states."""
     i0, j0, k0, di, dj, dk = box
     eps = torch.as_tensor(eps, dtype=DT).clone().requires_grad_(want_grad)
-    ce = []
-    for c in range(3):
-        base = ce_background[c].clone()
-        pad = torch.zeros_like(base)
-        ce_box = 0.5 / eps
-        inner = torch.nn.functional.pad(ce_box, (k0, base.shape[2] - k0 - dk, j0, base.shape[1] - j0 - dj,
-                                                 i0, base.shape[0] - i0 - di))
-        mask = torch.nn.functional.pad(torch.ones_like(ce_box), (k0, base.shape[2] - k0 - dk, j0, base.shape[1] - j0 - dj,
-                                                                 i0, base.shape[0] - i0 - di))
-        ce.append(base * (1 - mask) + inner + pad)
 
-    def run_seg(n_start, n_end, *state):
+    def make_ce():
+        ce = []
+        for c in range(3):
+            base = ce_background[c]
+            padding = (k0, base.shape[2] - k0 - dk, j0, base.shape[1] - j0 - dj, i0, base.shape[0] - i0 - di)
+            inner = torch.nn.functional.pad(0.5 / eps, padding)
+            mask = torch.nn.functional.pad(torch.ones((di, dj, dk), dtype=DT), padding)
+            ce.append(base * (1 - mask) + inner)
+        return ce
+
+    def run_seg(ce, n_start, n_end, *state):
         st = list(state)
         acc = torch.zeros((), dtype=DT)
         for n in range(n_start, n_end):
@@ -167,18 +172,52 @@ def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources,
                 acc = acc + (st[0][monitor] ** 2).sum() + (st[1][monitor] ** 2).sum() + (st[2][monitor] ** 2).sum()
         return (acc, *st)
 
-    st = model.zeros()
-    J = torch.zeros((), dtype=DT)
-    for s in range(1, nsteps + 1, seg):
-        e = min(s + seg, nsteps + 1)
-        if want_grad:
-            out = checkpoint(run_seg, s, e, *st, use_reentrant=False)
-        else:
-            with torch.no_grad():
-                out = run_seg(s, e, *st)
-        J = J + out[0]
-        st = list(out[1:])
+    def chunked(ce, s0, s1, st, grad):
+        J = torch.zeros((), dtype=DT)
+        for s in range(s0, s1, seg):
+            e = min(s + seg, s1)
+            if grad:
+                out = checkpoint(run_seg, ce, s, e, *st, use_reentrant=False)
+            else:
+                out = run_seg(ce, s, e, *st)
+            J = J + out[0]
+            st = list(out[1:])
+        return J, st
+
     if not want_grad:
+        with torch.no_grad():
+            J, _ = chunked(make_ce(), 1, nsteps + 1, model.zeros(), False)
         return float(J), None
-    J.backward()
-    return float(J.detach()), eps.grad.detach().clone()
+    if outer is None:
+        J, _ = chunked(make_ce(), 1, nsteps + 1, model.zeros(), True)
+        J.backward()
+        return float(J.detach()), eps.grad.detach().clone()
+    # two levels: states every `outer` steps, reverse sweep over the outer segments
+    bounds = list(range(1, nsteps + 1, outer)) + [nsteps + 1]
+    saved = {}
+    J = 0.0
+    with torch.no_grad():
+        ce0 = make_ce()
+        st = model.zeros()
+        for a, b in zip(bounds[:-1], bounds[1:]):
+            saved[a] = [t.clone() for t in st]
+            Jp, st = chunked(ce0, a, b, st, False)
+            J += float(Jp)
+        del ce0, st
+    lam = None
+    for a, b in reversed(list(zip(bounds[:-1], bounds[1:]))):
+        st0 = [t.clone().requires_grad_(True) for t in saved.pop(a)]
+        ce = make_ce()
+        Jp, st1 = chunked(ce, a, b, st0, True)
+        outs, grads = [], []
+        if Jp.requires_grad:  # segments before the objective window add nothing to J
+            outs.append(Jp)
+            grads.append(torch.ones((), dtype=DT))
+        if lam is not None:
+            outs += st1
+            grads += lam
+        if outs:
+            torch.autograd.backward(outs, grads)
+        lam = [t.grad if t.grad is not None else torch.zeros_like(t) for t in st0]
+        del st0, st1, ce, outs
+    return J, eps.grad.detach().clone()

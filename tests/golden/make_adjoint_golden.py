@@ -82,20 +82,27 @@ def main():
     eps32 = design_eps(cfg["box"])
     eps64 = eps32.astype(np.float64)
     vox = voxels(cfg["box"], cfg["nvox"])
-    t0 = time.time()
-    J = oracle_J(cfg, eps64)
-    print(f"J64 {J:.12g}  ({time.time() - t0:.1f} s per run)", flush=True)
-    hs = [3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4, 3e-5]
-    sweep = np.zeros((3, len(hs)))
-    for a_, v in enumerate(vox[:3]):
-        for b_, h in enumerate(hs):
-            sweep[a_, b_] = fd(cfg, eps64, v, h)
-        print("sweep", v, sweep[a_], flush=True)
-    change = np.abs(np.diff(sweep, axis=1)) / np.abs(sweep[:, 1:])
-    h = hs[int(np.argmin(change.max(0))) + 1]
-    print("h", h, flush=True)
-    fds = np.array([fd(cfg, eps64, v, h) for v in vox])
-    print("fd done", flush=True)
+    stage = a.out + ".fd.npz"  # finite-difference stage, kept so a rerun skips it
+    if os.path.exists(stage):
+        z = np.load(stage)
+        J, h, hs, sweep, fds = float(z["J64"]), float(z["h"]), list(z["hs"]), z["sweep"], z["fd"]
+        print("finite differences loaded from", stage, flush=True)
+    else:
+        t0 = time.time()
+        J = oracle_J(cfg, eps64)
+        print(f"J64 {J:.12g}  ({time.time() - t0:.1f} s per run)", flush=True)
+        hs = [3e-2, 1e-2, 3e-3, 1e-3, 3e-4, 1e-4, 3e-5]
+        sweep = np.zeros((3, len(hs)))
+        for a_, v in enumerate(vox[:3]):
+            for b_, h in enumerate(hs):
+                sweep[a_, b_] = fd(cfg, eps64, v, h)
+            print("sweep", v, sweep[a_], flush=True)
+        change = np.abs(np.diff(sweep, axis=1)) / np.abs(sweep[:, 1:])
+        h = hs[int(np.argmin(change.max(0))) + 1]
+        print("h", h, flush=True)
+        fds = np.array([fd(cfg, eps64, v, h) for v in vox])
+        np.savez(stage, J64=J, h=h, hs=np.array(hs), sweep=sweep, fd=fds)
+        print("fd done", flush=True)
     n, npml, box = cfg["n"], cfg["npml"], cfg["box"]
     m = TorchYee(n, n, n, 0, npml)
     ceb = [torch.full((n, n, n), 0.5, dtype=torch.float64) for _ in range(3)]
@@ -103,7 +110,7 @@ def main():
     mask[cfg["src"], npml:n - npml, npml:n - npml] = 1
     n0 = cfg["nsteps"] - cfg["window"] + 1
     Jt, g = objective_grad(m, ceb, box, torch.tensor(eps64), cfg["nsteps"], n0, cfg["mon"],
-                           lambda s: [(2, O.pulse(s) * mask)], seg=2 if not a.quick else 10)
+                           lambda s: [(2, O.pulse(s) * mask)], seg=2, outer=None if a.quick else 50)
     print(f"autograd J {Jt:.12g} (oracle {J:.12g})", flush=True)
     g = g.numpy()
     rel = np.abs(g[tuple(vox.T)] - fds) / np.abs(fds)
@@ -112,6 +119,8 @@ def main():
                         eps_sum=float(eps64.sum()), voxels=vox, h=h, hs=np.array(hs), sweep=sweep, fd=fds,
                         J64=J, grad_ref=g.astype(np.float32))
     print("wrote", a.out, flush=True)
+    if os.path.exists(stage):
+        os.remove(stage)
 
 
 if __name__ == "__main__":

@@ -51,14 +51,16 @@ def _fill(sim, seed):
 @pytest.mark.parametrize("ns,npml,T,CX,steps", [
     ((24, 20, 32), 0, 1, 7, 3),
     ((24, 20, 32), 0, 2, 64, 4),
+    ((24, 20, 32), 0, 7, 5, 5),
     ((17, 23, 45), 0, 4, 5, 5),
     ((30, 26, 40), 4, 2, 8, 6),
-    ((30, 26, 40), 4, 8, 3, 5),
+    ((30, 26, 40), 4, 6, 3, 5),
     ((29, 31, 57), 5, 4, 16, 7),
     ((40, 36, 160), 6, 2, 11, 4),
     ((22, 22, 200), 3, 1, 1, 3),
     ((12, 13, 1000), 4, 3, 5, 3),
-    ((26, 70, 300), 5, 16, 9, 2),
+    ((26, 70, 300), 5, 7, 9, 2),
+    ((20, 30, 300), 4, 5, 7, 3),
 ])
 def test_fused_matches_inplace(ns, npml, T, CX, steps):
     """Random fields (walls included) + CE + CPML + a point and a plane source + probes:

@@ -51,3 +51,21 @@ def test_autograd_gradient_equals_central_differences():
         fd = (objective_grad(m, ceb, box, e1, 80, 61, 18, src, want_grad=False)[0] -
               objective_grad(m, ceb, box, e2, 80, 61, 18, src, want_grad=False)[0]) / (2 * h)
         assert abs(fd - float(g[v])) <= 1e-7 * abs(float(g[v])) + 1e-12, (v, fd, float(g[v]))
+
+
+def test_two_level_reverse_sweep_equals_checkpointed():
+    """The segment-by-segment reverse sweep (states every `outer` steps) gives the same
+    J and gradient as the plain checkpointed backward."""
+    nx, ny, nz, pad, npml = 24, 22, 20, 4, 4
+    box = (9, 8, 7, 5, 5, 5)
+    m = TorchYee(nx, ny, nz, pad, npml)
+    ceb = [torch.full((nx, ny, nz + pad), 0.5, dtype=torch.float64) for _ in range(3)]
+    mask = torch.zeros((nx, ny, nz + pad), dtype=torch.float64)
+    mask[5, npml:ny - npml, npml:nz - npml] = 1
+    tf = lambda t: O.pulse(t, 20.0, 6.0, 10.0)
+    src = lambda n: [(2, tf(n) * mask)]
+    eps = torch.tensor(np.random.default_rng(1).uniform(1, 4, box[3:]))
+    J1, g1 = objective_grad(m, ceb, box, eps, 80, 61, 18, src, seg=10)
+    J2, g2 = objective_grad(m, ceb, box, eps, 80, 61, 18, src, seg=3, outer=17)
+    assert abs(J1 - J2) <= 1e-12 * abs(J1)
+    assert float((g1 - g2).abs().max()) <= 1e-12 * float(g1.abs().max())
