@@ -37,8 +37,10 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel (GB), from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a plain 1 GiB copy gives), keyed by
-# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt.
-MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69}
+# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt and
+# profiles/update_e_lut_yee_cpml12_1024cube_hbm_counters.txt.
+MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
+                       ("kemp", 1024, 12, "uint8 index + table"): 53.24}
 
 
 def parse():
