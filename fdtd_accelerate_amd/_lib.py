@@ -37,6 +37,7 @@ class FusedDesc(ctypes.Structure):
         ("src_sv", _P * MAX_SRC), ("src_svs", _F * MAX_SRC), ("src_overwrite", _I * MAX_SRC),
         ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
         ("rows_per_thread", _I), ("planes_per_chunk", _I),
+        ("ce_mat", _P * 3), ("ce_lut", _P),
     ]
 
 
@@ -77,6 +78,8 @@ _SIGS = {
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
+    "mifdtd_fused_run_timed": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P,
+                                    ctypes.POINTER(ctypes.c_float)]),
     "mifdtd_slab_run": (_I, [ctypes.POINTER(FusedDesc), _I, _I, _P, _P, _P]),
     "mifdtd_batch_create": (_I, [ctypes.POINTER(InstanceDesc), _I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "mifdtd_batch_sizes": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint)]),

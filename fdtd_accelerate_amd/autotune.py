@@ -32,6 +32,12 @@ def _load():
     return _table
 
 
+def lds_fits(T, CX, table):
+    """Whether the fused kernel's LDS (E and H row exchange 2 x 2 x T x 2 x 256 floats,
+    the k-1 carry (CX+1)(T+1)2 floats, the 3 x 256 CE table) fits its 64 KiB."""
+    return (768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2 <= 16384
+
+
 def heuristic(nx, ny, nzp, npml):
     """T owned rows per block (y halo overhead 1/T; at most 7: the block has T+1 waves),
     CX planes per chunk (seed-plane overhead 1/CX), keeping >= 1024 blocks in flight
@@ -57,6 +63,8 @@ def tune_fused(sim, candidates=((3, 32), (3, 64), (5, 32), (5, 64), (7, 32), (7,
     best = None
     results = {}
     for T, CX in candidates:
+        if not lds_fits(T, CX, f.ce_mat is not None):
+            continue
         st = FusedStepper(sim, rows_per_thread=T, planes_per_chunk=CX)
         st.run(1, None)
         ms = st.run(steps, None, time_kernel=True)

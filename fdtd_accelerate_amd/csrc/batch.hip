@@ -129,6 +129,7 @@ int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE,
             B.cp.bH = bH;
             B.cp.aH = aH;
             B.cp.pe = d.psiE;
+            B.cp.pe_out = d.psiE;
             B.cp.ph = d.psiH;
             B.cp.sX = (size_t)np_ * ny * nzp;
             B.cp.sY = (size_t)nx * np_ * nzp;

@@ -76,6 +76,7 @@ int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps, con
         cp.bH = d->bH;
         cp.aH = d->aH;
         cp.pe = d->psiE;
+        cp.pe_out = d->psiE;
         cp.ph = d->psiH;
         cp.sX = (size_t)np_ * ny * nzp;
         cp.sY = (size_t)nx * np_ * nzp;

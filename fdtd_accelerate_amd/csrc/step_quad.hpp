@@ -16,6 +16,7 @@ struct SCpml {
     int npml;
     const float *bE, *aE, *bH, *aH;
     float *pe, *ph;          // packed psi E / psi H
+    float *pe_out;           // where the E pass stores psi E (pe for in-place updates)
     size_t sX, sY, sZ;
     __device__ __forceinline__ size_t off(int f, int c) const
     {
@@ -110,20 +111,17 @@ __device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int
 // memory round trip even on CPML cells (psi pointers may alias the fields as far as the
 // compiler knows, so a load placed after a field store would wait for it).
 
-// E pass on quad f of the slab described by g.
+// E pass on the quad (i, j, k0..k0+3): reads E from exi.. and writes it to exo.. (the
+// same arrays for an in-place update, the next-step buffers for a ping-pong one); psi E
+// is read from cp.pe and written to cp.pe_out.
 template <bool HC, bool PML>
-__device__ __forceinline__ void e_quad(
-    const SGeom &g, unsigned f, float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
-    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+__device__ __forceinline__ void e_quad_at(
+    const SGeom &g, int i, int j, int k0, const float *exi, const float *eyi, const float *ezi, float *exo,
+    float *eyo, float *ezo, const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const SCpml &cp, const SSrc &src, const SProbe &prb)
 {
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
-    const unsigned row = f / g.Q;
-    const int k0 = (int)(f - row * g.Q) * 4;
-    const int di = (int)(row / (unsigned)ny);
-    const int j = (int)row - di * ny;
-    const int i = g.i0 + di;
     const size_t id = ((size_t)i * ny + j) * nzp + k0;
     const size_t sx = (size_t)ny * nzp;
     const bool jy = j < ny - 1, ix = i < nx - 1;
@@ -131,7 +129,7 @@ __device__ __forceinline__ void e_quad(
 
     // ---------------------------------------------------------------- loads
     const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
-    Q4 e0 = ld4(ex + id), e1 = ld4(ey + id), e2 = ld4(ez + id);
+    Q4 e0 = ld4(exi + id), e1 = ld4(eyi + id), e2 = ld4(ezi + id);
     const Q4 c0 = HC ? ld4(cex + id) : splat4(0.5f);
     const Q4 c1 = HC ? ld4(cey + id) : splat4(0.5f);
     const Q4 c2 = HC ? ld4(cez + id) : splat4(0.5f);
@@ -145,7 +143,7 @@ __device__ __forceinline__ void e_quad(
     const int npml = PML ? cp.npml : 0;
     int xs = 0, xp = 0, ys = 0, yp = 0, xt = -1, yt = -1;
     Q4 px1 = z4, px2 = z4, py1 = z4, py2 = z4;
-    float *ppx1 = nullptr, *ppx2 = nullptr, *ppy1 = nullptr, *ppy2 = nullptr;
+    size_t ox1 = 0, ox2 = 0, oy1 = 0, oy2 = 0;  // psi offsets of the x / y face quads
     float pz1[4] = {0.f, 0.f, 0.f, 0.f}, pz2[4] = {0.f, 0.f, 0.f, 0.f};
     int zt[4] = {-1, -1, -1, -1};
     bool anyz = false;
@@ -153,18 +151,18 @@ __device__ __forceinline__ void e_quad(
         xt = sslab_e(i, nx, npml, xs, xp);  // x slab planes are all < nx-1
         if (xt > 0) {
             const size_t o = ((size_t)xp * ny + j) * nzp + k0;
-            ppx1 = cp.pe + cp.off(xs, 0) + o;
-            ppx2 = cp.pe + cp.off(xs, 1) + o;
-            px1 = ld4(ppx1);
-            px2 = ld4(ppx2);
+            ox1 = cp.off(xs, 0) + o;
+            ox2 = cp.off(xs, 1) + o;
+            px1 = ld4(cp.pe + ox1);
+            px2 = ld4(cp.pe + ox2);
         }
         yt = sslab_e(j, ny, npml, ys, yp);  // y slab rows are all < ny-1
         if (yt > 0) {
             const size_t o = ((size_t)i * npml + yp) * nzp + k0;
-            ppy1 = cp.pe + cp.off(2 + ys, 0) + o;
-            ppy2 = cp.pe + cp.off(2 + ys, 1) + o;
-            py1 = ld4(ppy1);
-            py2 = ld4(ppy2);
+            oy1 = cp.off(2 + ys, 0) + o;
+            oy2 = cp.off(2 + ys, 1) + o;
+            py1 = ld4(cp.pe + oy1);
+            py2 = ld4(cp.pe + oy2);
         }
         if (k0 < npml || k0 + 3 >= nz - 1 - npml) {
             const size_t o = ((size_t)i * ny + j) * npml;
@@ -279,17 +277,17 @@ __device__ __forceinline__ void e_quad(
 
     // --------------------------------------------------------------- stores
     // (unchanged lanes and wall rows store back the value they loaded)
-    st4(ex + id, e0);
-    st4(ey + id, e1);
-    st4(ez + id, e2);
+    st4(exo + id, e0);
+    st4(eyo + id, e1);
+    st4(ezo + id, e2);
     if (PML) {
         if (xt > 0) {
-            st4(ppx1, px1);
-            st4(ppx2, px2);
+            st4(cp.pe_out + ox1, px1);
+            st4(cp.pe_out + ox2, px2);
         }
         if (yt > 0) {
-            st4(ppy1, py1);
-            st4(ppy2, py2);
+            st4(cp.pe_out + oy1, py1);
+            st4(cp.pe_out + oy2, py2);
         }
         if (anyz) {
 #pragma unroll
@@ -297,7 +295,7 @@ __device__ __forceinline__ void e_quad(
                 if (zt[l] > 0) {
                     int zs, zp;
                     sslab_e(k0 + l, nz, npml, zs, zp);
-                    float *q = cp.pe + cp.off(4 + zs, 0) + ((size_t)i * ny + j) * npml + zp;
+                    float *q = cp.pe_out + cp.off(4 + zs, 0) + ((size_t)i * ny + j) * npml + zp;
                     if (jy) q[0] = pz1[l];
                     if (ix) q[cp.sZ] = pz2[l];
                 }
@@ -308,20 +306,31 @@ __device__ __forceinline__ void e_quad(
     sprobe(prb, 2, i, j, k0, e2);
 }
 
-// H pass on quad f of the slab described by g.
-template <bool PML>
-__device__ __forceinline__ void h_quad(
-    const SGeom &g, unsigned f, const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
-    float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+// E pass on quad f of the slab described by g (in place).
+template <bool HC, bool PML>
+__device__ __forceinline__ void e_quad(
+    const SGeom &g, unsigned f, float *ex, float *ey, float *ez,
+    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+    const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const SCpml &cp, const SSrc &src, const SProbe &prb)
+{
+    const unsigned row = f / g.Q;
+    const int k0 = (int)(f - row * g.Q) * 4;
+    const int di = (int)(row / (unsigned)g.ny);
+    const int j = (int)row - di * g.ny;
+    e_quad_at<HC, PML>(g, g.i0 + di, j, k0, ex, ey, ez, ex, ey, ez, hx, hy, hz, cex, cey, cez, cp, src, prb);
+}
+
+// H pass on the quad (i, j, k0..k0+3): reads H from hxi.. and writes it to hxo.. (the
+// same arrays for an in-place update); psi H is updated in place (cp.ph).
+template <bool PML>
+__device__ __forceinline__ void h_quad_at(
+    const SGeom &g, int i, int j, int k0, const float *__restrict__ ex, const float *__restrict__ ey,
+    const float *__restrict__ ez, const float *hxi, const float *hyi, const float *hzi, float *hxo, float *hyo,
+    float *hzo, const SCpml &cp, const SSrc &src, const SProbe &prb)
 {
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
     (void)nx;
-    const unsigned row = f / g.Q;
-    const int k0 = (int)(f - row * g.Q) * 4;
-    const int di = (int)(row / (unsigned)ny);
-    const int j = (int)row - di * ny;
-    const int i = g.i0 + di;
     const size_t id = ((size_t)i * ny + j) * nzp + k0;
     const size_t sx = (size_t)ny * nzp;
     const bool jp = j > 0, ip = i > 0;
@@ -329,7 +338,7 @@ __device__ __forceinline__ void h_quad(
 
     // ---------------------------------------------------------------- loads
     const Q4 ex0 = ld4(ex + id), ey0 = ld4(ey + id), ez0 = ld4(ez + id);
-    Q4 h0 = ld4(hx + id), h1 = ld4(hy + id), h2 = ld4(hz + id);
+    Q4 h0 = ld4(hxi + id), h1 = ld4(hyi + id), h2 = ld4(hzi + id);
     const Q4 ezD = jp ? ld4(ez + id - nzp) : z4;   // ez(j-1)
     const Q4 exD = jp ? ld4(ex + id - nzp) : z4;   // ex(j-1)
     const float eyp = jp ? ey[id - 1] : 0.f;       // ey(k0-1): previous quad / previous row end
@@ -480,9 +489,9 @@ __device__ __forceinline__ void h_quad(
     sapply_src(src, 5, i, j, k0, h2);
 
     // --------------------------------------------------------------- stores
-    st4(hx + id, h0);
-    st4(hy + id, h1);
-    st4(hz + id, h2);
+    st4(hxo + id, h0);
+    st4(hyo + id, h1);
+    st4(hzo + id, h2);
     if (PML) {
         if (xt > 0) {
             st4(ppx1, px1);
@@ -508,6 +517,19 @@ __device__ __forceinline__ void h_quad(
     sprobe(prb, 3, i, j, k0, h0);
     sprobe(prb, 4, i, j, k0, h1);
     sprobe(prb, 5, i, j, k0, h2);
+}
+
+// H pass on quad f of the slab described by g (in place).
+template <bool PML>
+__device__ __forceinline__ void h_quad(
+    const SGeom &g, unsigned f, const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
+    float *hx, float *hy, float *hz, const SCpml &cp, const SSrc &src, const SProbe &prb)
+{
+    const unsigned row = f / g.Q;
+    const int k0 = (int)(f - row * g.Q) * 4;
+    const int di = (int)(row / (unsigned)g.ny);
+    const int j = (int)row - di * g.ny;
+    h_quad_at<PML>(g, g.i0 + di, j, k0, ex, ey, ez, hx, hy, hz, hx, hy, hz, cp, src, prb);
 }
 
 struct SFields {

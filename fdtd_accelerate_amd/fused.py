@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .autotune import fused_tiles, slab_planes
+from .autotune import fused_tiles, lds_fits, slab_planes
 
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 
@@ -50,9 +50,17 @@ class FusedStepper:
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0)
         self.T = rows_per_thread or T
         self.CX = planes_per_chunk or CX
-        self.bytes_per_cell = 60.0 if f.ce_on else 48.0
+        if not lds_fits(self.T, self.CX, f.ce_mat is not None):
+            raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} need more than 64 KiB of LDS")
         if cp is not None:
             cp.alt_psiE()
+
+    @property
+    def bytes_per_cell(self):
+        """Algorithmic HBM bytes per cell of one step (without CPML): E, H read and
+        written (48 B) plus CE read as float arrays (12 B) or uint8 indices (3 B)."""
+        f = self.sim.fields
+        return 51.0 if f.ce_mat is not None else 60.0 if f.ce_on else 48.0
 
     def _desc(self):
         sim = self.sim
@@ -65,6 +73,10 @@ class FusedStepper:
         if f.ce_on:
             for c in range(3):
                 d.ce[c] = f.ces[c].data_ptr()
+        if self.alt and f.ce_mat is not None:  # CE as material indices + table (fused only)
+            for c in range(3):
+                d.ce_mat[c] = f.ce_mat[c].data_ptr()
+            d.ce_lut = f.ce_lut.data_ptr()
         cp = sim.cpml
         if cp is not None:
             d.npml = cp.npml
@@ -97,7 +109,7 @@ class FusedStepper:
                 tv[k, n] = np.float32(s.tfunc(s.tstep + k))
         return np.ascontiguousarray(tv)
 
-    def _call(self, nsteps, buf):
+    def _call(self, nsteps, buf, timed=False):
         sim = self.sim
         f = sim.fields
         if buf is None and sim.probe_points:
@@ -105,34 +117,47 @@ class FusedStepper:
         d = self._desc()
         tv = self._tvals(nsteps)
         parity = ctypes.c_int(0)
-        _lib.call("mifdtd_fused_run", ctypes.byref(d), nsteps, tv.ctypes.data,
-                  None if buf is None else buf.data_ptr(), ctypes.byref(parity),
-                  _lib.stream_handle(f.stream))
+        args = (ctypes.byref(d), nsteps, tv.ctypes.data, None if buf is None else buf.data_ptr(),
+                ctypes.byref(parity), _lib.stream_handle(f.stream))
+        sweep_ms = ctypes.c_float(0.0)
+        if timed:
+            _lib.call("mifdtd_fused_run_timed", *args, ctypes.byref(sweep_ms))
+        else:
+            _lib.call("mifdtd_fused_run", *args)
         for s in sim.sources:
             s.tstep += nsteps
         if parity.value:
             f.ehs, self.alt = self.alt, f.ehs
             if sim.cpml is not None:
                 sim.cpml.swap_psiE()
+        return sweep_ms.value
+
+    def sweep_box(self):
+        """(xb, xe, yb, ye, kb, ke): the interior box the single-pass sweep updates (the
+        cells with no CPML term; the whole grid without CPML), as the C side sets it."""
+        f = self.sim.fields
+        nx, ny, nz = f.nx, f.ny, f.nz
+        cp = self.sim.cpml
+        if cp is None:
+            return 0, nx, 0, ny, 0, f.nz_pitch
+        p = cp.npml
+        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 4 * ((p + 4) // 4), 4 * ((nz - 1 - p) // 4))
+        if box[0] >= box[1] or box[2] >= box[3] or box[4] >= box[5]:
+            return nx, nx, ny, ny, f.nz_pitch, f.nz_pitch
+        return box
+
+    def sweep_cells(self):
+        xb, xe, yb, ye, kb, ke = self.sweep_box()
+        return (xe - xb) * (ye - yb) * (ke - kb)
 
     def run(self, nsteps, buf, time_kernel=False):
         """Advance nsteps; ``buf`` is an optional device (nsteps, nprobe) probe buffer.
-        With ``time_kernel`` each launch is bracketed by HIP events on the fields'
-        stream and the average launch time (ms) is returned."""
+        With ``time_kernel`` HIP events bracket every interior-sweep launch on the fields'
+        stream and the average sweep duration (ms) is returned."""
         if not time_kernel:
             self._call(nsteps, buf)
             return None
-        st = self.sim.fields.stream
-        evs = []
-        for k in range(nsteps):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(st)
-            self._call(1, None if buf is None else buf[k:k + 1])
-            b.record(st)
-            evs.append((a, b))
-        torch.cuda.synchronize()
-        return sum(a.elapsed_time(b) for a, b in evs) / max(nsteps, 1)
+        return self._call(nsteps, buf, timed=True)
 
 
 class SlabStepper(FusedStepper):
@@ -148,9 +173,31 @@ class SlabStepper(FusedStepper):
         self.T = self.CX = 0
         cp = sim.cpml
         self.S = slab_planes or slab_planes_for(f, cp)
-        self.bytes_per_cell = 60.0 if f.ce_on else 48.0
 
-    def _call(self, nsteps, buf):
+    @property
+    def bytes_per_cell(self):
+        return 60.0 if self.sim.fields.ce_on else 48.0
+
+    def run(self, nsteps, buf, time_kernel=False):
+        """Advance nsteps; ``buf`` is an optional device (nsteps, nprobe) probe buffer.
+        With ``time_kernel`` each step's launches are bracketed by HIP events on the
+        fields' stream and the average step time (ms) is returned."""
+        if not time_kernel:
+            self._call(nsteps, buf)
+            return None
+        st = self.sim.fields.stream
+        evs = []
+        for k in range(nsteps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(st)
+            self._call(1, None if buf is None else buf[k:k + 1])
+            b.record(st)
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(b) for a, b in evs) / max(nsteps, 1)
+
+    def _call(self, nsteps, buf, timed=False):
         sim = self.sim
         f = sim.fields
         if buf is None and sim.probe_points:

@@ -159,16 +159,19 @@ int mifdtd_gather(const float *f, const long long *offsets, int count, float *ou
 
 /* ------------------------------------------------------------- fused step ----
  * The whole time step (core E, CPML-E on all faces, E sources, core H, CPML-H, H
- * sources, probes) in one kernel launch per step, streaming each array once (a
- * work-group of rows_per_thread+1 waves owns that many y rows of an x chunk and
- * recomputes E of the row below; rows_per_thread * planes_per_chunk bounds its LDS):
+ * sources, probes) as one single-pass sweep over the interior box of cells that carry
+ * no CPML term (each array streamed once; a work-group of rows_per_thread+1 waves owns
+ * that many y rows of an x chunk and recomputes E of the row below;
+ * rows_per_thread * planes_per_chunk bounds its LDS), between an E pass and an H pass
+ * over the CPML shell around it (three launches per step with CPML, one without):
  * state of step n is read from e[] / psiE*[], step n+1 is written to e_alt[] /
  * psiE*_alt[] (every cell, walls copied), and the two sets swap every step.  psiH is
  * updated in place.  Same per-cell arithmetic as update_e/update_h + the CPML and
  * source calls above (values bit-identical; masked zero lanes may differ in sign).
  * No reference counterpart: it fuses the reference step loop of
  * KU/cpu/test/020-dielectric-cpml.py:58-73 (E, CPML-E, source, H, CPML-H).
- * Limits: CPML on all six faces or none, CH = 0.5 (no CH arrays),
+ * Limits: CPML on all six faces or none, CH = 0.5 (no CH arrays), with CPML and
+ * ce_mat the float ce[] arrays too (the shell passes read them),
  * at most MIFDTD_FUSED_MAX_SRC sources and MIFDTD_FUSED_MAX_PROBE probes.
  * Components: 0..5 = ex, ey, ez, hx, hy, hz.  Face index f = 2*axis + side.
  * MIFDTD_PSI_PACKED layout of psiE / psiE_alt / psiH (floats): for f = 0..5, psi1 then
@@ -197,6 +200,11 @@ typedef struct {
     int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
     int rows_per_thread;  /* fused: y rows owned by one work-group, 1..7 (T+1 waves) */
     int planes_per_chunk; /* fused: x planes marched by one work-group */
+    /* fused only: CE as uint8 material indices (nx, ny, nz_pitch) per component into
+     * ce_lut[3][256] (the float CE values); when ce_mat[0] is set, ce[] is ignored.
+     * Index arrays must be 4-byte aligned. */
+    const unsigned char *ce_mat[3];
+    const float *ce_lut;
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
@@ -204,6 +212,12 @@ typedef struct {
  * nsteps & 1: 1 means the current state is in e_alt[] / psiE*_alt[]. */
 int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
                      float *probe_series, int *parity, mifdtd_stream_t stream);
+/* The same, with HIP events around each interior-sweep launch: *sweep_ms receives the
+ * average sweep duration (waits for the stream to reach the last sweep).  No reference
+ * counterpart (measurement). */
+int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
+                           float *probe_series, int *parity, mifdtd_stream_t stream,
+                           float *sweep_ms);
 
 /* ------------------------------------------------------------- slab step ----
  * The same whole step, in place, as two launches per slab of slab_planes consecutive

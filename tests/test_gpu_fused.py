@@ -89,6 +89,36 @@ def test_fused_matches_inplace(ns, npml, T, CX, steps):
                 assert same_values(sims[0].cpml.psiH[key][c].cpu().numpy(), sims[1].cpml.psiH[key][c].cpu().numpy()), ("psiH", key, c)
 
 
+@pytest.mark.parametrize("ns,npml,T,CX,steps", [
+    ((24, 20, 32), 0, 3, 5, 3),
+    ((30, 26, 40), 4, 7, 8, 5),
+    ((29, 31, 57), 5, 5, 16, 4),
+    ((12, 13, 1000), 4, 2, 5, 3),
+])
+def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
+    """CE held as uint8 material indices + table (Fields.compress_ce) on the fused engine:
+    equal to the in-place engine reading the float CE arrays."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", T, CX)]
+    nx, ny, nz = ns
+    rng = np.random.default_rng(5)
+    vals = (0.5 / np.array([1.0, 1.7, 2.25, 4.0, 6.5, 9.0, 12.0])).astype(np.float32)
+    mats = [rng.integers(0, len(vals), (nx, ny, nz)) for _ in range(3)]
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    for s in sims:
+        _fill(s, 11)
+        s.fields.set_ces(*[vals[m] for m in mats])
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_probe("ey", (nx // 2, ny // 2 + 1, nz // 2))
+    assert sims[1].fields.compress_ce()
+    series = [s.run(steps) for s in sims]
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert same_values(a, b), (name, np.abs(a - b).max())
+    assert same_values(series[0], series[1])
+
+
 def test_fused_continues_across_calls_and_engines():
     """Steps split over several run() calls (odd/even parity, buffer swaps) and an engine
     switch mid-run equal one continuous run."""
