@@ -6,7 +6,9 @@ non-overlapping dielectric spheres (radius U[8,40], eps_r U[1,12], background 1)
 Gaussian-modulated pulse, Ez probe; synthetic data.  One step = E update + CPML-E +
 source + H update + CPML-H (+ probe gather).
 
-    python bench.py [--steps K] [--warmup W] [--engine kemp|slab|fused] [--n 1024]
+    python bench.py [--steps K] [--warmup W] [--engine fused|kemp|slab] [--n 1024]
+        (default engine fused: single-pass interior sweep + CPML shell passes, CE as
+         uint8 material indices; kemp: the in-place drop-in kernels)
     python bench.py --workload s4 [--n 512]   (4th-order symplectic, 100 spheres, CPML 10,
                                               Courant 0.9 x stability limit)
     python bench.py --workload batch [--steps 300] [--tune]   (64 uneven Yee+CPML 8
@@ -37,10 +39,12 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel (GB), from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a plain 1 GiB copy gives), keyed by
-# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt and
-# profiles/update_e_lut_yee_cpml12_1024cube_hbm_counters.txt.
+# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt,
+# profiles/update_e_lut_yee_cpml12_1024cube_hbm_counters.txt and
+# profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep).
 MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
-                       ("kemp", 1024, 12, "uint8 index + table"): 53.24}
+                       ("kemp", 1024, 12, "uint8 index + table"): 53.24,
+                       ("fused", 1024, 12, "uint8 index + table"): 55.76}
 
 
 def parse():
@@ -51,7 +55,7 @@ def parse():
     p.add_argument("--n", type=int, default=1024)
     p.add_argument("--npml", type=int, default=12)
     p.add_argument("--spheres", type=int, default=200)
-    p.add_argument("--engine", default="kemp", choices=("fused", "kemp", "slab"))
+    p.add_argument("--engine", default="fused", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint"))
