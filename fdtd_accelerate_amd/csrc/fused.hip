@@ -471,10 +471,13 @@ __global__ __launch_bounds__(512, 4) void k_fused(
 // CPML shell: every cell outside the interior box, as up to six boxes (x low/high: all
 // rows; y low/high: the box's planes; z low/high: the box's planes and rows), each
 // (planes [i0,i1), rows [j0,j1), quads [q0,q1)) with a thread per quad; start[b] is the
-// first thread of box b.
+// first thread of box b.  Quads run fastest, then the box's shorter one of planes and
+// rows, so that the i+1 / i-1 and j+1 / j-1 neighbour rows a quad reads are read by
+// nearby threads too (L2 hits).
 struct ShellBoxes {
     int n;
     int i0[6], i1[6], j0[6], j1[6], q0[6], q1[6];
+    int i_inner[6];  // planes before rows in the thread order
     unsigned start[7];
 };
 
@@ -484,10 +487,16 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
     while (b + 1 < s.n && f >= s.start[b + 1]) ++b;
     unsigned r = f - s.start[b];
     const unsigned nq = (unsigned)(s.q1[b] - s.q0[b]), nj = (unsigned)(s.j1[b] - s.j0[b]);
+    const unsigned ni = (unsigned)(s.i1[b] - s.i0[b]);
     const unsigned q = r % nq;
     r /= nq;
-    j = s.j0[b] + (int)(r % nj);
-    i = s.i0[b] + (int)(r / nj);
+    if (s.i_inner[b]) {
+        i = s.i0[b] + (int)(r % ni);
+        j = s.j0[b] + (int)(r / ni);
+    } else {
+        j = s.j0[b] + (int)(r % nj);
+        i = s.i0[b] + (int)(r / nj);
+    }
     k0 = 4 * (s.q0[b] + (int)q);
 }
 
@@ -623,6 +632,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             if (i1 <= i0 || j1 <= j0 || q1 <= q0) return;
             const int b = sb.n++;
             sb.i0[b] = i0; sb.i1[b] = i1; sb.j0[b] = j0; sb.j1[b] = j1; sb.q0[b] = q0; sb.q1[b] = q1;
+            sb.i_inner[b] = (i1 - i0) < (j1 - j0);
             sb.start[b] = (unsigned)total;
             total += (unsigned long long)(i1 - i0) * (j1 - j0) * (q1 - q0);
         };

@@ -39,14 +39,16 @@ def _bwd(f, dim):
 class TorchYee:
     """(nx, ny, nzp) float64 fields, pad lanes = nzp - nz, CPML npml on all faces."""
 
-    def __init__(self, nx, ny, nz, pad=0, npml=0, R=1e-6, m=4):
+    def __init__(self, nx, ny, nz, pad=0, npml=0, R=1e-6, m=4, device="cpu"):
         self.nx, self.ny, self.nz, self.pad, self.npml = nx, ny, nz, pad, npml
         nzp = nz + pad
         self.nzp = nzp
-        I = torch.arange(nx).view(-1, 1, 1)
-        J = torch.arange(ny).view(1, -1, 1)
-        K = torch.arange(nzp).view(1, 1, -1)
-        full = torch.ones((nx, ny, nzp), dtype=torch.bool)
+        self.device = torch.device(device)
+        dev = self.device
+        I = torch.arange(nx, device=dev).view(-1, 1, 1)
+        J = torch.arange(ny, device=dev).view(1, -1, 1)
+        K = torch.arange(nzp, device=dev).view(1, 1, -1)
+        full = torch.ones((nx, ny, nzp), dtype=torch.bool, device=dev)
         # rows written by the core and lanes kept (valid) inside those rows
         self.erow = [(J < ny - 1) & full, (I < nx - 1) & full, (I < nx - 1) & (J < ny - 1) & full]
         self.elane = [(K < nz - 1) & full, (K < nz - 1) & full, (K < nz) & full]
@@ -56,14 +58,16 @@ class TorchYee:
         self.hvalid = [r & l for r, l in zip(self.hrow, self.hlane)]
         if npml:
             bE, aE, bH, aH = O.cpml_tables(npml, m, R, dt=0.5, dtype=np.float64)
-            self.tab = {k: torch.tensor(v, dtype=DT) for k, v in (("bE", bE), ("aE", aE), ("bH", bH), ("aH", aH))}
+            self.tab = {k: torch.tensor(v, dtype=DT, device=dev) for k, v in (("bE", bE), ("aE", aE), ("bH", bH), ("aH", aH))}
 
     def zeros(self):
-        z = lambda: torch.zeros((self.nx, self.ny, self.nzp), dtype=DT)
+        z = lambda: torch.zeros((self.nx, self.ny, self.nzp), dtype=DT, device=self.device)
         st = [z() for _ in range(6)]
         if self.npml:
-            st += [torch.zeros(self._psi_shape(ax), dtype=DT) for ax in range(3) for _ in range(2) for _ in range(2)]
-            st += [torch.zeros(self._psi_shape(ax), dtype=DT) for ax in range(3) for _ in range(2) for _ in range(2)]
+            st += [torch.zeros(self._psi_shape(ax), dtype=DT, device=self.device) for ax in range(3) for _ in range(2)
+                   for _ in range(2)]
+            st += [torch.zeros(self._psi_shape(ax), dtype=DT, device=self.device) for ax in range(3) for _ in range(2)
+                   for _ in range(2)]
         return st
 
     def _psi_shape(self, ax):
@@ -79,7 +83,7 @@ class TorchYee:
         else:
             q = n - self.npml + p if side else p
             t = self.npml + 2 + p if side else p + 1
-        return torch.tensor(q), torch.tensor(t)
+        return torch.tensor(q, device=self.device), torch.tensor(t, device=self.device)
 
     @staticmethod
     def _bc(v, ax):
@@ -140,7 +144,7 @@ class TorchYee:
 
 
 def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources, seg=10, outer=None,
-                   want_grad=True):
+                   want_grad=True, progress=None):
     """J and dJ/deps (float64) for the design box ``box`` = (i0, j0, k0, di, dj, dk).
 
     ce_background: 3 float64 CE tensors; on the box, CE = 0.5/eps on all components.
@@ -149,7 +153,8 @@ def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources,
     (one saved state per segment).  With ``outer`` the forward keeps a state every
     ``outer`` steps only and the reverse sweep recomputes each outer segment (with
     ``seg``-step checkpoints inside), carrying the adjoint of the state between
-    segments: the gradient is the same, the peak memory ~ (outer/seg + 1) states."""
+    segments: the gradient is the same, the peak memory ~ (outer/seg + 1) states.
+    ``progress``: optional function called with a message per outer segment."""
     i0, j0, k0, di, dj, dk = box
     eps = torch.as_tensor(eps, dtype=DT).clone().requires_grad_(want_grad)
 
@@ -159,13 +164,13 @@ def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources,
             base = ce_background[c]
             padding = (k0, base.shape[2] - k0 - dk, j0, base.shape[1] - j0 - dj, i0, base.shape[0] - i0 - di)
             inner = torch.nn.functional.pad(0.5 / eps, padding)
-            mask = torch.nn.functional.pad(torch.ones((di, dj, dk), dtype=DT), padding)
+            mask = torch.nn.functional.pad(torch.ones((di, dj, dk), dtype=DT, device=eps.device), padding)
             ce.append(base * (1 - mask) + inner)
         return ce
 
     def run_seg(ce, n_start, n_end, *state):
         st = list(state)
-        acc = torch.zeros((), dtype=DT)
+        acc = torch.zeros((), dtype=DT, device=eps.device)
         for n in range(n_start, n_end):
             st = model.step(st, ce, sources(n))
             if n >= n0:
@@ -173,7 +178,7 @@ def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources,
         return (acc, *st)
 
     def chunked(ce, s0, s1, st, grad):
-        J = torch.zeros((), dtype=DT)
+        J = torch.zeros((), dtype=DT, device=eps.device)
         for s in range(s0, s1, seg):
             e = min(s + seg, s1)
             if grad:
@@ -206,13 +211,15 @@ def objective_grad(model, ce_background, box, eps, nsteps, n0, monitor, sources,
         del ce0, st
     lam = None
     for a, b in reversed(list(zip(bounds[:-1], bounds[1:]))):
+        if progress:
+            progress(f"reverse sweep: steps {a}..{b - 1}")
         st0 = [t.clone().requires_grad_(True) for t in saved.pop(a)]
         ce = make_ce()
         Jp, st1 = chunked(ce, a, b, st0, True)
         outs, grads = [], []
         if Jp.requires_grad:  # segments before the objective window add nothing to J
             outs.append(Jp)
-            grads.append(torch.ones((), dtype=DT))
+            grads.append(torch.ones((), dtype=DT, device=eps.device))
         if lam is not None:
             outs += st1
             grads += lam

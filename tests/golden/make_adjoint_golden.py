@@ -12,7 +12,12 @@ of the config-4 design problem (run here, on the CPU, with the oracle):
 * the float64 gradient on the whole box from torch.autograd through the float64
   restatement oracle/torch_ref.py (checkpointed).
 
-    python tests/golden/make_adjoint_golden.py [--quick]
+    python tests/golden/make_adjoint_golden.py [--quick] [--device cuda]
+
+The finite differences run on the CPU (the C oracle) and are kept in <out>.fd.npz until
+the script ends, so a rerun (e.g. with --device cuda for the autograd stage, whose
+float64 graph of the 160^3 problem needs more host memory than a small machine has)
+skips them.
 """
 import argparse
 import os
@@ -76,6 +81,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--out", default=os.path.join(HERE, "adjoint_config4.npz"))
+    ap.add_argument("--device", default="cpu", help="where the float64 autograd runs (cpu or cuda)")
     a = ap.parse_args()
     cfg = config(a.quick)
     torch.set_num_threads(os.cpu_count() or 8)
@@ -104,15 +110,17 @@ def main():
         np.savez(stage, J64=J, h=h, hs=np.array(hs), sweep=sweep, fd=fds)
         print("fd done", flush=True)
     n, npml, box = cfg["n"], cfg["npml"], cfg["box"]
-    m = TorchYee(n, n, n, 0, npml)
-    ceb = [torch.full((n, n, n), 0.5, dtype=torch.float64) for _ in range(3)]
-    mask = torch.zeros((n, n, n), dtype=torch.float64)
+    dev = torch.device(a.device)
+    m = TorchYee(n, n, n, 0, npml, device=dev)
+    ceb = [torch.full((n, n, n), 0.5, dtype=torch.float64, device=dev) for _ in range(3)]
+    mask = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
     mask[cfg["src"], npml:n - npml, npml:n - npml] = 1
     n0 = cfg["nsteps"] - cfg["window"] + 1
-    Jt, g = objective_grad(m, ceb, box, torch.tensor(eps64), cfg["nsteps"], n0, cfg["mon"],
-                           lambda s: [(2, O.pulse(s) * mask)], seg=2, outer=None if a.quick else 50)
+    Jt, g = objective_grad(m, ceb, box, torch.tensor(eps64, device=dev), cfg["nsteps"], n0, cfg["mon"],
+                           lambda s: [(2, O.pulse(s) * mask)], seg=2, outer=None if a.quick else 50,
+                           progress=lambda msg: print(msg, flush=True))
     print(f"autograd J {Jt:.12g} (oracle {J:.12g})", flush=True)
-    g = g.numpy()
+    g = g.cpu().numpy()
     rel = np.abs(g[tuple(vox.T)] - fds) / np.abs(fds)
     print("autograd vs fd max rel", rel.max(), flush=True)
     np.savez_compressed(a.out, config=np.array(repr(cfg)), eps_seed=EPS_SEED, vox_seed=VOX_SEED,
