@@ -33,16 +33,19 @@ def _load():
 
 
 def lds_fits(T, CX, table):
-    """Whether the fused kernel's LDS (E and H row exchange 2 x 2 x T x 2 x 256 floats,
-    the k-1 carry (CX+1)(T+1)2 floats, the 3 x 256 CE table) fits its 64 KiB."""
-    return (768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2 <= 16384
+    """Whether the sweep's LDS (E and H row exchange 2 x 2 x T x 2 x 256 floats, the k-1
+    carry (CX+1)(T+1)2 floats, the 3 x 256 CE table, ~0.4 KiB of source/probe tables)
+    fits the 160 KiB of one gfx950 work-group."""
+    return 4 * ((768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2) + 512 <= 160 * 1024
 
 
 def heuristic(nx, ny, nzp, npml):
-    """T owned rows per block (y halo overhead 1/T; at most 7: the block has T+1 waves),
-    CX planes per chunk (seed-plane overhead 1/CX), keeping >= 1024 blocks in flight
-    (4 per CU on 256 CUs)."""
-    T = min(7, max(1, ny // 8))
+    """T owned rows per block (y halo overhead 1/T; the block has T+1 waves), CX planes
+    per chunk (seed-plane overhead 1/CX), keeping >= 1024 blocks.  T = 9 measured best
+    at 1024^3 (10.87 ms per sweep against 11.40 for T = 7 and 10.98 for T = 11, CX = 64,
+    interleaved on one MI355X): a smaller halo share outweighs the lower occupancy of
+    one 10-wave work-group per CU."""
+    T = min(9, max(1, ny // 8))
     nrb = -(-ny // T)
     cx = min(nx, 64)
     while cx > 8 and nrb * (-(-nx // cx)) < 1024:

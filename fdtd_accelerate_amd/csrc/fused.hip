@@ -39,7 +39,8 @@
 
 namespace mifdtd {
 
-constexpr int kMaxRows = 7;  // owned rows per block (a block has rows+1 waves)
+constexpr int kMaxRows = 15;  // owned rows per block (a block has rows+1 waves, at most 1024 threads)
+constexpr size_t kMaxLds = 160 * 1024;  // LDS of one work-group on gfx950
 constexpr int kMaxSrc = 4;
 constexpr int kMaxProbe = 8;
 
@@ -269,7 +270,7 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
 // parity.  Source and probe tables are copied to LDS once per block and consulted only
 // by blocks whose rows they touch.
 template <int CEM>
-__global__ __launch_bounds__(512, 4) void k_fused(
+__global__ __launch_bounds__(1024, 4) void k_fused(
     FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
@@ -621,7 +622,9 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     g.nblocks = (int)nbl;
     const size_t lds_bytes = sizeof(float) * ((cem == kCeTable ? 768 : 0) + (size_t)2 * 2 * T * 2 * 256 +
                                               (size_t)(CX + 1) * (T + 1) * 2);
-    if (lds_bytes > 64 * 1024) return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS (> 64 KiB)", T, CX, lds_bytes);
+    if (lds_bytes + sizeof(FSrc) + sizeof(FProbe) > kMaxLds)
+        return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS (> %zu)", T, CX, lds_bytes,
+                    kMaxLds);
 
     // shell boxes (only with CPML)
     ShellBoxes sb{};

@@ -198,7 +198,7 @@ typedef struct {
     int nprobe;
     int probe_comp[MIFDTD_FUSED_MAX_PROBE];
     int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
-    int rows_per_thread;  /* fused: y rows owned by one work-group, 1..7 (T+1 waves) */
+    int rows_per_thread;  /* fused: y rows owned by one work-group, 1..15 (T+1 waves) */
     int planes_per_chunk; /* fused: x planes marched by one work-group */
     /* fused only: CE as uint8 material indices (nx, ny, nz_pitch) per component into
      * ce_lut[3][256] (the float CE values); when ce_mat[0] is set, ce[] is ignored.

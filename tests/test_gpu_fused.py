@@ -61,6 +61,9 @@ def _fill(sim, seed):
     ((12, 13, 1000), 4, 3, 5, 3),
     ((26, 70, 300), 5, 7, 9, 2),
     ((20, 30, 300), 4, 5, 7, 3),
+    ((26, 70, 300), 5, 11, 9, 2),
+    ((30, 40, 48), 4, 15, 8, 3),
+    ((17, 45, 530), 3, 13, 64, 2),
 ])
 def test_fused_matches_inplace(ns, npml, T, CX, steps):
     """Random fields (walls included) + CE + CPML + a point and a plane source + probes:
@@ -94,6 +97,7 @@ def test_fused_matches_inplace(ns, npml, T, CX, steps):
     ((30, 26, 40), 4, 7, 8, 5),
     ((29, 31, 57), 5, 5, 16, 4),
     ((12, 13, 1000), 4, 2, 5, 3),
+    ((29, 40, 300), 5, 15, 16, 3),
 ])
 def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
     """CE held as uint8 material indices + table (Fields.compress_ce) on the fused engine:
