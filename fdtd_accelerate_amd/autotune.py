@@ -58,9 +58,10 @@ def fused_tiles(nx, ny, nzp, npml):
     return t if t is not None else heuristic(nx, ny, nzp, npml)
 
 
-def tune_fused(sim, candidates=((3, 32), (3, 64), (5, 32), (5, 64), (7, 32), (7, 64), (7, 128)), steps=5, save=True):
-    """Time each (T, CX) on ``sim`` (a YeeSim with engine='fused') and record the best.
-    Runs real steps (the simulation advances by len(candidates) * (steps + 1) steps)."""
+def tune_fused(sim, candidates=((7, 64), (9, 64), (9, 96), (11, 64), (11, 128), (13, 64), (15, 96)), steps=5, save=True):
+    """Time the interior sweep for each (T, CX) on ``sim`` (a YeeSim with engine='fused')
+    and record the best.  Runs real steps (the simulation advances by
+    len(candidates) * (steps + 1) steps)."""
     from .fused import FusedStepper
     f = sim.fields
     best = None
