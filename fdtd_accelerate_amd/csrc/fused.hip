@@ -491,29 +491,39 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
     const unsigned ni = (unsigned)(s.i1[b] - s.i0[b]);
     const unsigned q = r % nq;
     r /= nq;
-    if (s.i_inner[b]) {
-        i = s.i0[b] + (int)(r % ni);
-        j = s.j0[b] + (int)(r / ni);
-    } else {
-        j = s.j0[b] + (int)(r % nj);
-        i = s.i0[b] + (int)(r / nj);
-    }
+    // selects rather than branches on i_inner (which the compiler turns into stores of
+    // i and j through a stack address)
+    const bool ii = s.i_inner[b] != 0;
+    const unsigned inner = ii ? ni : nj;
+    const int a = (int)(r % inner), o = (int)(r / inner);
+    i = s.i0[b] + (ii ? a : o);
+    j = s.j0[b] + (ii ? o : a);
     k0 = 4 * (s.q0[b] + (int)q);
 }
 
-template <bool HC>
+// CE: CEM as in the sweep (kCeTable stages the 3 x 256 table in LDS per block).
+template <int CEM>
 __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigned total, const float *ex, const float *ey,
                                                  const float *ez, float *oex, float *oey, float *oez,
                                                  const float *__restrict__ hx, const float *__restrict__ hy,
                                                  const float *__restrict__ hz, const float *__restrict__ cex,
                                                  const float *__restrict__ cey, const float *__restrict__ cez,
+                                                 const unsigned char *__restrict__ mx,
+                                                 const unsigned char *__restrict__ my,
+                                                 const unsigned char *__restrict__ mz, const float *__restrict__ lut,
                                                  SCpml cp, SSrc src, SProbe prb)
 {
+    __shared__ float tab[CEM == kCeTable ? 768 : 1];
+    if (CEM == kCeTable) {
+        for (int t = threadIdx.x; t < 768; t += 256) tab[t] = lut[t];
+        __syncthreads();
+    }
     const unsigned f = blockIdx.x * 256u + threadIdx.x;
     if (f >= total) return;
     int i, j, k0;
     shell_quad(sb, f, i, j, k0);
-    e_quad_at<HC, true>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb);
+    e_quad_at<CEM, true>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my, mz,
+                         tab);
 }
 
 __global__ __launch_bounds__(256) void k_shell_h(SGeom g, ShellBoxes sb, unsigned total, const float *__restrict__ oex,
@@ -582,8 +592,6 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (!d->psiE || !d->psiE_alt || !d->psiH) return fail(MIFDTD_ENULL, "packed psi buffers are NULL");
         if (!aligned16(d->psiE) || !aligned16(d->psiE_alt) || !aligned16(d->psiH))
             return fail(MIFDTD_EALIGN, "packed psi buffers not 16-byte aligned");
-        if (tbl && !hc)
-            return fail(MIFDTD_ENULL, "with CPML the float CE arrays are needed next to the material indices");
         scp.npml = np_;
         scp.bE = d->bE; scp.aE = d->aE; scp.bH = d->bH; scp.aH = d->aH;
         scp.ph = d->psiH;
@@ -712,14 +720,15 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (pml) {
             scp.pe = pin;
             scp.pe_out = pout;
-            if (hc)
-                k_shell_e<true><<<shell_blocks, 256, 0, st>>>(sg, sb, (unsigned)total, in[0], in[1], in[2], out[0],
-                                                              out[1], out[2], in[3], in[4], in[5], d->ce[0], d->ce[1],
-                                                              d->ce[2], scp, ssrc, sprb);
-            else
-                k_shell_e<false><<<shell_blocks, 256, 0, st>>>(sg, sb, (unsigned)total, in[0], in[1], in[2], out[0],
-                                                               out[1], out[2], in[3], in[4], in[5], nullptr, nullptr,
-                                                               nullptr, scp, ssrc, sprb);
+#define MIFDTD_SHELL_E_LAUNCH(CEV)                                                                               \
+    k_shell_e<CEV><<<shell_blocks, 256, 0, st>>>(sg, sb, (unsigned)total, in[0], in[1], in[2], out[0], out[1],   \
+                                                 out[2], in[3], in[4], in[5], d->ce[0], d->ce[1], d->ce[2],      \
+                                                 d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, scp, ssrc, \
+                                                 sprb)
+            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable);
+            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays);
+            else MIFDTD_SHELL_E_LAUNCH(kCeConst);
+#undef MIFDTD_SHELL_E_LAUNCH
             MIFDTD_CHECK_LAUNCH("CPML shell E launch");
         }
         if (sweep) {

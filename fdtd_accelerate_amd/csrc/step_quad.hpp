@@ -111,15 +111,31 @@ __device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int
 // memory round trip even on CPML cells (psi pointers may alias the fields as far as the
 // compiler knows, so a load placed after a field store would wait for it).
 
+// CE of a quad as uint8 material indices m[0..3] into a 256-entry float table.
+__device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m, const float *tab)
+{
+    const uchar4 t = *reinterpret_cast<const uchar4 *>(m);
+    Q4 q;
+    q.v[0] = tab[t.x];
+    q.v[1] = tab[t.y];
+    q.v[2] = tab[t.z];
+    q.v[3] = tab[t.w];
+    return q;
+}
+
 // E pass on the quad (i, j, k0..k0+3): reads E from exi.. and writes it to exo.. (the
 // same arrays for an in-place update, the next-step buffers for a ping-pong one); psi E
-// is read from cp.pe and written to cp.pe_out.
-template <bool HC, bool PML>
+// is read from cp.pe and written to cp.pe_out.  CE: CM = 0 the constant 0.5, 1 the float
+// arrays cex.., 2 the uint8 material indices mx.. into tab ([3][256] floats, the same
+// values as the arrays, so the products are identical).
+template <int CM, bool PML>
 __device__ __forceinline__ void e_quad_at(
     const SGeom &g, int i, int j, int k0, const float *exi, const float *eyi, const float *ezi, float *exo,
     float *eyo, float *ezo, const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
-    const SCpml &cp, const SSrc &src, const SProbe &prb)
+    const SCpml &cp, const SSrc &src, const SProbe &prb, const unsigned char *__restrict__ mx = nullptr,
+    const unsigned char *__restrict__ my = nullptr, const unsigned char *__restrict__ mz = nullptr,
+    const float *tab = nullptr)
 {
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
     const size_t id = ((size_t)i * ny + j) * nzp + k0;
@@ -130,9 +146,9 @@ __device__ __forceinline__ void e_quad_at(
     // ---------------------------------------------------------------- loads
     const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
     Q4 e0 = ld4(exi + id), e1 = ld4(eyi + id), e2 = ld4(ezi + id);
-    const Q4 c0 = HC ? ld4(cex + id) : splat4(0.5f);
-    const Q4 c1 = HC ? ld4(cey + id) : splat4(0.5f);
-    const Q4 c2 = HC ? ld4(cez + id) : splat4(0.5f);
+    const Q4 c0 = CM == 2 ? ce_from_table(mx + id, tab) : CM ? ld4(cex + id) : splat4(0.5f);
+    const Q4 c1 = CM == 2 ? ce_from_table(my + id, tab + 256) : CM ? ld4(cey + id) : splat4(0.5f);
+    const Q4 c2 = CM == 2 ? ce_from_table(mz + id, tab + 512) : CM ? ld4(cez + id) : splat4(0.5f);
     const Q4 hzU = jy ? ld4(hz + id + nzp) : z4;   // hz(j+1)
     const Q4 hxU = jy ? ld4(hx + id + nzp) : z4;   // hx(j+1)
     const float hyn = jy ? hy[id + 4] : 0.f;       // hy(k0+4): next quad / next row start

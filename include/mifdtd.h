@@ -170,8 +170,7 @@ int mifdtd_gather(const float *f, const long long *offsets, int count, float *ou
  * source calls above (values bit-identical; masked zero lanes may differ in sign).
  * No reference counterpart: it fuses the reference step loop of
  * KU/cpu/test/020-dielectric-cpml.py:58-73 (E, CPML-E, source, H, CPML-H).
- * Limits: CPML on all six faces or none, CH = 0.5 (no CH arrays), with CPML and
- * ce_mat the float ce[] arrays too (the shell passes read them),
+ * Limits: CPML on all six faces or none, CH = 0.5 (no CH arrays),
  * at most MIFDTD_FUSED_MAX_SRC sources and MIFDTD_FUSED_MAX_PROBE probes.
  * Components: 0..5 = ex, ey, ez, hx, hy, hz.  Face index f = 2*axis + side.
  * MIFDTD_PSI_PACKED layout of psiE / psiE_alt / psiH (floats): for f = 0..5, psi1 then
@@ -201,8 +200,8 @@ typedef struct {
     int rows_per_thread;  /* fused: y rows owned by one work-group, 1..15 (T+1 waves) */
     int planes_per_chunk; /* fused: x planes marched by one work-group */
     /* fused only: CE as uint8 material indices (nx, ny, nz_pitch) per component into
-     * ce_lut[3][256] (the float CE values); when ce_mat[0] is set, ce[] is ignored.
-     * Index arrays must be 4-byte aligned. */
+     * ce_lut[3][256] (the float CE values), read by the sweep and the shell E pass;
+     * when ce_mat[0] is set, ce[] is ignored.  Index arrays must be 4-byte aligned. */
     const unsigned char *ce_mat[3];
     const float *ce_lut;
 } mifdtd_fused_desc;
