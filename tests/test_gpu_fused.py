@@ -123,6 +123,38 @@ def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
     assert same_values(series[0], series[1])
 
 
+def test_fused_field_arena(monkeypatch):
+    """Both field sets carved from one allocation with staggered starts (the layout the
+    fused engine uses for arrays of 1 GiB and more, forced here on a small grid): same
+    values as the in-place engine over odd and even step counts, all twelve arrays in
+    one storage at k * (size + gap) bytes, gap = size/8 rounded up to 2 MiB."""
+    from fdtd_accelerate_amd import fused as F
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    monkeypatch.setattr(F, "_ARENA_MIN_BYTES", 1)
+    ns, npml = (30, 26, 40), 4
+    nx, ny, nz = ns
+    sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", 3, 8)]
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    for s in sims:
+        _fill(s, 7)
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_probe("hy", (nx // 2, ny // 2, nz // 2 + 1))
+    for steps in (3, 4):
+        series = [s.run(steps) for s in sims]
+        assert same_values(series[0], series[1])
+        for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            a = sims[0].field(name).cpu().numpy()
+            b = sims[1].field(name).cpu().numpy()
+            assert same_values(a, b), (steps, name, np.abs(a - b).max())
+    st = sims[1]._fused
+    arrays = sims[1].fields.ehs + st.alt
+    base = arrays[0].untyped_storage().data_ptr()
+    assert all(t.untyped_storage().data_ptr() == base for t in arrays)
+    nbytes = arrays[0].numel() * 4
+    step = nbytes + -(-(nbytes // 8) // (2 << 20)) * (2 << 20)
+    assert sorted(t.data_ptr() - base for t in arrays) == [k * step for k in range(12)]
+
+
 def test_fused_continues_across_calls_and_engines():
     """Steps split over several run() calls (odd/even parity, buffer swaps) and an engine
     switch mid-run equal one continuous run."""
