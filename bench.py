@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep).
 MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
                        ("kemp", 1024, 12, "uint8 index + table"): 53.24,
-                       ("fused", 1024, 12, "uint8 index + table"): 54.05}
+                       ("fused", 1024, 12, "uint8 index + table"): 54.01}
 
 
 def parse():
@@ -319,7 +319,7 @@ def run_adjoint(args):
         "metric": METRIC, "value": round(n * p.nsteps / (fwd * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
         "steps": p.nsteps, "warmup": 1, "ms_per_step": round(fwd / p.nsteps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "adjoint gradient: 160^3 Yee, CPML 10, 64^3 design box eps_r U[1,4], Ez plane source x=20, "
+        "config": {"workload": "adjoint gradient: 160^3 Yee, CPML 10, 64^3 design box eps_r U[1,4], Ez line source on x=20 (y=80, all interior z), "
                                "J = sum of |E|^2 on x=140 over steps 601-800, checkpoints every 50", "cells": n},
         "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3),
         "J": J, "grad_norm": float(g.norm()),

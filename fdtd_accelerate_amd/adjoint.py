@@ -46,9 +46,10 @@ class DesignProblem:
         if not (0 < i0 and i0 + di < nx and 0 < j0 and j0 + dj < ny - 1 and 0 < k0 and k0 + dk < nz - 1):
             raise ValueError(f"design box {self.box} must lie inside the grid {n}")
         self.sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e")
-        # Ez soft source on the plane x = src_plane over the non-CPML interior
+        # Ez soft line source on the plane x = src_plane: the line y = ny/2 along z over the
+        # non-CPML interior
         if src_box is None:
-            src_box = ((src_plane, npml, npml), (src_plane, ny - npml - 1, nz - npml - 1))
+            src_box = ((src_plane, ny // 2, npml), (src_plane, ny // 2, nz - npml - 1))
         self.src = self.sim.add_source("ez", src_box[0], src_box[1], self.tfunc)
         self.stepper = SlabStepper(self.sim)
         f = self.sim.fields

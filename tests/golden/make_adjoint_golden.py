@@ -2,8 +2,8 @@
 of the config-4 design problem (run here, on the CPU, with the oracle):
 
   160^3 grid, CPML 10, 64^3 design box at the centre with eps_r ~ U[1,4] (float32,
-  numpy.random.default_rng(EPS_SEED)), Ez soft source on the plane x = 20 over the
-  non-CPML interior driven by the config-0 pulse, objective J = sum over steps 601..800
+  numpy.random.default_rng(EPS_SEED)), Ez soft line source on the plane x = 20 (the line
+  y = 80 along z over the non-CPML interior) driven by the config-0 pulse, objective J = sum over steps 601..800
   of sum over the plane x = 140 of |E|^2.
 
 * central finite differences of the float64 C oracle (OracleSim) at 32 seeded voxels,
@@ -58,7 +58,7 @@ def oracle_J(cfg, eps64):
     for c in ces:
         c[i0:i0 + di, j0:j0 + dj, k0:k0 + dk] = 0.5 / eps64
     o = O.OracleSim(n, n, n, 0, npml, np.float64, ces=ces)
-    o.add_source("ez", (cfg["src"], npml, npml), (cfg["src"], n - npml - 1, n - npml - 1), O.pulse)
+    o.add_source("ez", (cfg["src"], n // 2, npml), (cfg["src"], n // 2, n - npml - 1), O.pulse)
     n0 = cfg["nsteps"] - cfg["window"] + 1
     J = 0.0
     for s in range(1, cfg["nsteps"] + 1):
@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--out", default=os.path.join(HERE, "adjoint_config4.npz"))
     ap.add_argument("--device", default="cpu", help="where the float64 autograd runs (cpu or cuda)")
+    ap.add_argument("--fd-only", action="store_true", help="stop after the finite-difference stage")
     a = ap.parse_args()
     cfg = config(a.quick)
     torch.set_num_threads(os.cpu_count() or 8)
@@ -109,12 +110,14 @@ def main():
         fds = np.array([fd(cfg, eps64, v, h) for v in vox])
         np.savez(stage, J64=J, h=h, hs=np.array(hs), sweep=sweep, fd=fds)
         print("fd done", flush=True)
+    if a.fd_only:
+        return
     n, npml, box = cfg["n"], cfg["npml"], cfg["box"]
     dev = torch.device(a.device)
     m = TorchYee(n, n, n, 0, npml, device=dev)
     ceb = [torch.full((n, n, n), 0.5, dtype=torch.float64, device=dev) for _ in range(3)]
     mask = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
-    mask[cfg["src"], npml:n - npml, npml:n - npml] = 1
+    mask[cfg["src"], n // 2, npml:n - npml] = 1
     n0 = cfg["nsteps"] - cfg["window"] + 1
     Jt, g = objective_grad(m, ceb, box, torch.tensor(eps64, device=dev), cfg["nsteps"], n0, cfg["mon"],
                            lambda s: [(2, O.pulse(s) * mask)], seg=2, outer=None if a.quick else 50,

@@ -36,7 +36,7 @@ def _reference(p, eps):
     m = TorchYee(f.nx, f.ny, f.nz, f.pad, p.npml)
     ceb = [torch.full((f.nx, f.ny, f.nz_pitch), 0.5, dtype=torch.float64) for _ in range(3)]
     mask = torch.zeros((f.nx, f.ny, f.nz_pitch), dtype=torch.float64)
-    mask[6, p.npml:f.ny - p.npml, p.npml:f.nz - p.npml] = 1
+    mask[6, f.ny // 2, p.npml:f.nz - p.npml] = 1  # Ez line source: x = 6, y = ny/2, interior z
     tf = lambda t: O.pulse(t, 25.0, 8.0, 12.0)
     return objective_grad(m, ceb, p.box, torch.tensor(eps.astype(np.float64)), p.nsteps, p.n0, p.monitor,
                           lambda n: [(2, tf(n) * mask)])
