@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep).
 MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
                        ("kemp", 1024, 12, "uint8 index + table"): 53.24,
-                       ("fused", 1024, 12, "uint8 index + table"): 54.01}
+                       ("fused", 1024, 12, "uint8 index + table"): 51.83}
 
 
 def parse():

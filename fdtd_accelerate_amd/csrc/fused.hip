@@ -10,7 +10,8 @@
 //   3. shell H pass: H_new (core + CPML-H + H sources + probes) outside the box
 //      (h_quad_at).
 // The interior box is the set of cells whose E and H updates have no CPML term:
-// planes, rows and z cells [npml+1, n-1-npml) per axis (z rounded inward to quads).  Its
+// planes, rows and z cells [npml+1, n-1-npml) per axis (z rounded inward to 128-byte
+// lines, 32 cells, so that the shell's z pieces and the sweep share no line).  Its
 // E halo (plane xb-1, row yb-1) holds no CPML term either, so the sweep recomputes it;
 // the E_new at z = kb-1 it needs comes from the shell E pass, and the shell H pass reads
 // the box's E_new.  Without CPML the box is the whole grid and only the sweep runs.
@@ -65,7 +66,7 @@ struct FGeom {
     int nchunks; // chunks over [xb, xe)
     int xb, xe;  // planes of the interior box
     int yb, ye;  // rows of the interior box
-    int kb, ke;  // z range of the interior box (multiples of 4)
+    int kb, ke;  // z range of the interior box (multiples of 32)
     int zb0;     // first z segment start: kb rounded down to 256 (segments sit on 1 KiB)
     int nseg;    // z segments of 256 floats from zb0 covering [kb, ke)
     int nblocks;
@@ -609,7 +610,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         const int np_ = d->npml;
         g.xb = np_ + 1; g.xe = nx - 1 - np_;
         g.yb = np_ + 1; g.ye = ny - 1 - np_;
-        g.kb = 4 * ((np_ + 1 + 3) / 4); g.ke = 4 * ((nz - 1 - np_) / 4);
+        g.kb = 32 * ((np_ + 1 + 31) / 32); g.ke = 32 * ((nz - 1 - np_) / 32);
         if (g.xb >= g.xe || g.yb >= g.ye || g.kb >= g.ke) {  // no interior: the shell is the grid
             g.xb = g.xe = nx;
             g.yb = g.ye = ny;

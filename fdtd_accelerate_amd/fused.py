@@ -174,7 +174,7 @@ class FusedStepper:
         if cp is None:
             return 0, nx, 0, ny, 0, f.nz_pitch
         p = cp.npml
-        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 4 * ((p + 4) // 4), 4 * ((nz - 1 - p) // 4))
+        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32))
         if box[0] >= box[1] or box[2] >= box[3] or box[4] >= box[5]:
             return nx, nx, ny, ny, f.nz_pitch, f.nz_pitch
         return box
