@@ -231,7 +231,7 @@ def run_batch(args):
     import torch
     from fdtd_accelerate_amd import batch as B
     from fdtd_accelerate_amd.fused import SlabStepper
-    sims = B.make_config3(seed=3, count=args.instances, sizes=tuple(int(v) for v in args.sizes.split(",")))
+    sims = B.make_uneven_batch(seed=3, count=args.instances, sizes=tuple(int(v) for v in args.sizes.split(",")))
     cells = sum(s.fields.nx * s.fields.ny * s.fields.nz for s in sims)
     bt = B.BatchedYee(sims)
     tuned = None
@@ -292,7 +292,7 @@ def run_batch(args):
 
 
 def run_adjoint(args):
-    """Config 4: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
+    """The 160^3 design problem: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
     64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
     import numpy as np
     import torch

@@ -6,7 +6,7 @@ every instance, then the H pass, with each instance's CPML, sources and probes f
 in (per-cell arithmetic of the in-place engine).  Source values are computed on the
 host for all K steps and copied to the device once; probe series come back per instance.
 
-``make_config3`` builds the uneven batch of the batched benchmark (64 cubes with edges
+``make_uneven_batch`` builds the uneven batch of the batched benchmark (64 cubes with edges
 drawn from {64, 96, 128, 160, 192}, own sphere permittivity maps, 8-cell CPML, seeded
 source and probe), and ``tune`` picks the launch options (graph capture, XCD block
 runs) by timing them on the batch itself.
@@ -190,7 +190,7 @@ def tune(batch, candidates=((False, 0), (True, 0), (False, 4), (True, 4), (False
     return best, res
 
 
-def make_config3(seed=3, count=64, sizes=(64, 96, 128, 160, 192), npml=8, engine="kemp"):
+def make_uneven_batch(seed=3, count=64, sizes=(64, 96, 128, 160, 192), npml=8, engine="kemp"):
     """The uneven batch: ``count`` cubes with edges drawn uniformly from ``sizes``, each
     with its own 10-40 non-overlapping spheres (eps_r U[1,12], radius U[3, N/8]), CE on,
     CPML ``npml`` (m = 4, R = 1e-6), an Ez source at a seeded interior point driven by the

@@ -1,9 +1,9 @@
-"""Writes tests/golden/adjoint_config4.npz: FP64 reference data for the adjoint gradient
-of the config-4 design problem (run here, on the CPU, with the oracle):
+"""Writes tests/golden/adjoint_design160.npz: FP64 reference data for the adjoint gradient
+of the 160^3 design problem (run here, on the CPU, with the oracle):
 
   160^3 grid, CPML 10, 64^3 design box at the centre with eps_r ~ U[1,4] (float32,
   numpy.random.default_rng(EPS_SEED)), Ez soft line source on the plane x = 20 (the line
-  y = 80 along z over the non-CPML interior) driven by the config-0 pulse, objective J = sum over steps 601..800
+  y = 80 along z over the non-CPML interior) driven by the Gaussian-modulated pulse, objective J = sum over steps 601..800
   of sum over the plane x = 140 of |E|^2.
 
 * central finite differences of the float64 C oracle (OracleSim) at 32 seeded voxels,
@@ -80,7 +80,7 @@ def fd(cfg, eps64, v, h):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--out", default=os.path.join(HERE, "adjoint_config4.npz"))
+    ap.add_argument("--out", default=os.path.join(HERE, "adjoint_design160.npz"))
     ap.add_argument("--device", default="cpu", help="where the float64 autograd runs (cpu or cuda)")
     ap.add_argument("--fd-only", action="store_true", help="stop after the finite-difference stage")
     a = ap.parse_args()

@@ -1,5 +1,5 @@
 """GPU adjoint gradient (hand-written adjoint kernels, checkpoint and recompute) against
-the FP64 reference gradient (autograd through oracle/torch_ref.py) and, at config 4,
+the FP64 reference gradient (autograd through oracle/torch_ref.py) and, on the 160^3 design problem,
 against the committed FP64 central differences at 32 seeded voxels."""
 import os
 
@@ -78,12 +78,12 @@ def test_autograd_function():
     np.testing.assert_allclose(g, 2.0 * g2, rtol=1e-6)
 
 
-_GOLD = os.path.join(ROOT, "tests", "golden", "adjoint_config4.npz")
+_GOLD = os.path.join(ROOT, "tests", "golden", "adjoint_design160.npz")
 
 
-@pytest.mark.skipif(not os.path.exists(_GOLD), reason="config-4 FP64 reference data not generated")
-def test_config4_adjoint_vs_fd_and_fp64_gradient():
-    """Config 4 (160^3, CPML 10, 64^3 design box, 800 steps, checkpoints every 50):
+@pytest.mark.skipif(not os.path.exists(_GOLD), reason="160^3 design-problem FP64 reference data not generated")
+def test_design160_adjoint_vs_fd_and_fp64_gradient():
+    """The 160^3 design problem (CPML 10, 64^3 design box, 800 steps, checkpoints every 50):
     adjoint gradient vs FP64 central differences at 32 voxels (max rel < 1e-3) and vs the
     FP64 reference gradient over the box (relative L2 < 1e-4)."""
     from fdtd_accelerate_amd.adjoint import DesignProblem

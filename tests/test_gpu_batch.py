@@ -83,7 +83,7 @@ def test_batch_continues_and_no_cpml():
 
 
 def test_batch_instance_vs_oracle():
-    """One instance of a batch (128^3 vacuum, CPML 10, config-0 source) against the
+    """One instance of a batch (128^3 vacuum, CPML 10, the dipole run's pulse source) against the
     float32 oracle over 300 steps, next to two other instances."""
     from fdtd_accelerate_amd.batch import BatchedYee
     from fdtd_accelerate_amd.sim import YeeSim

@@ -1,14 +1,14 @@
 """Parity at the full sizes the throughput figures are measured on, from seeded random
 initial fields (every cell, the walls and the CPML slabs live from the first step):
 
-* config 1 -- the 1024^3 Yee workload (200-sphere dielectric, CPML 12, Ez point source
+* the 1024^3 Yee workload (200-sphere dielectric, CPML 12, Ez point source
   and probe): the in-place engine equals the float32 C oracle after 2 steps; the fused
   engine (CE as uint8 material indices, field sets in one staggered allocation) equals
   the in-place engine (float CE) in every field, psi value and probe sample after 2 and
   then 5 steps;
-* config 2 -- the 512^3 4th-order symplectic workload (100-sphere dielectric, CPML 10,
+* the 512^3 4th-order symplectic workload (100-sphere dielectric, CPML 10,
   S = 0.9 S_max): equal to the float32 C oracle after 2 steps;
-* config 3 -- the 64-cube uneven batch (CPML 8): every instance of the batched run
+* the 64-cube uneven batch (CPML 8): every instance of the batched run
   equals the same instance run alone after 3 steps.
 
 Values are compared with -0 == +0 (the fused kernel's masked lanes may store +0 where
@@ -71,7 +71,7 @@ def _randomize(tensors, seed):
         t.normal_(generator=g)
 
 
-def test_config1_1024cube_engines_and_oracle():
+def test_yee_1024cube_spheres_engines_and_oracle():
     _need(160)
     ref = _yee("kemp", False)
     fus = _yee("fused", True)
@@ -116,7 +116,7 @@ def test_config1_1024cube_engines_and_oracle():
     assert all(t.untyped_storage().data_ptr() == base for t in arrays)
 
 
-def test_config2_512cube_s4_vs_oracle():
+def test_s4_512cube_spheres_vs_oracle():
     _need(20)
     from fdtd_accelerate_amd import geometry
     from fdtd_accelerate_amd.s4 import S4Sim
@@ -145,11 +145,11 @@ def test_config2_512cube_s4_vs_oracle():
         assert _equal_host(np.asfortranarray(g.field(name).cpu().numpy()), o.ehs[k]), name
 
 
-def test_config3_full_batch_equals_single_runs():
+def test_uneven_64cube_batch_equals_single_runs():
     _need(40)
     from fdtd_accelerate_amd import batch as B
-    a = B.make_config3(seed=3)
-    b = B.make_config3(seed=3)
+    a = B.make_uneven_batch(seed=3)
+    b = B.make_uneven_batch(seed=3)
     assert len(a) == 64
     for n, (sa, sb) in enumerate(zip(a, b)):
         _randomize(sa.fields.ehs, 100 + n)

@@ -177,8 +177,8 @@ def test_fused_continues_across_calls_and_engines():
         assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
 
 
-def test_fused_config0_vs_oracle():
-    """Config 0 (128^3 vacuum, CPML 10, point source, probe) on the fused engine, 400 steps:
+def test_fused_dipole128_vs_oracle():
+    """The 128^3 vacuum dipole run (CPML 10, point source, probe) on the fused engine, 400 steps:
     probe series and fields equal the float32 oracle."""
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse
@@ -253,8 +253,8 @@ def test_slab_matches_inplace(ns, npml, S, steps, ce):
                 assert sims[0].cpml.psiH[key][c].cpu().numpy().tobytes() == sims[1].cpml.psiH[key][c].cpu().numpy().tobytes(), ("psiH", key, c)
 
 
-def test_slab_config0_vs_oracle():
-    """Config 0 (128^3 vacuum, CPML 10, point source, probe) on the slab engine, 400
+def test_slab_dipole128_vs_oracle():
+    """The 128^3 vacuum dipole run (CPML 10, point source, probe) on the slab engine, 400
     steps: probe series and fields byte-equal to the float32 oracle."""
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse

@@ -127,9 +127,9 @@ def test_s4_step_matches_oracle(shape, npml, steps):
                 assert ref.tobytes() == got.tobytes(), (key, c)
 
 
-def test_s4_config2_scaled_vs_oracle():
-    """Config 2 at 64^3: 0.9 S_max, 10-cell CPML, sphere dielectric, Ez point source with
-    the config-0 pulse, 150 steps: probe series and fields equal the float32 oracle."""
+def test_s4_spheres_64cube_vs_oracle():
+    """The 4th-order sphere workload at 64^3: 0.9 S_max, 10-cell CPML, sphere dielectric, Ez point source with
+    the Gaussian-modulated pulse, 150 steps: probe series and fields equal the float32 oracle."""
     from fdtd_accelerate_amd import geometry
     from fdtd_accelerate_amd.s4 import S4Sim, courant_limit
     n, npml, steps = 64, 10, 150

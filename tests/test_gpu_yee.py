@@ -290,8 +290,8 @@ def test_gather_probe():
     assert host(out).tolist() == [0.0, 999.0, 17.0, 512.0]
 
 
-# ------------------------------------------------------------------ config 0 end-to-end
-def _config0_oracle(dtype, steps):
+# ------------------------------------------- 128^3 vacuum dipole run end-to-end
+def _dipole128_oracle(dtype, steps):
     s = O.OracleSim(128, 128, 128, 0, 10, dtype)
     s.add_source("ez", (64, 64, 64), (64, 64, 64), O.pulse)
     s.add_probe("ez", (96, 64, 64))
@@ -299,7 +299,7 @@ def _config0_oracle(dtype, steps):
     return s
 
 
-def test_config0_128cube_1000_steps():
+def test_dipole_128cube_1000_steps():
     """Plain Yee, 128^3, vacuum, 10-cell CPML (R=1e-6, m=4), Ez point dipole at the
     centre driven by the Gaussian-modulated pulse, Ez probe at (96,64,64), 1000 steps:
     the GPU run is bit-identical to the float32 oracle (fields and probe series); the
@@ -311,18 +311,18 @@ def test_config0_128cube_1000_steps():
     sim.add_source("ez", (64, 64, 64), (64, 64, 64), gaussian_pulse())
     sim.add_probe("ez", (96, 64, 64))
     series = sim.run(steps)[:, 0]
-    o32 = _config0_oracle(np.float32, steps)
+    o32 = _dipole128_oracle(np.float32, steps)
     ref32 = np.array(o32.series, np.float32)[:, 0]
     assert series.shape == (steps,)
     assert same_bits(series, ref32)
     for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
         assert same_bits(sim.field(name).cpu().numpy(), o32.field(name)), name
-    o64 = _config0_oracle(np.float64, steps)
+    o64 = _dipole128_oracle(np.float64, steps)
     ref64 = np.array(o64.series)[:, 0]
     err = np.abs(series.astype(np.float64) - ref64).max() / np.abs(ref64).max()
     ferr = max(np.abs(sim.field(n).cpu().numpy().astype(np.float64) - o64.field(n)).max()
                for n in ("ex", "ey", "ez")) / max(np.abs(o64.field(n)).max() for n in ("ex", "ey", "ez"))
-    print(f"config0: probe max-rel err vs fp64 = {err:.3e}; E-field max err / max|E| = {ferr:.3e}", flush=True)
+    print(f"dipole128: probe max-rel err vs fp64 = {err:.3e}; E-field max err / max|E| = {ferr:.3e}", flush=True)
     assert err < 1e-4 and ferr < 1e-4
     # the pulse reached the probe and the timing matches the source (peak after step 100)
     assert np.abs(series).max() > 1e-4 and np.argmax(np.abs(series)) > 100
