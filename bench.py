@@ -17,6 +17,9 @@ source + H update + CPML-H (+ probe gather).
     python bench.py --workload adjoint   (160^3 design problem: forward 800 steps with
                        checkpoints every 50, adjoint gradient on the 64^3 box; reports
                        backward / forward time)
+    python bench.py --workload pbc   (the 1024^3 sphere workload with periodic x and y and
+                       CPML on the z faces, in-place engine; steps timed without and
+                       with the periodic boundaries)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -58,7 +61,7 @@ def parse():
     p.add_argument("--engine", default="fused", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc"))
     p.add_argument("--instances", type=int, default=64)
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
@@ -291,6 +294,45 @@ def run_batch(args):
     }
 
 
+def run_pbc(args):
+    """Periodic boundaries: the sphere workload of the default run with periodic x and y
+    (Pbc(fields, "xy")) and CPML only on the z faces, on the in-place engine (the engine
+    that carries periodic boundaries).  The same grid is timed first without and then
+    with the periodic boundaries; the dominant kernel is update_e either way."""
+    import torch
+    from fdtd_accelerate_amd import Pbc, geometry
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    n, npml = args.n, args.npml
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="e", cpml_faces=("", "", "fb"))
+    lo, hi = (npml,) * 3, (n - npml,) * 3
+    rmax = min(40.0, (n - 2 * npml) / 8)
+    spheres = geometry.random_spheres(2024, args.spheres, (min(8.0, rmax / 2), rmax), (1.0, 12.0), lo, hi)
+    geometry.paint_ce(sim.fields.ces, spheres, (n, n, n))
+    c = n // 2
+    sim.add_source("ez", (c, c, c), (c, c, c), gaussian_pulse())
+    sim.add_probe("ez", (c + n // 4, c, c))
+    sim.fields.compress_ce()
+    torch.cuda.synchronize()
+    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+    Pbc(sim.fields, "xy")
+    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    cells = n ** 3
+    achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, periodic x and y, CPML {npml} on the "
+                               f"z faces, Ez point source", "engine": "kemp", "cells": cells},
+        "ms_per_step_without_periodic": round(ms0, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": kbytes, "avg_launch_ms": round(kern_ms, 4)},
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def run_adjoint(args):
     """The 160^3 design problem: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
     64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
@@ -391,6 +433,9 @@ def main():
         if args.steps == 200:
             args.steps = 300
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "pbc":
+        print(json.dumps(run_pbc(args)), flush=True)
         return
     if args.workload == "s4":
         if args.n == 1024:

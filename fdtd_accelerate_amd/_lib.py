@@ -77,6 +77,7 @@ _SIGS = {
     "mifdtd_cpml_h": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
+    "mifdtd_pbc": (_I, [_P, _P] + [_I] * 6 + [_P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
     "mifdtd_fused_run_timed": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P,
                                     ctypes.POINTER(ctypes.c_float)]),

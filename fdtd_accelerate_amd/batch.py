@@ -45,6 +45,8 @@ class BatchedYee:
                 raise ValueError("the batched engine needs CPML on all six faces (or none)")
             if len(s.sources) > _lib.MAX_SRC or len(s.probe_points) > _lib.MAX_PROBE:
                 raise ValueError("too many sources or probes on an instance")
+            if any(getattr(i, "priority_type", "") == "pbc" for i in s.fields.instance_list):
+                raise ValueError("the batched engine has no periodic boundaries")
         self.sims = list(sims)
         self.device = sims[0].fields.device
         self.stream = stream if stream is not None else torch.cuda.Stream(self.device)

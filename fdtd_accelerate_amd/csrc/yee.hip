@@ -645,6 +645,33 @@ __global__ __launch_bounds__(kBlock) void k_gather(const float *__restrict__ f, 
     if (n < count) out[n] = f[off[n]];
 }
 
+// ------------------------------------------------------------- periodic boundary
+// Copy plane `src` of two components onto plane `dst` across axis `axis`: element n of
+// the plane is (j, k) of plane i for x (ny * nzp elements, contiguous), (i, k) of row j
+// for y (nx rows of nzp), (i, j) at z = plane for z (nx * ny elements, stride nzp).
+__global__ __launch_bounds__(kBlock) void k_pbc(float *__restrict__ f1, float *__restrict__ f2, int nx, int ny,
+                                                int nzp, int axis, int src, int dst, unsigned count)
+{
+    const unsigned n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= count) return;
+    size_t s, d;
+    if (axis == 0) {
+        const size_t plane = (size_t)ny * nzp;
+        s = (size_t)src * plane + n;
+        d = (size_t)dst * plane + n;
+    } else if (axis == 1) {
+        const size_t i = n / (unsigned)nzp, k = n % (unsigned)nzp;
+        s = (i * ny + src) * nzp + k;
+        d = (i * ny + dst) * nzp + k;
+    } else {
+        const size_t row = n;  // i * ny + j
+        s = row * nzp + src;
+        d = row * nzp + dst;
+    }
+    f1[d] = f1[s];
+    f2[d] = f2[s];
+}
+
 }  // namespace mifdtd
 
 // =========================================================================== C ABI
@@ -837,6 +864,26 @@ int mifdtd_source(float *f, int nx, int ny, int nzp, int i0, int i1, int j0, int
     const unsigned g = (unsigned)((tot + kBlock - 1) / kBlock);
     k_source<<<g, kBlock, 0, to_stream(stream)>>>(f, ny, nzp, i0, j0, k0, li, lj, lk, tval, sv, svs, overwrite);
     MIFDTD_CHECK_LAUNCH("source launch");
+    return 0;
+}
+
+int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int axis, int e_half,
+               mifdtd_stream_t stream)
+{
+    if (!f1 || !f2) return fail(MIFDTD_ENULL, "field pointer is NULL");
+    if (int rc = check_grid(nx, ny, nz_pitch, pad)) return rc;
+    if (axis < 0 || axis > 2) return fail(MIFDTD_EINVAL, "axis %d outside [0,2]", axis);
+    if (f1 == f2) return fail(MIFDTD_EINVAL, "the two components of a periodic copy must differ");
+    const int n = axis == 0 ? nx : axis == 1 ? ny : nz_pitch - pad;
+    if (n < 2) return fail(MIFDTD_EINVAL, "a periodic axis needs at least 2 cells");
+    const int src = e_half ? 0 : n - 1, dst = e_half ? n - 1 : 0;
+    const unsigned long long count = axis == 0 ? (unsigned long long)ny * nz_pitch
+                                   : axis == 1 ? (unsigned long long)nx * nz_pitch
+                                               : (unsigned long long)nx * ny;
+    if (count >= (1ull << 32) - kBlock) return fail(MIFDTD_ERANGE, "periodic plane too large");
+    const unsigned g = (unsigned)((count + kBlock - 1) / kBlock);
+    k_pbc<<<g, kBlock, 0, to_stream(stream)>>>(f1, f2, nx, ny, nz_pitch, axis, src, dst, (unsigned)count);
+    MIFDTD_CHECK_LAUNCH("periodic copy launch");
     return 0;
 }
 

@@ -32,6 +32,8 @@ def _check_whole_step(sim, engine):
     cp = sim.cpml
     if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
         raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
+    if any(getattr(i, "priority_type", "") == "pbc" for i in f.instance_list):
+        raise ValueError(f"the {engine} engine has no periodic boundaries; use engine='kemp'")
     if len(sim.sources) > _lib.MAX_SRC:
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_SRC} sources")
     if len(sim.probe_points) > _lib.MAX_PROBE:

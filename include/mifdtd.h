@@ -151,6 +151,16 @@ int mifdtd_source(float *f, int nx, int ny, int nz_pitch,
                   float tval, const float *sv, float svs, int overwrite,
                   mifdtd_stream_t stream);
 
+/* Periodic boundary: replaces Pbc.update of K7/cpu/pbc.py:25-31 (the slice table of
+ * K7/util/common_exchange.py:1-29; K7/gpu/pbc.py is the same copy on device).  For axis
+ * 0, 1, 2 (x, y, z), f1 and f2 are the two components tangential to it (x: ey, ez or
+ * hy, hz; y: ex, ez or hx, hz; z: ex, ey or hx, hy).  e_half != 0 (after the E half
+ * step): plane n-1 takes plane 0; e_half == 0 (after the H half step): plane 0 takes
+ * plane n-1.  n is nx, ny or the real nz (nz_pitch - pad); x and y planes include the
+ * pad.  Axes are applied one call each, in the caller's order. */
+int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int axis, int e_half,
+               mifdtd_stream_t stream);
+
 /* Probe / GetFields gather (K7/cpu/get_set_fields.py:47-50, K7/gpu/src/copy.cl:3-14):
  *   out[n] = f[offsets[n]]  for n < count   (offsets: device int64 element offsets) */
 int mifdtd_gather(const float *f, const long long *offsets, int count, float *out,

@@ -14,9 +14,15 @@ What it holds:
   ``make -C oracle ref`` into ``oracle/_ref`` (only where /root/reference exists).
 * ``cpml_tables`` / ``pulse`` — closed forms of the CPML profile and the
   Gaussian-modulated source (``KU/cpu/cpml.py:107-126``, ``KB/source/source.py:247-249``).
-* ``OracleSim`` — the full step (E core, CPML-E faces, E sources, H core, CPML-H faces,
-  H sources, probes) in the order of ``KU/cpu/test/020-dielectric-cpml.py:58-73`` and the
-  instance priorities of ``K7/util/common.py:383-397``.
+* ``OracleSim`` — the full step (E core, CPML-E faces, E sources, periodic E planes, H
+  core, CPML-H faces, H sources, periodic H planes, probes) in the order of
+  ``KU/cpu/test/020-dielectric-cpml.py:58-73`` and the instance priorities of
+  ``K7/util/common.py:383-397``.
+* ``pbc_copy`` — the periodic-boundary plane copy of ``K7/cpu/pbc.py:25-31`` with the
+  plane table of ``K7/util/common_exchange.py:1-29``.  Parity pinned by the replication
+  invariant (a P-periodic state stays P-periodic: a run over period P equals, bit for
+  bit, the same state replicated over period 2P), which a wrong plane index, copy
+  direction or component fails.
 """
 from __future__ import annotations
 
@@ -359,6 +365,28 @@ class OracleS4Sim:
 
 
 # --------------------------------------------------------------------- full step
+# Periodic boundaries (K7/util/common_exchange.py:1-29): the field components tangential
+# to each axis; after the E half step plane n-1 takes plane 0, after the H half step
+# plane 0 takes plane n-1 (n the real cell count; the x and y planes include the z pad).
+PBC_COMPONENTS = {"x": {"e": (1, 2), "h": (4, 5)}, "y": {"e": (0, 2), "h": (3, 5)},
+                  "z": {"e": (0, 1), "h": (3, 4)}}
+
+
+def pbc_copy(ehs, ns, axes, e_or_h):
+    """K7/cpu/pbc.py:25-31: for each axis of ``axes`` in order, copy the tangential
+    components' boundary plane across the grid (ns = real (nx, ny, nz))."""
+    for axis in axes:
+        a = "xyz".index(axis)
+        n = ns[a]
+        src, dst = (0, n - 1) if e_or_h == "e" else (n - 1, 0)
+        for c in PBC_COMPONENTS[axis][e_or_h]:
+            f = ehs[c]
+            get = [slice(None)] * 3
+            put = [slice(None)] * 3
+            get[a], put[a] = src, dst
+            f[tuple(put)] = f[tuple(get)]
+
+
 class OracleSim:
     """CPU oracle of one Yee+CPML simulation on the KEMP layout.
 
@@ -371,8 +399,9 @@ class OracleSim:
     """
 
     def __init__(self, nx, ny, nz, pad=0, npml=0, dtype=np.float32, ces=None, chs=None,
-                 cpml_faces=((True, True), (True, True), (True, True)), cpml_R=1e-6, cpml_m=4):
+                 cpml_faces=((True, True), (True, True), (True, True)), cpml_R=1e-6, cpml_m=4, pbc=""):
         self.nx, self.ny, self.nz, self.pad = nx, ny, nz, pad
+        self.pbc = pbc  # periodic axes, applied in this order
         self.nzp = nz + pad
         self.dtype = np.dtype(dtype)
         shp = (nx, ny, self.nzp)
@@ -422,6 +451,7 @@ class OracleSim:
                         p1, p2 = self.psiE[ax, sd]
                         cpml_e(ax, sd, self.ehs, self.ces, p1, p2, self.bE, self.aE, self.pad, self.npml)
         self._sources("e")
+        pbc_copy(self.ehs, (self.nx, self.ny, self.nz), self.pbc, "e")
         update_h(self.ehs, self.chs, self.pad)
         if self.npml:
             for ax in range(3):
@@ -430,6 +460,7 @@ class OracleSim:
                         p1, p2 = self.psiH[ax, sd]
                         cpml_h(ax, sd, self.ehs, self.chs, p1, p2, self.bH, self.aH, self.pad, self.npml)
         self._sources("h")
+        pbc_copy(self.ehs, (self.nx, self.ny, self.nz), self.pbc, "h")
         if self.probes:
             self.series.append([self.field(n)[p] for n, p in self.probes])
 

@@ -1,0 +1,94 @@
+"""Periodic boundaries on the GPU (``Pbc``, ``mifdtd_pbc``) against the float32 oracle:
+random fields (walls and boundary planes included), CE, CPML on the non-periodic faces,
+a point source and probes; every field and probe sample byte-equal after the steps.
+Covers each axis, several axes in both orders, z pads, and the engines that reject a
+periodic grid."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from fdtd_accelerate_amd import _lib
+    _lib.load()
+
+
+def _same(a, b):
+    return a.shape == b.shape and (np.ascontiguousarray(a) + np.float32(0)).tobytes() == \
+        (np.ascontiguousarray(b) + np.float32(0)).tobytes()
+
+
+@pytest.mark.parametrize("ns,axes,npml,seg,steps", [
+    ((20, 18, 24), "x", 4, 64, 9),
+    ((18, 21, 20), "y", 3, 64, 8),
+    ((16, 18, 21), "z", 4, 16, 10),   # nz 21: pad 3, the z copy uses the real last cell
+    ((20, 22, 24), "xy", 4, 64, 7),
+    ((20, 22, 26), "yx", 0, 32, 7),
+    ((17, 19, 23), "xyz", 0, 16, 12),
+    ((18, 16, 30), "zx", 5, 64, 6),
+])
+def test_pbc_matches_oracle(ns, axes, npml, seg, steps):
+    from fdtd_accelerate_amd import Pbc
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    nx, ny, nz = ns
+    faces = tuple("" if "xyz"[a] in axes else "fb" for a in range(3))
+    sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e", cpml_faces=faces, segment_nbytes=seg)
+    f = sim.fields
+    rng = np.random.default_rng(len(axes) * 100 + nz)
+    init = [rng.standard_normal(ns).astype(np.float32) for _ in range(6)]
+    ces = [(0.5 / rng.uniform(1, 6, ns)).astype(np.float32) for _ in range(3)]
+    f.set_ehs(*init)
+    f.set_ces(*ces)
+    Pbc(f, axes)
+    tf = gaussian_pulse(8.0, 3.0, 10.0)
+    src = (nx // 2, ny // 2, nz // 2)
+    sim.add_source("ez", src, src, tf)
+    sim.add_probe("ey", (0, ny // 2, nz // 2))
+    sim.add_probe("hz", (nx - 1, ny - 1, nz - 1))
+    pad = f.pad
+    o = O.OracleSim(nx, ny, nz, pad, npml, np.float32,
+                    ces=[np.pad(c, ((0, 0), (0, 0), (0, pad)), constant_values=np.float32(0.5)) for c in ces],
+                    cpml_faces=tuple((("xyz"[a] not in axes),) * 2 for a in range(3)), pbc=axes)
+    for k in range(6):
+        o.ehs[k][:, :, :nz] = init[k]
+    o.add_source("ez", src, src, tf)
+    o.add_probe("ey", (0, ny // 2, nz // 2))
+    o.add_probe("hz", (nx - 1, ny - 1, nz - 1))
+    series = sim.run(steps)
+    o.run(steps)
+    assert _same(series, np.array(o.series, np.float32))
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+    # the boundary planes carry the periodic copies
+    a = "xyz".index(axes[0])
+    n = (nx, ny, nz)[a]
+    comp = {"x": "ey", "y": "ex", "z": "ex"}[axes[0]]
+    g = sim.field(comp).cpu().numpy()[:, :, :nz]
+    assert np.array_equal(np.take(g, n - 1, axis=a), np.take(g, 0, axis=a))
+
+
+def test_whole_step_engines_reject_periodic_grids():
+    from fdtd_accelerate_amd import Pbc
+    from fdtd_accelerate_amd.batch import BatchedYee
+    from fdtd_accelerate_amd.sim import YeeSim
+    for engine in ("fused", "slab"):
+        s = YeeSim(16, 16, 16, npml=0, coeff_use="", engine=engine)
+        Pbc(s.fields, "x")
+        with pytest.raises(ValueError, match="periodic"):
+            s.run(1)
+    s = YeeSim(16, 16, 16, npml=0, coeff_use="")
+    Pbc(s.fields, "z")
+    with pytest.raises(ValueError, match="periodic"):
+        BatchedYee([s])
+    with pytest.raises(AssertionError):
+        Pbc(s.fields, "w")

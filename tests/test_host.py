@@ -87,6 +87,12 @@ def test_argument_errors_without_gpu():
     assert rc == -1 and "too small for npml" in _lib.last_error()
     rc = lib.mifdtd_source(16, 4, 4, 8, 0, 4, 0, 0, 0, 0, 1.0, None, 1.0, 0, None)
     assert rc == -1 and "outside" in _lib.last_error()
+    rc = lib.mifdtd_pbc(16, 32, 4, 4, 8, 0, 3, 1, None)
+    assert rc == -1 and "axis" in _lib.last_error()
+    rc = lib.mifdtd_pbc(16, 16, 4, 4, 8, 0, 0, 1, None)
+    assert rc == -1 and "differ" in _lib.last_error()
+    rc = lib.mifdtd_pbc(None, 16, 4, 4, 8, 0, 0, 1, None)
+    assert rc == -3
     with pytest.raises(RuntimeError):
         _lib.call("mifdtd_set_pad", -1)
 
