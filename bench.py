@@ -20,6 +20,9 @@ source + H update + CPML-H (+ probe gather).
     python bench.py --workload pbc   (the 1024^3 sphere workload with periodic x and y and
                        CPML on the z faces, in-place engine; steps timed without and
                        with the periodic boundaries)
+    python bench.py --workload tfsf  (1024^3, CPML 12, a TFSF plane wave along x on the box
+                       [64, n-65]^3 around the sphere dielectric, in-place engine;
+                       steps timed without and with the plane wave)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -61,7 +64,7 @@ def parse():
     p.add_argument("--engine", default="fused", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf"))
     p.add_argument("--instances", type=int, default=64)
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
@@ -333,6 +336,43 @@ def run_pbc(args):
     }
 
 
+def run_tfsf(args):
+    """TFSF plane wave: 1024^3 with CPML on all faces, the sphere dielectric inside the
+    box [64, n-65]^3 (its faces in vacuum), a plane wave along x (wavelength 20 cells,
+    polarization angle 0.3) entering through all six faces; in-place engine.  The same
+    grid is timed without and then with the plane wave."""
+    import torch
+    from fdtd_accelerate_amd import Tfsf, geometry
+    from fdtd_accelerate_amd.sim import YeeSim
+    n, npml = args.n, args.npml
+    lo_box, hi_box = 64, n - 65
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="e")
+    lo, hi = (lo_box + 8,) * 3, (hi_box - 8,) * 3
+    rmax = min(40.0, (hi_box - lo_box) / 8)
+    spheres = geometry.random_spheres(2024, args.spheres, (min(8.0, rmax / 2), rmax), (1.0, 12.0), lo, hi)
+    geometry.paint_ce(sim.fields.ces, spheres, (n, n, n))
+    sim.fields.compress_ce()
+    torch.cuda.synchronize()
+    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+    Tfsf(sim.fields, (lo_box,) * 3, (hi_box,) * 3, ("fb", "fb", "fb"), 20.0, ("normal", "x"), 0.3)
+    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    cells = n ** 3
+    achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres inside the TFSF box [{lo_box},{hi_box}]^3, "
+                               f"CPML {npml} all faces, x-directed TFSF plane wave (wavelength 20, polarization 0.3)",
+                   "engine": "kemp", "cells": cells},
+        "ms_per_step_without_tfsf": round(ms0, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": kbytes, "avg_launch_ms": round(kern_ms, 4)},
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def run_adjoint(args):
     """The 160^3 design problem: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
     64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
@@ -433,6 +473,9 @@ def main():
         if args.steps == 200:
             args.steps = 300
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "tfsf":
+        print(json.dumps(run_tfsf(args)), flush=True)
         return
     if args.workload == "pbc":
         print(json.dumps(run_pbc(args)), flush=True)

@@ -12,11 +12,12 @@ from .core import Core
 from .cpml import Cpml, cpml_coefficients, sigma_max
 from .incident import IncidentDirect, GetFields, SetFields
 from .pbc import Pbc
+from .tfsf import Tfsf
 from .sources import gaussian_pulse
 from . import geometry
 
 __all__ = [
-    "Fields", "Core", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc",
+    "Fields", "Core", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc", "Tfsf",
     "cpml_coefficients", "sigma_max", "gaussian_pulse", "geometry",
     "check_type", "check_value",
 ]

@@ -24,6 +24,14 @@ class CpmlHalf(ctypes.Structure):
 
 MAX_SRC = 4
 MAX_PROBE = 8
+TFSF_MAX_PATCH = 12
+
+
+class TfsfPatches(ctypes.Structure):
+    """``mifdtd_tfsf_patches``: face patches of one TFSF half step."""
+    _fields_ = [("n", _I), ("axis", _I), ("comp", _I * TFSF_MAX_PATCH), ("box", (_I * 6) * TFSF_MAX_PATCH),
+                ("base", _I * TFSF_MAX_PATCH), ("coef", ctypes.c_float * TFSF_MAX_PATCH),
+                ("start", ctypes.c_uint * (TFSF_MAX_PATCH + 1))]
 
 
 class FusedDesc(ctypes.Structure):
@@ -78,6 +86,8 @@ _SIGS = {
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_pbc": (_I, [_P, _P] + [_I] * 6 + [_P]),
+    "mifdtd_tfsf_line": (_I, [_P, _P, _P, _I, _I, _F, _P]),
+    "mifdtd_tfsf_faces": (_I, [_P, _P, _P, _I, _I, _I, _I, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),
     "mifdtd_fused_run_timed": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P,
                                     ctypes.POINTER(ctypes.c_float)]),

@@ -47,6 +47,8 @@ class BatchedYee:
                 raise ValueError("too many sources or probes on an instance")
             if any(getattr(i, "priority_type", "") == "pbc" for i in s.fields.instance_list):
                 raise ValueError("the batched engine has no periodic boundaries")
+            if any(type(i).__name__ == "Tfsf" for i in s.fields.instance_list):
+                raise ValueError("the batched engine has no TFSF source")
         self.sims = list(sims)
         self.device = sims[0].fields.device
         self.stream = stream if stream is not None else torch.cuda.Stream(self.device)

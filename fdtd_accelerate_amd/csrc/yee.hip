@@ -672,6 +672,55 @@ __global__ __launch_bounds__(kBlock) void k_pbc(float *__restrict__ f1, float *_
     f2[d] = f2[s];
 }
 
+// ------------------------------------------------------------------ TFSF plane wave
+// The 1-D incident line (one work-group): E half: E[1:n-1] -= 0.5 (H[2:] -
+// H[1:n-1]), the two-step-delay ends (abc[0..1] front, abc[2..3] back), E[3] = src;
+// H half: H[1:] -= 0.5 (E[1:] - E[:-1]).  Each half reads only the other array.
+__global__ __launch_bounds__(1024) void k_tfsf_line(float *__restrict__ e, float *__restrict__ h, float *__restrict__ abc,
+                                                    int n, int e_half, float src)
+{
+    if (e_half) {
+        for (int m = 1 + threadIdx.x; m < n - 1; m += blockDim.x) e[m] = e[m] - 0.5f * (h[m + 1] - h[m]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            e[0] = abc[0];
+            abc[0] = abc[1];
+            abc[1] = e[1];
+            e[n - 1] = abc[2];
+            abc[2] = abc[3];
+            abc[3] = e[n - 2];
+            e[3] = src;
+        }
+    } else {
+        for (int m = 1 + threadIdx.x; m < n; m += blockDim.x) h[m] = h[m] - 0.5f * (e[m] - e[m - 1]);
+    }
+}
+
+// Face correction of one patch: the box [i0..i1, j0..j1, k0..k1] of component comp
+// takes f += 0.5 * (coef * line[base + (coordinate along axis - box start)]).  One launch
+// per patch (P.n = 1): patches of one half step meet on the box edges, where a cell
+// takes two corrections in table order.
+__global__ __launch_bounds__(kBlock) void k_tfsf_faces(mifdtd_tfsf_patches P, float *ex, float *ey, float *ez,
+                                                       float *hx, float *hy, float *hz,
+                                                       const float *__restrict__ line, int ny, int nzp)
+{
+    const unsigned t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= P.start[P.n]) return;
+    int p = 0;
+    while (p + 1 < P.n && t >= P.start[p + 1]) ++p;
+    const int *b = P.box[p];
+    const unsigned r = t - P.start[p];
+    const unsigned lk = (unsigned)(b[5] - b[4] + 1), lj = (unsigned)(b[3] - b[2] + 1);
+    const int k = b[4] + (int)(r % lk);
+    const int j = b[2] + (int)((r / lk) % lj);
+    const int i = b[0] + (int)(r / lk / lj);
+    const int along = P.axis == 0 ? i - b[0] : P.axis == 1 ? j - b[2] : k - b[4];
+    const float v = 0.5f * (P.coef[p] * line[P.base[p] + along]);
+    float *f = P.comp[p] == 0 ? ex : P.comp[p] == 1 ? ey : P.comp[p] == 2 ? ez : P.comp[p] == 3 ? hx : P.comp[p] == 4 ? hy : hz;
+    const size_t id = ((size_t)i * ny + j) * nzp + k;
+    f[id] = f[id] + v;
+}
+
 }  // namespace mifdtd
 
 // =========================================================================== C ABI
@@ -884,6 +933,56 @@ int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int 
     const unsigned g = (unsigned)((count + kBlock - 1) / kBlock);
     k_pbc<<<g, kBlock, 0, to_stream(stream)>>>(f1, f2, nx, ny, nz_pitch, axis, src, dst, (unsigned)count);
     MIFDTD_CHECK_LAUNCH("periodic copy launch");
+    return 0;
+}
+
+int mifdtd_tfsf_line(float *e, float *h, float *abc, int n, int e_half, float src, mifdtd_stream_t stream)
+{
+    if (!e || !h || !abc) return fail(MIFDTD_ENULL, "TFSF line pointer is NULL");
+    if (n < 6) return fail(MIFDTD_EINVAL, "TFSF line of %d points (at least 6)", n);
+    k_tfsf_line<<<1, 1024, 0, to_stream(stream)>>>(e, h, abc, n, e_half, src);
+    MIFDTD_CHECK_LAUNCH("TFSF line launch");
+    return 0;
+}
+
+int mifdtd_tfsf_faces(const mifdtd_tfsf_patches *p, float *const f[6], const float *line, int line_n, int nx, int ny,
+                      int nz_pitch, mifdtd_stream_t stream)
+{
+    if (!p || !f || !line) return fail(MIFDTD_ENULL, "TFSF argument is NULL");
+    for (int q = 0; q < 6; q++)
+        if (!f[q]) return fail(MIFDTD_ENULL, "field pointer is NULL");
+    if (p->n < 0 || p->n > MIFDTD_TFSF_MAX_PATCH) return fail(MIFDTD_EINVAL, "patch count %d", p->n);
+    if (p->axis < 0 || p->axis > 2) return fail(MIFDTD_EINVAL, "axis %d", p->axis);
+    mifdtd_tfsf_patches P = *p;
+    unsigned long long tot = 0;
+    for (int q = 0; q < P.n; q++) {
+        const int *b = P.box[q];
+        if (P.comp[q] < 0 || P.comp[q] > 5) return fail(MIFDTD_EINVAL, "patch %d component %d", q, P.comp[q]);
+        if (b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] >= nx || b[3] >= ny || b[5] >= nz_pitch || b[0] > b[1] ||
+            b[2] > b[3] || b[4] > b[5])
+            return fail(MIFDTD_EINVAL, "patch %d box outside the grid", q);
+        const int len = P.axis == 0 ? b[1] - b[0] : P.axis == 1 ? b[3] - b[2] : b[5] - b[4];
+        if (P.base[q] < 0 || P.base[q] + len >= line_n) return fail(MIFDTD_EINVAL, "patch %d reads past the line", q);
+        P.start[q] = (unsigned)tot;
+        tot += (unsigned long long)(b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+        if (tot >= (1ull << 31)) return fail(MIFDTD_ERANGE, "TFSF faces too large");
+    }
+    P.start[P.n] = (unsigned)tot;
+    // one launch per patch, in table order (edge cells take two corrections)
+    for (int q = 0; q < P.n; q++) {
+        mifdtd_tfsf_patches Q{};
+        Q.n = 1;
+        Q.axis = P.axis;
+        Q.comp[0] = P.comp[q];
+        for (int m = 0; m < 6; m++) Q.box[0][m] = P.box[q][m];
+        Q.base[0] = P.base[q];
+        Q.coef[0] = P.coef[q];
+        Q.start[0] = 0;
+        Q.start[1] = P.start[q + 1] - P.start[q];  // cells of patch q (at least 1)
+        const unsigned g = (Q.start[1] + kBlock - 1) / kBlock;
+        k_tfsf_faces<<<g, kBlock, 0, to_stream(stream)>>>(Q, f[0], f[1], f[2], f[3], f[4], f[5], line, ny, nz_pitch);
+        MIFDTD_CHECK_LAUNCH("TFSF faces launch");
+    }
     return 0;
 }
 

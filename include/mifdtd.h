@@ -161,6 +161,28 @@ int mifdtd_source(float *f, int nx, int ny, int nz_pitch,
 int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int axis, int e_half,
                mifdtd_stream_t stream);
 
+/* TFSF plane wave, normal incidence: replaces Tfsf.updateE / updateH of
+ * kuFDTD/v0.2.0/dim3/tfsf.py:229-242 (Fdtd1d :14-40, faces and incident coefficients
+ * :138-226).  mifdtd_tfsf_line advances the 1-D incident line of n points: e_half != 0:
+ * E[1:n-1] -= 0.5 (H[2:] - H[1:n-1]), two-step-delay ends (abc[0..1] front, abc[2..3]
+ * back, device floats), then E[3] = src; e_half == 0: H[1:] -= 0.5 (E[1:] - E[:-1]).
+ * mifdtd_tfsf_faces adds, for every patch in table order, 0.5 * (coef * line[base +
+ * offset along the propagation axis]) to the patch box of one component (f: ex, ey, ez,
+ * hx, hy, hz); a cell on a box edge takes the corrections of both faces, in that order. */
+#define MIFDTD_TFSF_MAX_PATCH 12
+typedef struct {
+    int n;                                 /* patches */
+    int axis;                              /* propagation axis 0, 1, 2 */
+    int comp[MIFDTD_TFSF_MAX_PATCH];       /* 0..5 = ex..hz */
+    int box[MIFDTD_TFSF_MAX_PATCH][6];     /* i0, i1, j0, j1, k0, k1 (inclusive) */
+    int base[MIFDTD_TFSF_MAX_PATCH];       /* line index at the box start along the axis */
+    float coef[MIFDTD_TFSF_MAX_PATCH];
+    unsigned start[MIFDTD_TFSF_MAX_PATCH + 1];  /* filled by the call */
+} mifdtd_tfsf_patches;
+int mifdtd_tfsf_line(float *e, float *h, float *abc, int n, int e_half, float src, mifdtd_stream_t stream);
+int mifdtd_tfsf_faces(const mifdtd_tfsf_patches *p, float *const f[6], const float *line, int line_n, int nx, int ny,
+                      int nz_pitch, mifdtd_stream_t stream);
+
 /* Probe / GetFields gather (K7/cpu/get_set_fields.py:47-50, K7/gpu/src/copy.cl:3-14):
  *   out[n] = f[offsets[n]]  for n < count   (offsets: device int64 element offsets) */
 int mifdtd_gather(const float *f, const long long *offsets, int count, float *out,

@@ -387,6 +387,94 @@ def pbc_copy(ehs, ns, axes, e_or_h):
             f[tuple(put)] = f[tuple(get)]
 
 
+class OracleTfsf:
+    """Total-field / scattered-field plane wave, normal incidence (``KU/dim3/tfsf.py``,
+    kuFDTD v0.2.0; its oblique branch is empty).  A float32 1-D grid (``Fdtd1d``,
+    ``:14-40``) carries the wave: E[1:-1] -= 0.5 (H[2:] - H[1:-1]), two-step-delay
+    boundaries at both ends, a hard source E[3] = sin(wdt * tstep); H[1:] -= 0.5 (E[1:] -
+    E[:-1]).  1-D index 5 is the box's first plane along the propagation axis.  After
+    each 3-D half step the faces of the box [pt0, pt1] (inclusive) named by
+    ``apply_direction`` take the incident correction of ``calc_face_list`` /
+    ``calc_inc_list`` (``:138-226``): E planes just outside a face += 0.5 * c * H_inc, H
+    planes on the face += 0.5 * c * E_inc, c = +-sin/cos of the polarization angle.
+    Parity is pinned by the scattered-field invariant: in vacuum the field outside the
+    box stays at round-off level while the box carries the wave.
+    """
+
+    def __init__(self, pt0, pt1, apply_direction, wdt, axis, p_angle):
+        self.pt0 = list(pt0)
+        self.pt1 = [v + 1 for v in pt1]   # tfsf.py:111 (pt2 += 1)
+        self.apply = apply_direction
+        self.wdt = wdt
+        self.axis = axis
+        a = "xyz".index(axis)
+        n = self.pt1[a] - self.pt0[a] + 1
+        self.E = np.zeros(n + 10, np.float32)
+        self.H = np.zeros(n + 10, np.float32)
+        self.abc = [np.float32(0)] * 4
+        self.tstep = 1
+        i1, i2 = 5, self.pt1[a] - self.pt0[a] + 5
+        # float32 coefficients: the reference multiplies float32 arrays by them under
+        # value-based casting, i.e. in float32
+        si, co = np.float32(np.sin(p_angle)), np.float32(np.cos(p_angle))
+        sl1, sl2, sl3, sl4, sl5 = (slice(i1 - 1, i1), slice(i1, i1 + 1), slice(i1, i2), slice(i1, i2 + 1),
+                                   slice(i2, i2 + 1))
+        # (ce1, ce2, ch1, ch2, sliceE1, sliceE2, sliceH1, sliceH2) of Inc1..Inc6
+        z = np.float32(0)
+        inc = [(-si, co, co, si, sl1, sl1, sl2, sl2), (si, -co, -co, -si, sl5, sl5, sl5, sl5),
+               (z, si, z, -co, sl4, sl3, sl3, sl4), (z, -si, z, co, sl4, sl3, sl3, sl4),
+               (-co, z, -si, z, sl3, sl4, sl4, sl3), (co, z, si, z, sl3, sl4, sl4, sl3)]
+        order, newaxis = {"x": ([0, 1, 2, 3, 4, 5], [0, 0, 1, 1, 1, 1]),
+                          "y": ([4, 5, 0, 1, 2, 3], [1, 1, 0, 0, 0, 0]),
+                          "z": ([2, 3, 4, 5, 0, 1], [0, 0, 0, 0, 0, 0])}[axis]
+        (i1_, j1, k1), (i2_, j2, k2) = self.pt0, self.pt1
+        slx, slxp, sly, slyp, slz, slzp = (slice(i1_, i2_), slice(i1_, i2_ + 1), slice(j1, j2), slice(j1, j2 + 1),
+                                           slice(k1, k2), slice(k1, k2 + 1))
+        # (e1, e2, h1, h2 component indices, slice_e1, slice_e2, slice_h1, slice_h2) per face
+        faces = [((1, 2, 4, 5), (i1_ - 1, slyp, slz), (i1_ - 1, sly, slzp), (i1_, sly, slzp), (i1_, slyp, slz)),
+                 ((1, 2, 4, 5), (i2_, slyp, slz), (i2_, sly, slzp), (i2_, sly, slzp), (i2_, slyp, slz)),
+                 ((2, 0, 5, 3), (slx, j1 - 1, slzp), (slxp, j1 - 1, slz), (slxp, j1, slz), (slx, j1, slzp)),
+                 ((2, 0, 5, 3), (slx, j2, slzp), (slxp, j2, slz), (slxp, j2, slz), (slx, j2, slzp)),
+                 ((0, 1, 3, 4), (slxp, sly, k1 - 1), (slx, slyp, k1 - 1), (slx, slyp, k1), (slxp, sly, k1)),
+                 ((0, 1, 3, 4), (slxp, sly, k2), (slx, slyp, k2), (slx, slyp, k2), (slxp, sly, k2))]
+        self.pairs = []
+        for f in range(6):
+            if ("f" if f % 2 == 0 else "b") in apply_direction[f // 2]:
+                self.pairs.append((faces[f], inc[order[f]], newaxis[f]))
+
+    @staticmethod
+    def _expand(v, newaxis_i):
+        sl = [slice(None)]
+        sl.insert(newaxis_i, np.newaxis)
+        return v[tuple(sl)]
+
+    def update_e(self, ehs):
+        E, H = self.E, self.H
+        E[1:-1] -= 0.5 * (H[2:] - H[1:-1])
+        E[0] = self.abc[0]
+        self.abc[0] = self.abc[1]
+        self.abc[1] = E[1]
+        E[-1] = self.abc[2]
+        self.abc[2] = self.abc[3]
+        self.abc[3] = E[-2]
+        E[3] = np.float32(np.sin(self.wdt * self.tstep))
+        self.tstep += 1
+        for (comps, se1, se2, sh1, sh2), (ce1, ce2, ch1, ch2, sE1, sE2, sH1, sH2), nax in self.pairs:
+            h1 = self._expand(ch1 * H[sH1], nax)
+            h2 = self._expand(ch2 * H[sH2], nax)
+            ehs[comps[0]][se1] += 0.5 * h1
+            ehs[comps[1]][se2] += 0.5 * h2
+
+    def update_h(self, ehs):
+        E, H = self.E, self.H
+        H[1:] -= 0.5 * (E[1:] - E[:-1])
+        for (comps, se1, se2, sh1, sh2), (ce1, ce2, ch1, ch2, sE1, sE2, sH1, sH2), nax in self.pairs:
+            e1 = self._expand(ce1 * E[sE1], nax)
+            e2 = self._expand(ce2 * E[sE2], nax)
+            ehs[comps[2]][sh1] += 0.5 * e1
+            ehs[comps[3]][sh2] += 0.5 * e2
+
+
 class OracleSim:
     """CPU oracle of one Yee+CPML simulation on the KEMP layout.
 
@@ -402,6 +490,7 @@ class OracleSim:
                  cpml_faces=((True, True), (True, True), (True, True)), cpml_R=1e-6, cpml_m=4, pbc=""):
         self.nx, self.ny, self.nz, self.pad = nx, ny, nz, pad
         self.pbc = pbc  # periodic axes, applied in this order
+        self.tfsf = []  # OracleTfsf instances (priority 'incident', after the sources)
         self.nzp = nz + pad
         self.dtype = np.dtype(dtype)
         shp = (nx, ny, self.nzp)
@@ -451,6 +540,8 @@ class OracleSim:
                         p1, p2 = self.psiE[ax, sd]
                         cpml_e(ax, sd, self.ehs, self.ces, p1, p2, self.bE, self.aE, self.pad, self.npml)
         self._sources("e")
+        for t in self.tfsf:
+            t.update_e(self.ehs)
         pbc_copy(self.ehs, (self.nx, self.ny, self.nz), self.pbc, "e")
         update_h(self.ehs, self.chs, self.pad)
         if self.npml:
@@ -460,6 +551,8 @@ class OracleSim:
                         p1, p2 = self.psiH[ax, sd]
                         cpml_h(ax, sd, self.ehs, self.chs, p1, p2, self.bH, self.aH, self.pad, self.npml)
         self._sources("h")
+        for t in self.tfsf:
+            t.update_h(self.ehs)
         pbc_copy(self.ehs, (self.nx, self.ny, self.nz), self.pbc, "h")
         if self.probes:
             self.series.append([self.field(n)[p] for n, p in self.probes])

@@ -1,0 +1,99 @@
+"""TFSF plane wave on the GPU (``Tfsf``, ``mifdtd_tfsf_line`` / ``mifdtd_tfsf_faces``)
+against the float32 oracle: every field byte-equal (-0 == +0) after the steps, for each
+propagation axis, two polarization angles, all or some faces, with CPML and a
+dielectric outside the box; and the scattered-field invariant on the GPU run."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from fdtd_accelerate_amd import _lib
+    _lib.load()
+
+
+def _same(a, b):
+    return a.shape == b.shape and (np.ascontiguousarray(a) + np.float32(0)).tobytes() == \
+        (np.ascontiguousarray(b) + np.float32(0)).tobytes()
+
+
+def _pair(ns, npml, box0, box1, apply, axis, angle, wavelength, ce=None, seg=64):
+    from fdtd_accelerate_amd import Tfsf
+    from fdtd_accelerate_amd.sim import YeeSim
+    sim = YeeSim(*ns, npml=npml, coeff_use="e" if ce is not None else "", segment_nbytes=seg)
+    f = sim.fields
+    if ce is not None:
+        f.set_ces(*ce)
+    t = Tfsf(f, box0, box1, apply, wavelength, ("normal", axis), angle)
+    pad = f.pad
+    ces = None if ce is None else [np.pad(c, ((0, 0), (0, 0), (0, pad)), constant_values=np.float32(0.5)) for c in ce]
+    o = O.OracleSim(*ns, pad, npml, np.float32, ces=ces)
+    o.tfsf.append(O.OracleTfsf(box0, box1, apply, t.wdt, axis, angle))
+    return sim, o
+
+
+@pytest.mark.parametrize("axis,angle,apply,npml,seg", [
+    ("x", 0.0, ("fb", "fb", "fb"), 5, 64),
+    ("y", 0.7, ("fb", "fb", "fb"), 5, 64),
+    ("z", 1.1, ("fb", "fb", "fb"), 0, 16),
+    ("x", 0.4, ("f", "fb", "b"), 4, 64),
+    ("z", 0.0, ("fb", "", "f"), 5, 32),
+])
+def test_tfsf_matches_oracle(axis, angle, apply, npml, seg):
+    ns = (30, 28, 33)
+    rng = np.random.default_rng(5)
+    ce = [np.full(ns, 0.5, np.float32) for _ in range(3)]
+    for c in ce:  # a dielectric slab outside the box, on the far side along each axis
+        c[:, :, 26:29] = np.float32(0.5 / 3.0)
+        c[24:27] = (0.5 / rng.uniform(1, 4, c[24:27].shape)).astype(np.float32)
+    sim, o = _pair(ns, npml, (8, 7, 9), (21, 20, 22), apply, axis, angle, 10.0, ce, seg)
+    sim.run(70)
+    o.run(70)
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+    assert np.abs(o.ehs[2]).max() > 0.1 or np.abs(o.ehs[1]).max() > 0.1
+
+
+def test_tfsf_scattered_field_stays_quiet_on_gpu():
+    from fdtd_accelerate_amd import Tfsf
+    from fdtd_accelerate_amd.sim import YeeSim
+    n, npml, lo, hi = 48, 8, 14, 33
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="")
+    Tfsf(sim.fields, (lo,) * 3, (hi,) * 3, ("fb", "fb", "fb"), 12.0, ("normal", "y"), 0.3)
+    sim.run(120)
+    f = [sim.field(nm).cpu().numpy() for nm in NAMES]
+    inside = max(np.abs(x[lo + 1:hi, lo + 1:hi, lo + 1:hi]).max() for x in f)
+    m = np.zeros((n, n, n), bool)
+    m[npml + 1:n - npml - 1, npml + 1:n - npml - 1, npml + 1:n - npml - 1] = True
+    m[lo - 2:hi + 3, lo - 2:hi + 3, lo - 2:hi + 3] = False
+    outside = max(np.abs(x[m]).max() for x in f)
+    assert inside > 0.5 and outside < 1e-5 * inside, (inside, outside)
+
+
+def test_tfsf_arguments_and_engines():
+    from fdtd_accelerate_amd import Tfsf
+    from fdtd_accelerate_amd.batch import BatchedYee
+    from fdtd_accelerate_amd.sim import YeeSim
+    s = YeeSim(16, 16, 16, npml=0, coeff_use="")
+    with pytest.raises(NotImplementedError):
+        Tfsf(s.fields, (3, 3, 3), (12, 12, 12), ("fb",) * 3, 8.0, ("oblique", 0.3, 0.2))
+    with pytest.raises(ValueError):
+        Tfsf(s.fields, (0, 3, 3), (12, 12, 12), ("fb",) * 3, 8.0)
+    t = Tfsf(s.fields, (None, None, None), (None, None, None), ("fb",) * 3, 8.0)
+    assert t.pt0 == [1, 1, 1] and t.pt1 == [15, 15, 15]
+    with pytest.raises(ValueError, match="TFSF"):
+        BatchedYee([s])
+    for engine in ("fused", "slab"):
+        s2 = YeeSim(16, 16, 16, npml=0, coeff_use="", engine=engine)
+        Tfsf(s2.fields, (3, 3, 3), (12, 12, 12), ("fb",) * 3, 8.0)
+        with pytest.raises(ValueError, match="TFSF"):
+            s2.run(1)

@@ -1,5 +1,6 @@
 """CPU tests of the host side: argument checks with the reference's exceptions, index
 helpers (the reference's doctest values), the C ABI library exports, media generator."""
+import ctypes
 import os
 import re
 
@@ -93,6 +94,12 @@ def test_argument_errors_without_gpu():
     assert rc == -1 and "differ" in _lib.last_error()
     rc = lib.mifdtd_pbc(None, 16, 4, 4, 8, 0, 0, 1, None)
     assert rc == -3
+    rc = lib.mifdtd_tfsf_line(16, 16, 16, 5, 1, 0.0, None)
+    assert rc == -1 and "at least 6" in _lib.last_error()
+    P = _lib.TfsfPatches()
+    P.n = _lib.TFSF_MAX_PATCH + 1
+    rc = lib.mifdtd_tfsf_faces(ctypes.byref(P), (ctypes.c_void_p * 6)(*[16] * 6), 16, 20, 8, 8, 8, None)
+    assert rc == -1 and "patch count" in _lib.last_error()
     with pytest.raises(RuntimeError):
         _lib.call("mifdtd_set_pad", -1)
 
