@@ -23,6 +23,9 @@ source + H update + CPML-H (+ probe gather).
     python bench.py --workload tfsf  (1024^3, CPML 12, a TFSF plane wave along x on the box
                        [64, n-65]^3 around the sphere dielectric, in-place engine;
                        steps timed without and with the plane wave)
+    python bench.py --workload drude (1024^3, CPML 12, a Drude box [256, 767]^3 (wp 0.5,
+                       gamma 0.05) inside the sphere dielectric, in-place engine; steps
+                       timed without and with the Drude medium)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -64,7 +67,7 @@ def parse():
     p.add_argument("--engine", default="fused", choices=("fused", "kemp", "slab"))
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf", "drude"))
     p.add_argument("--instances", type=int, default=64)
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
@@ -373,6 +376,40 @@ def run_tfsf(args):
     }
 
 
+def run_drude(args):
+    """Drude dispersion: the default sphere workload (CPML on all faces) with a Drude box
+    [n/4, 3n/4)^3 (plasma frequency 0.5, damping 0.05 per unit time) on the in-place
+    engine.  The same grid is timed without and then with the Drude medium; the Drude
+    update reads E, CE, J, CJA, CJB and writes E and J (28 B per component, 84 B per box
+    cell)."""
+    import torch
+    from fdtd_accelerate_amd import Drude
+    a = argparse.Namespace(**vars(args))
+    a.engine = "kemp"
+    a.no_ce_lut = True  # the Drude update reads the float CE arrays
+    sim = build_sim(a)
+    n = args.n
+    p0, p1 = (n // 4,) * 3, (3 * n // 4 - 1,) * 3
+    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+    Drude(sim.fields, p0, p1, 0.5, 0.05)
+    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    cells = n ** 3
+    achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
+    return {
+        "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point "
+                               f"source, Drude box [{p0[0]},{p1[0]}]^3 (wp 0.5, gamma 0.05)", "engine": "kemp",
+                   "cells": cells, "drude_cells": (p1[0] - p0[0] + 1) ** 3},
+        "ms_per_step_without_drude": round(ms0, 4),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": kbytes, "avg_launch_ms": round(kern_ms, 4)},
+        "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
+    }
+
+
 def run_adjoint(args):
     """The 160^3 design problem: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
     64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
@@ -473,6 +510,9 @@ def main():
         if args.steps == 200:
             args.steps = 300
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "drude":
+        print(json.dumps(run_drude(args)), flush=True)
         return
     if args.workload == "tfsf":
         print(json.dumps(run_tfsf(args)), flush=True)

@@ -13,11 +13,12 @@ from .cpml import Cpml, cpml_coefficients, sigma_max
 from .incident import IncidentDirect, GetFields, SetFields
 from .pbc import Pbc
 from .tfsf import Tfsf
+from .drude import Drude
 from .sources import gaussian_pulse
 from . import geometry
 
 __all__ = [
-    "Fields", "Core", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc", "Tfsf",
+    "Fields", "Core", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc", "Tfsf", "Drude",
     "cpml_coefficients", "sigma_max", "gaussian_pulse", "geometry",
     "check_type", "check_value",
 ]

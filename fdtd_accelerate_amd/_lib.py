@@ -86,6 +86,7 @@ _SIGS = {
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),
     "mifdtd_gather": (_I, [_P, _P, _I, _P, _P]),
     "mifdtd_pbc": (_I, [_P, _P] + [_I] * 6 + [_P]),
+    "mifdtd_drude_e": (_I, [_P] * 5 + [_I] * 9 + [_P]),
     "mifdtd_tfsf_line": (_I, [_P, _P, _P, _I, _I, _F, _P]),
     "mifdtd_tfsf_faces": (_I, [_P, _P, _P, _I, _I, _I, _I, _P]),
     "mifdtd_fused_run": (_I, [ctypes.POINTER(FusedDesc), _I, _P, _P, ctypes.POINTER(_I), _P]),

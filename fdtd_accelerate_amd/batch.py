@@ -49,6 +49,8 @@ class BatchedYee:
                 raise ValueError("the batched engine has no periodic boundaries")
             if any(type(i).__name__ == "Tfsf" for i in s.fields.instance_list):
                 raise ValueError("the batched engine has no TFSF source")
+            if any(type(i).__name__ == "Drude" for i in s.fields.instance_list):
+                raise ValueError("the batched engine has no Drude medium")
         self.sims = list(sims)
         self.device = sims[0].fields.device
         self.stream = stream if stream is not None else torch.cuda.Stream(self.device)

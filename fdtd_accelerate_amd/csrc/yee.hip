@@ -672,6 +672,38 @@ __global__ __launch_bounds__(kBlock) void k_pbc(float *__restrict__ f1, float *_
     f2[d] = f2[s];
 }
 
+// ------------------------------------------------------------------ Drude dispersion
+// Auxiliary-current update of the region [i0, i0+li) x [j0, j0+lj) x [k0, k0+lk): per
+// component e' = e - ce J, J' = cja J + cjb e' (J, cja, cjb region-sized, C order; ce
+// the grid's E coefficient, 0.5 where the pointer is null).
+__global__ __launch_bounds__(kBlock) void k_drude_e(float *ex, float *ey, float *ez, const float *cex, const float *cey,
+                                                    const float *cez, float *jx, float *jy, float *jz,
+                                                    const float *__restrict__ ax, const float *__restrict__ ay,
+                                                    const float *__restrict__ az, const float *__restrict__ bx,
+                                                    const float *__restrict__ by, const float *__restrict__ bz, int ny,
+                                                    int nzp, int i0, int j0, int k0, int lj, int lk, unsigned count)
+{
+    const unsigned n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= count) return;
+    const int dk = (int)(n % (unsigned)lk);
+    const unsigned r = n / (unsigned)lk;
+    const int dj = (int)(r % (unsigned)lj), di = (int)(r / (unsigned)lj);
+    const size_t id = ((size_t)(i0 + di) * ny + (j0 + dj)) * nzp + (k0 + dk);
+    float *const e[3] = {ex, ey, ez};
+    const float *const ce[3] = {cex, cey, cez};
+    float *const J[3] = {jx, jy, jz};
+    const float *const A[3] = {ax, ay, az};
+    const float *const B[3] = {bx, by, bz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float f = J[c][n];
+        const float c0 = ce[c] ? ce[c][id] : 0.5f;
+        const float et = e[c][id] - c0 * f;
+        e[c][id] = et;
+        J[c][n] = A[c][n] * f + B[c][n] * et;
+    }
+}
+
 // ------------------------------------------------------------------ TFSF plane wave
 // The 1-D incident line (one work-group): E half: E[1:n-1] -= 0.5 (H[2:] -
 // H[1:n-1]), the two-step-delay ends (abc[0..1] front, abc[2..3] back), E[3] = src;
@@ -933,6 +965,26 @@ int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int 
     const unsigned g = (unsigned)((count + kBlock - 1) / kBlock);
     k_pbc<<<g, kBlock, 0, to_stream(stream)>>>(f1, f2, nx, ny, nz_pitch, axis, src, dst, (unsigned)count);
     MIFDTD_CHECK_LAUNCH("periodic copy launch");
+    return 0;
+}
+
+int mifdtd_drude_e(float *const e[3], const float *const ce[3], float *const j[3], const float *const cja[3],
+                   const float *const cjb[3], int nx, int ny, int nz_pitch, int i0, int j0, int k0, int li, int lj,
+                   int lk, mifdtd_stream_t stream)
+{
+    if (!e || !j || !cja || !cjb) return fail(MIFDTD_ENULL, "Drude argument is NULL");
+    for (int c = 0; c < 3; c++)
+        if (!e[c] || !j[c] || !cja[c] || !cjb[c]) return fail(MIFDTD_ENULL, "Drude array is NULL");
+    if (li < 1 || lj < 1 || lk < 1 || i0 < 0 || j0 < 0 || k0 < 0 || i0 + li > nx || j0 + lj > ny || k0 + lk > nz_pitch)
+        return fail(MIFDTD_EINVAL, "Drude region outside the grid");
+    const unsigned long long tot = (unsigned long long)li * lj * lk;
+    if (tot >= (1ull << 32) - kBlock) return fail(MIFDTD_ERANGE, "Drude region too large");
+    const float *c0 = ce ? ce[0] : nullptr, *c1 = ce ? ce[1] : nullptr, *c2 = ce ? ce[2] : nullptr;
+    const unsigned g = (unsigned)((tot + kBlock - 1) / kBlock);
+    k_drude_e<<<g, kBlock, 0, to_stream(stream)>>>(e[0], e[1], e[2], c0, c1, c2, j[0], j[1], j[2], cja[0], cja[1], cja[2],
+                                                   cjb[0], cjb[1], cjb[2], ny, nz_pitch, i0, j0, k0, lj, lk,
+                                                   (unsigned)tot);
+    MIFDTD_CHECK_LAUNCH("Drude launch");
     return 0;
 }
 

@@ -36,6 +36,8 @@ def _check_whole_step(sim, engine):
         raise ValueError(f"the {engine} engine has no periodic boundaries; use engine='kemp'")
     if any(type(i).__name__ == "Tfsf" for i in f.instance_list):
         raise ValueError(f"the {engine} engine has no TFSF source; use engine='kemp'")
+    if any(type(i).__name__ == "Drude" for i in f.instance_list):
+        raise ValueError(f"the {engine} engine has no Drude medium; use engine='kemp'")
     if len(sim.sources) > _lib.MAX_SRC:
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_SRC} sources")
     if len(sim.probe_points) > _lib.MAX_PROBE:

@@ -161,6 +161,15 @@ int mifdtd_source(float *f, int nx, int ny, int nz_pitch,
 int mifdtd_pbc(float *f1, float *f2, int nx, int ny, int nz_pitch, int pad, int axis, int e_half,
                mifdtd_stream_t stream);
 
+/* Drude dispersion (auxiliary current): replaces drude_ade.update_e of
+ * kuFDTD/v0.2.0/dim3/cpu/core/drude_ade.c:14-99 (called after the core E update,
+ * KU/dim3/cpu/matter.py:210-221).  On the region [i0, i0+li) x [j0, j0+lj) x
+ * [k0, k0+lk), per component c: e' = e - ce J, J' = cja J + cjb e'.  j, cja, cjb are
+ * region-sized C-order arrays; ce may be NULL (or hold NULLs) for the constant 0.5. */
+int mifdtd_drude_e(float *const e[3], const float *const ce[3], float *const j[3], const float *const cja[3],
+                   const float *const cjb[3], int nx, int ny, int nz_pitch, int i0, int j0, int k0, int li, int lj,
+                   int lk, mifdtd_stream_t stream);
+
 /* TFSF plane wave, normal incidence: replaces Tfsf.updateE / updateH of
  * kuFDTD/v0.2.0/dim3/tfsf.py:229-242 (Fdtd1d :14-40, faces and incident coefficients
  * :138-226).  mifdtd_tfsf_line advances the 1-D incident line of n points: e_half != 0:

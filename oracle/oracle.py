@@ -475,6 +475,43 @@ class OracleTfsf:
             ehs[comps[3]][sh2] += 0.5 * e2
 
 
+def drude_coefficients(plasma_freq, gamma, dt, dtype=np.float32):
+    """CJA = (2 - gamma dt) / (2 + gamma dt), CJB = 4 (dt wp)^2 / (2 + gamma dt)
+    (``KU/dim3/cpu/matter.py:192-202``), in the arrays' precision."""
+    g = np.asarray(gamma, dtype)
+    w = np.asarray(plasma_freq, dtype)
+    tmp = 2 + g * dt
+    return (2 - g * dt) / tmp, 4 * (dt * w) ** 2 / tmp
+
+
+class OracleDrude:
+    """Drude dispersion by the auxiliary-current method (``KU/dim3/cpu/core/drude_ade.c``,
+    ``KU/dim3/cpu/matter.py:104-221``): right after the core E update, on every cell of
+    the box [pt0, pt1] (inclusive) and each component c,
+        e' = e - CE_c J_c ;  J_c' = CJA_c J_c + CJB_c e'
+    (J, CJA, CJB region-sized; CE the grid's E coefficient, 0.5 without arrays).
+    Priority 'current' (after core E, before CPML-E).  Pinned by the plasma oscillation
+    of a uniform periodic medium: with zero curl the update is the 2x2 recurrence of
+    determinant CJA and trace 1 + CJA - CJB CE, so a uniform E oscillates at
+    arccos(1 - (wp dt)^2 / 2) per step (gamma = 0) and decays by sqrt(CJA) per step.
+    """
+
+    def __init__(self, pt0, pt1, cja, cjb, dtype=np.float32):
+        self.sl = tuple(slice(a, b + 1) for a, b in zip(pt0, pt1))
+        shp = tuple(b - a + 1 for a, b in zip(pt0, pt1))
+        self.cja = [np.broadcast_to(np.asarray(c, dtype), shp).copy() for c in cja]
+        self.cjb = [np.broadcast_to(np.asarray(c, dtype), shp).copy() for c in cjb]
+        self.J = [np.zeros(shp, dtype) for _ in range(3)]
+
+    def update_e(self, ehs, ces):
+        for c in range(3):
+            e = ehs[c][self.sl]
+            ce = ces[c][self.sl] if ces is not None else e.dtype.type(0.5)
+            e_tmp = e - ce * self.J[c]
+            ehs[c][self.sl] = e_tmp
+            self.J[c] = self.cja[c] * self.J[c] + self.cjb[c] * e_tmp
+
+
 class OracleSim:
     """CPU oracle of one Yee+CPML simulation on the KEMP layout.
 
@@ -491,6 +528,7 @@ class OracleSim:
         self.nx, self.ny, self.nz, self.pad = nx, ny, nz, pad
         self.pbc = pbc  # periodic axes, applied in this order
         self.tfsf = []  # OracleTfsf instances (priority 'incident', after the sources)
+        self.drude = []  # OracleDrude instances (priority 'current', after the core E update)
         self.nzp = nz + pad
         self.dtype = np.dtype(dtype)
         shp = (nx, ny, self.nzp)
@@ -533,6 +571,8 @@ class OracleSim:
 
     def step(self):
         update_e(self.ehs, self.ces, self.pad)
+        for d in self.drude:
+            d.update_e(self.ehs, self.ces)
         if self.npml:
             for ax in range(3):
                 for sd in range(2):

@@ -1,0 +1,106 @@
+"""Drude dispersion on the GPU (``Drude``, ``mifdtd_drude_e``) against the float32
+oracle: every field, the probe series and the auxiliary current byte-equal (-0 == +0)
+after the steps -- a Drude box with per-cell plasma frequency and damping inside a
+dielectric grid with CPML and a point source, a whole-grid uniform medium with periodic
+boundaries (the plasma oscillation), and the engines that reject a Drude medium."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from fdtd_accelerate_amd import _lib
+    _lib.load()
+
+
+def _same(a, b):
+    return a.shape == b.shape and (np.ascontiguousarray(a) + np.float32(0)).tobytes() == \
+        (np.ascontiguousarray(b) + np.float32(0)).tobytes()
+
+
+def test_drude_box_matches_oracle():
+    from fdtd_accelerate_amd import Drude
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, npml, steps = (26, 24, 29), 4, 60
+    p0, p1 = (9, 8, 10), (17, 15, 19)
+    shape = tuple(b - a + 1 for a, b in zip(p0, p1))
+    rng = np.random.default_rng(3)
+    ces = [(0.5 / rng.uniform(1, 4, ns)).astype(np.float32) for _ in range(3)]
+    wp = [rng.uniform(0.2, 1.5, shape).astype(np.float32) for _ in range(3)]
+    gamma = [rng.uniform(0.0, 0.3, shape).astype(np.float32) for _ in range(3)]
+    sim = YeeSim(*ns, npml=npml, coeff_use="e")
+    f = sim.fields
+    f.set_ces(*ces)
+    d = Drude(f, p0, p1, wp, gamma)
+    tf = gaussian_pulse(10.0, 4.0, 12.0)
+    src, prb = (13, 12, 14), (15, 12, 14)
+    sim.add_source("ez", src, src, tf)
+    sim.add_probe("ez", prb)
+    pad = f.pad
+    o = O.OracleSim(*ns, pad, npml, np.float32,
+                    ces=[np.pad(c, ((0, 0), (0, 0), (0, pad)), constant_values=np.float32(0.5)) for c in ces])
+    cja, cjb = zip(*[O.drude_coefficients(w, g, f.dt) for w, g in zip(wp, gamma)])
+    od = O.OracleDrude(p0, p1, cja, cjb)
+    o.drude.append(od)
+    o.add_source("ez", src, src, tf)
+    o.add_probe("ez", prb)
+    series = sim.run(steps)
+    o.run(steps)
+    assert _same(series, np.array(o.series, np.float32))
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+    for c in range(3):
+        assert _same(d.J[c].cpu().numpy(), od.J[c]), ("J", c)
+    assert np.abs(od.J[2]).max() > 1e-3
+
+
+def test_drude_uniform_periodic_plasma_oscillation():
+    from fdtd_accelerate_amd import Drude, Pbc
+    from fdtd_accelerate_amd.sim import YeeSim
+    n, wp, steps = 9, 0.6, 80
+    sim = YeeSim(n, n, n, npml=0, coeff_use="")
+    f = sim.fields
+    f.set_ehs(*[np.ones((n, n, n), np.float32)] * 3, *[np.zeros((n, n, n), np.float32)] * 3)
+    Drude(f, (0, 0, 0), (n - 1,) * 3, wp, 0.0)
+    Pbc(f, "xyz")
+    sim.add_probe("ez", (4, 4, 4))
+    series = sim.run(steps)[:, 0]
+    o = O.OracleSim(n, n, n, f.pad, 0, np.float32, pbc="xyz")
+    for k in range(3):
+        o.ehs[k][:, :, :n] = 1.0
+    cja, cjb = O.drude_coefficients(wp, 0.0, f.dt)
+    o.drude.append(O.OracleDrude((0, 0, 0), (n - 1,) * 3, [cja] * 3, [cjb] * 3))
+    o.add_probe("ez", (4, 4, 4))
+    o.run(steps)
+    assert _same(series, np.array(o.series, np.float32)[:, 0])
+    # float32 plasma oscillation: e(n+2) = (2 - (wp dt)^2) e(n+1) - e(n) up to round-off
+    c = 2.0 - (wp * f.dt) ** 2
+    s = series.astype(np.float64)
+    assert np.abs(s[2:] - (c * s[1:-1] - s[:-2])).max() < 1e-5
+
+
+def test_drude_arguments_and_engines():
+    from fdtd_accelerate_amd import Drude
+    from fdtd_accelerate_amd.batch import BatchedYee
+    from fdtd_accelerate_amd.sim import YeeSim
+    s = YeeSim(16, 16, 16, npml=0, coeff_use="")
+    with pytest.raises(ValueError):
+        Drude(s.fields, (0, 0, 0), (16, 3, 3), 1.0)
+    Drude(s.fields, (2, 2, 2), (5, 5, 5), 1.0, 0.1)
+    with pytest.raises(ValueError, match="Drude"):
+        BatchedYee([s])
+    for engine in ("fused", "slab"):
+        s2 = YeeSim(16, 16, 16, npml=0, coeff_use="", engine=engine)
+        Drude(s2.fields, (2, 2, 2), (5, 5, 5), 1.0)
+        with pytest.raises(ValueError, match="Drude"):
+            s2.run(1)
