@@ -24,6 +24,33 @@ def _device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+# Arrays at least this large are carved from one allocation with staggered starts.
+ARENA_MIN_BYTES = 1 << 30
+_ARENA_ROUND = 2 << 20
+
+
+def staggered_arrays(count, shape, dtype, device, zero=True):
+    """``count`` arrays of ``shape``.  When one array holds 1 GiB or more they are views
+    of one allocation, array k starting k * (size + gap) bytes in, gap = size / 8 rounded
+    up to 2 MiB, which fixes their relative placement: twelve separately allocated 4 GiB
+    arrays ran the 1024^3 fused step in 11.6-14.0 ms depending on where they landed
+    (13.7 ms as the first allocation of a process), carved this way 12.81-12.87 ms in six
+    allocations on two GPUs; the in-place step 16.47-16.54 ms against 16.08-16.97 ms.
+    Smaller arrays are allocated one by one."""
+    n = 1
+    for v in shape:
+        n *= v
+    esize = torch.empty((), dtype=dtype).element_size()
+    nbytes = n * esize
+    make = torch.zeros if zero else torch.empty
+    if nbytes < ARENA_MIN_BYTES:
+        return [make(shape, dtype=dtype, device=device) for _ in range(count)]
+    gap = -(-(nbytes // 8) // _ARENA_ROUND) * _ARENA_ROUND
+    step = (nbytes + gap) // esize
+    arena = make((count - 1) * step + n, dtype=dtype, device=device)
+    return [arena[k * step:k * step + n].view(shape) for k in range(count)]
+
+
 class Fields:
     def __init__(self, nx, ny, nz, coeff_use="", precision_float="single",
                  segment_nbytes=64, device=None, stream=None):
@@ -45,9 +72,6 @@ class Fields:
         self.device = torch.device(device) if device is not None else _device()
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
 
-        def zeros():
-            return torch.zeros((nx, ny, nz_pitch), dtype=dtype, device=self.device)
-
         self.nx, self.ny, self.nz = nx, ny, nz
         self.ns = [nx, ny, nz]
         self.ns_pitch = [nx, ny, nz_pitch]
@@ -65,7 +89,7 @@ class Fields:
         self.dx = 1.0
         self.dt = 0.5
 
-        self.ehs = [zeros() for _ in range(6)]
+        self.ehs = staggered_arrays(6, (nx, ny, nz_pitch), dtype, self.device)
         self.ces = [self._half() for _ in range(3)] if self.ce_on else None
         self.chs = [self._half() for _ in range(3)] if self.ch_on else None
         self.ce_mat = None  # per-component uint8 indices into ce_lut (compress_ce)

@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import fields as _fields
 from .autotune import fused_tiles, lds_fits, slab_planes
 
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
@@ -44,32 +45,17 @@ def _check_whole_step(sim, engine):
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_PROBE} probes")
 
 
-# Field arrays at least this large go into one allocation with staggered starts.
-_ARENA_MIN_BYTES = 1 << 30
-_ARENA_ROUND = 2 << 20
-
-
 def _ping_pong_sets(fields):
-    """The current and the alternate field set of the ping-pong step (6 arrays each).
-
-    Arrays of 1 GiB or more are carved from one allocation: ex, ey, ez, hx, hy, hz of
-    the current set, then of the alternate set, array k starting k * (size + gap) bytes
-    in, gap = size / 8 rounded up to 2 MiB.  That fixes their relative placement.  At
-    1024^3 (4 GiB arrays, 512 MiB gap) the step ran 12.81-12.87 ms in six such
-    allocations on two GPUs, where twelve separate allocations ran 11.6-14.0 ms
-    depending on where they landed (13.7 ms as the first allocation of a process) and
-    small gaps (4 KiB - 256 MiB) ran 13.2-13.7 ms.  The current values are copied in;
-    smaller arrays keep their own allocations."""
+    """The current and the alternate field set of the ping-pong step (6 arrays each):
+    ex, ey, ez, hx, hy, hz of the current set, then of the alternate set, carved together
+    by ``staggered_arrays`` (one allocation with staggered starts for arrays of 1 GiB and
+    more); the current values are copied in.  Smaller arrays keep the current set and get
+    a separately allocated alternate set."""
     cur = fields.ehs
     t0 = cur[0]
-    nbytes = t0.numel() * t0.element_size()
-    if nbytes < _ARENA_MIN_BYTES:
+    if t0.numel() * t0.element_size() < _fields.ARENA_MIN_BYTES:
         return list(cur), [torch.empty_like(t) for t in cur]
-    gap = -(-(nbytes // 8) // _ARENA_ROUND) * _ARENA_ROUND
-    step = (nbytes + gap) // t0.element_size()
-    n = t0.numel()
-    arena = torch.empty(11 * step + n, dtype=t0.dtype, device=t0.device)
-    views = [arena[k * step:k * step + n].view(t0.shape) for k in range(12)]
+    views = _fields.staggered_arrays(12, tuple(t0.shape), t0.dtype, t0.device, zero=False)
     with torch.cuda.stream(fields.stream):
         for v, t in zip(views[:6], cur):
             v.copy_(t)

@@ -124,13 +124,14 @@ def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
 
 
 def test_fused_field_arena(monkeypatch):
-    """Both field sets carved from one allocation with staggered starts (the layout the
-    fused engine uses for arrays of 1 GiB and more, forced here on a small grid): same
-    values as the in-place engine over odd and even step counts, all twelve arrays in
-    one storage at k * (size + gap) bytes, gap = size/8 rounded up to 2 MiB."""
-    from fdtd_accelerate_amd import fused as F
+    """Field sets carved from one allocation with staggered starts (the layout used for
+    arrays of 1 GiB and more, forced here on a small grid, for the in-place engine's six
+    arrays and the fused engine's twelve): the fused engine equals the in-place engine
+    over odd and even step counts, and its twelve arrays sit in one storage at
+    k * (size + gap) bytes, gap = size/8 rounded up to 2 MiB."""
+    from fdtd_accelerate_amd import fields as F
     from fdtd_accelerate_amd.sources import gaussian_pulse
-    monkeypatch.setattr(F, "_ARENA_MIN_BYTES", 1)
+    monkeypatch.setattr(F, "ARENA_MIN_BYTES", 1)
     ns, npml = (30, 26, 40), 4
     nx, ny, nz = ns
     sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", 3, 8)]
@@ -146,6 +147,8 @@ def test_fused_field_arena(monkeypatch):
             a = sims[0].field(name).cpu().numpy()
             b = sims[1].field(name).cpu().numpy()
             assert same_values(a, b), (steps, name, np.abs(a - b).max())
+    ref = sims[0].fields.ehs
+    assert all(t.untyped_storage().data_ptr() == ref[0].untyped_storage().data_ptr() for t in ref)
     st = sims[1]._fused
     arrays = sims[1].fields.ehs + st.alt
     base = arrays[0].untyped_storage().data_ptr()
