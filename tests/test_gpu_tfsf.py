@@ -97,3 +97,36 @@ def test_tfsf_arguments_and_engines():
         Tfsf(s2.fields, (3, 3, 3), (12, 12, 12), ("fb",) * 3, 8.0)
         with pytest.raises(ValueError, match="TFSF"):
             s2.run(1)
+
+
+def test_tfsf_plane_wave_on_drude_sphere_matches_oracle():
+    """A Drude sphere (per-cell plasma frequency inside a radius, 0 outside) in a TFSF
+    box with CPML around it: the scatterer case of both features together, byte-equal
+    to the oracle (Drude right after the core E update, the plane wave after the
+    sources, as their priorities order them)."""
+    from fdtd_accelerate_amd import Drude, Tfsf
+    from fdtd_accelerate_amd.sim import YeeSim
+    ns, npml, steps = (34, 32, 30), 5, 80
+    box0, box1 = (9, 8, 8), (24, 23, 21)
+    d0, d1 = (13, 12, 11), (20, 19, 18)
+    shape = tuple(b - a + 1 for a, b in zip(d0, d1))
+    g = np.indices(shape).astype(np.float32)
+    c = np.array([(s - 1) / 2 for s in shape], np.float32).reshape(3, 1, 1, 1)
+    inside = ((g - c) ** 2).sum(0) <= 3.2 ** 2
+    wp = np.where(inside, np.float32(0.9), np.float32(0)).astype(np.float32)
+    sim = YeeSim(*ns, npml=npml, coeff_use="")
+    f = sim.fields
+    t = Tfsf(f, box0, box1, ("fb", "fb", "fb"), 9.0, ("normal", "z"), 0.5)
+    dr = Drude(f, d0, d1, wp, 0.05)
+    o = O.OracleSim(*ns, f.pad, npml, np.float32)
+    o.tfsf.append(O.OracleTfsf(box0, box1, ("fb", "fb", "fb"), t.wdt, "z", 0.5))
+    cja, cjb = O.drude_coefficients(wp, np.full(shape, 0.05, np.float32), f.dt)
+    od = O.OracleDrude(d0, d1, [cja] * 3, [cjb] * 3)
+    o.drude.append(od)
+    sim.run(steps)
+    o.run(steps)
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+    for c_ in range(3):
+        assert _same(dr.J[c_].cpu().numpy(), od.J[c_]), ("J", c_)
+    assert max(np.abs(x).max() for x in od.J) > 1e-3
