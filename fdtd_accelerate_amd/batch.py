@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib, geometry
+from .fused import in_place_only
 from .sim import YeeSim
 from .sources import gaussian_pulse
 
@@ -45,12 +46,9 @@ class BatchedYee:
                 raise ValueError("the batched engine needs CPML on all six faces (or none)")
             if len(s.sources) > _lib.MAX_SRC or len(s.probe_points) > _lib.MAX_PROBE:
                 raise ValueError("too many sources or probes on an instance")
-            if any(getattr(i, "priority_type", "") == "pbc" for i in s.fields.instance_list):
-                raise ValueError("the batched engine has no periodic boundaries")
-            if any(type(i).__name__ == "Tfsf" for i in s.fields.instance_list):
-                raise ValueError("the batched engine has no TFSF source")
-            if any(type(i).__name__ == "Drude" for i in s.fields.instance_list):
-                raise ValueError("the batched engine has no Drude medium")
+            for cls, what in in_place_only():
+                if any(isinstance(i, cls) for i in s.fields.instance_list):
+                    raise ValueError(f"the batched engine has no {what}")
         self.sims = list(sims)
         self.device = sims[0].fields.device
         self.stream = stream if stream is not None else torch.cuda.Stream(self.device)

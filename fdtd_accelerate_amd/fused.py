@@ -25,6 +25,15 @@ from .autotune import fused_tiles, lds_fits, slab_planes
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 
 
+def in_place_only():
+    """Instance types only the in-place engine applies (it runs the Fields instance list;
+    the whole-step and batched engines take their step from a descriptor)."""
+    from .drude import Drude
+    from .pbc import Pbc
+    from .tfsf import Tfsf
+    return ((Pbc, "periodic boundaries"), (Tfsf, "TFSF source"), (Drude, "Drude medium"))
+
+
 def _check_whole_step(sim, engine):
     """Limits shared by the whole-step engines (same descriptor)."""
     f = sim.fields
@@ -33,12 +42,9 @@ def _check_whole_step(sim, engine):
     cp = sim.cpml
     if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
         raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
-    if any(getattr(i, "priority_type", "") == "pbc" for i in f.instance_list):
-        raise ValueError(f"the {engine} engine has no periodic boundaries; use engine='kemp'")
-    if any(type(i).__name__ == "Tfsf" for i in f.instance_list):
-        raise ValueError(f"the {engine} engine has no TFSF source; use engine='kemp'")
-    if any(type(i).__name__ == "Drude" for i in f.instance_list):
-        raise ValueError(f"the {engine} engine has no Drude medium; use engine='kemp'")
+    for cls, what in in_place_only():
+        if any(isinstance(i, cls) for i in f.instance_list):
+            raise ValueError(f"the {engine} engine has no {what}; use engine='kemp'")
     if len(sim.sources) > _lib.MAX_SRC:
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_SRC} sources")
     if len(sim.probe_points) > _lib.MAX_PROBE:
