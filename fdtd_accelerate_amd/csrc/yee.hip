@@ -728,10 +728,10 @@ __global__ __launch_bounds__(1024) void k_tfsf_line(float *__restrict__ e, float
     }
 }
 
-// Face correction of one patch: the box [i0..i1, j0..j1, k0..k1] of component comp
-// takes f += 0.5 * (coef * line[base + (coordinate along axis - box start)]).  One launch
-// per patch (P.n = 1): patches of one half step meet on the box edges, where a cell
-// takes two corrections in table order.
+// Face correction of the patches in P: the box [i0..i1, j0..j1, k0..k1] of component
+// comp takes f += 0.5 * (coef * line[base + (coordinate along axis - box start)]).
+// Patches of one component meet on the box edges, where a cell takes two corrections in
+// table order, so the patches of one launch are of distinct components.
 __global__ __launch_bounds__(kBlock) void k_tfsf_faces(mifdtd_tfsf_patches P, float *ex, float *ey, float *ez,
                                                        float *hx, float *hy, float *hz,
                                                        const float *__restrict__ line, int ny, int nzp)
@@ -1020,18 +1020,30 @@ int mifdtd_tfsf_faces(const mifdtd_tfsf_patches *p, float *const f[6], const flo
         if (tot >= (1ull << 31)) return fail(MIFDTD_ERANGE, "TFSF faces too large");
     }
     P.start[P.n] = (unsigned)tot;
-    // one launch per patch, in table order (edge cells take two corrections)
+    // Launch r holds the r-th patch (in table order) of every component: a cell that
+    // two patches correct (a box edge) takes them in table order, and the patches of
+    // one launch write different arrays.
+    int rank[MIFDTD_TFSF_MAX_PATCH], seen[6] = {0, 0, 0, 0, 0, 0}, nrank = 0;
     for (int q = 0; q < P.n; q++) {
+        rank[q] = seen[P.comp[q]]++;
+        if (rank[q] + 1 > nrank) nrank = rank[q] + 1;
+    }
+    for (int r = 0; r < nrank; r++) {
         mifdtd_tfsf_patches Q{};
-        Q.n = 1;
         Q.axis = P.axis;
-        Q.comp[0] = P.comp[q];
-        for (int m = 0; m < 6; m++) Q.box[0][m] = P.box[q][m];
-        Q.base[0] = P.base[q];
-        Q.coef[0] = P.coef[q];
-        Q.start[0] = 0;
-        Q.start[1] = P.start[q + 1] - P.start[q];  // cells of patch q (at least 1)
-        const unsigned g = (Q.start[1] + kBlock - 1) / kBlock;
+        unsigned cells = 0;
+        for (int q = 0; q < P.n; q++) {
+            if (rank[q] != r) continue;
+            const int m = Q.n++;
+            Q.comp[m] = P.comp[q];
+            for (int c = 0; c < 6; c++) Q.box[m][c] = P.box[q][c];
+            Q.base[m] = P.base[q];
+            Q.coef[m] = P.coef[q];
+            Q.start[m] = cells;
+            cells += P.start[q + 1] - P.start[q];  // at least 1 per patch
+        }
+        Q.start[Q.n] = cells;
+        const unsigned g = (cells + kBlock - 1) / kBlock;
         k_tfsf_faces<<<g, kBlock, 0, to_stream(stream)>>>(Q, f[0], f[1], f[2], f[3], f[4], f[5], line, ny, nz_pitch);
         MIFDTD_CHECK_LAUNCH("TFSF faces launch");
     }
