@@ -1,7 +1,8 @@
 """Drude dispersion on the GPU (``Drude``, ``mifdtd_drude_e``) against the float32
 oracle: every field, the probe series and the auxiliary current byte-equal (-0 == +0)
 after the steps -- a Drude box with per-cell plasma frequency and damping inside a
-dielectric grid with CPML and a point source, a whole-grid uniform medium with periodic
+dielectric grid with CPML and a point source (inside the CPML frame, over the whole grid
+and across two CPML slabs, the latter two from random fields), a whole-grid uniform medium with periodic
 boundaries (the plasma oscillation), and the engines that reject a Drude medium."""
 import numpy as np
 import pytest
@@ -27,20 +28,27 @@ def _same(a, b):
         (np.ascontiguousarray(b) + np.float32(0)).tobytes()
 
 
-def test_drude_box_matches_oracle():
+@pytest.mark.parametrize("p0,p1,random_init", [
+    ((9, 8, 10), (17, 15, 19), False),   # inside the CPML frame
+    ((0, 0, 0), (25, 23, 28), True),     # the whole grid: walls and every CPML slab
+    ((0, 5, 20), (6, 23, 28), True),     # a corner box across the x-front and z-back slabs
+], ids=["interior", "whole_grid", "cpml_corner"])
+def test_drude_box_matches_oracle(p0, p1, random_init):
     from fdtd_accelerate_amd import Drude
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse
     ns, npml, steps = (26, 24, 29), 4, 60
-    p0, p1 = (9, 8, 10), (17, 15, 19)
     shape = tuple(b - a + 1 for a, b in zip(p0, p1))
     rng = np.random.default_rng(3)
     ces = [(0.5 / rng.uniform(1, 4, ns)).astype(np.float32) for _ in range(3)]
     wp = [rng.uniform(0.2, 1.5, shape).astype(np.float32) for _ in range(3)]
     gamma = [rng.uniform(0.0, 0.3, shape).astype(np.float32) for _ in range(3)]
+    init = [rng.standard_normal(ns).astype(np.float32) for _ in range(6)] if random_init else None
     sim = YeeSim(*ns, npml=npml, coeff_use="e")
     f = sim.fields
     f.set_ces(*ces)
+    if init is not None:
+        f.set_ehs(*init)
     d = Drude(f, p0, p1, wp, gamma)
     tf = gaussian_pulse(10.0, 4.0, 12.0)
     src, prb = (13, 12, 14), (15, 12, 14)
@@ -49,6 +57,9 @@ def test_drude_box_matches_oracle():
     pad = f.pad
     o = O.OracleSim(*ns, pad, npml, np.float32,
                     ces=[np.pad(c, ((0, 0), (0, 0), (0, pad)), constant_values=np.float32(0.5)) for c in ces])
+    if init is not None:
+        for k in range(6):
+            o.ehs[k][:, :, :ns[2]] = init[k]
     cja, cjb = zip(*[O.drude_coefficients(w, g, f.dt) for w, g in zip(wp, gamma)])
     od = O.OracleDrude(p0, p1, cja, cjb)
     o.drude.append(od)
