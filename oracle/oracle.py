@@ -14,15 +14,26 @@ What it holds:
   ``make -C oracle ref`` into ``oracle/_ref`` (only where /root/reference exists).
 * ``cpml_tables`` / ``pulse`` — closed forms of the CPML profile and the
   Gaussian-modulated source (``KU/cpu/cpml.py:107-126``, ``KB/source/source.py:247-249``).
-* ``OracleSim`` — the full step (E core, CPML-E faces, E sources, periodic E planes, H
-  core, CPML-H faces, H sources, periodic H planes, probes) in the order of
-  ``KU/cpu/test/020-dielectric-cpml.py:58-73`` and the instance priorities of
-  ``K7/util/common.py:383-397``.
+* ``OracleSim`` — the full step (E core, Drude currents, CPML-E faces, E sources, TFSF
+  E faces, periodic E planes, H core, CPML-H faces, H sources, TFSF H faces, periodic H
+  planes, probes) in the order of ``KU/cpu/test/020-dielectric-cpml.py:58-73`` and the
+  instance priorities of ``K7/util/common.py:383-397``.
+* ``OracleS4Sim`` and ``s4_*`` — the 4th-order symplectic stages of
+  ``S4/Python-C/dielectric.c:5-96`` in the stage loop of ``S4/PyCUDA/100-base.py:124-141``
+  (the C stages are pinned against the Python restatement and the integrator's
+  closed-form amplification matrix), with the CPML this engine adds to that path.
 * ``pbc_copy`` — the periodic-boundary plane copy of ``K7/cpu/pbc.py:25-31`` with the
   plane table of ``K7/util/common_exchange.py:1-29``.  Parity pinned by the replication
   invariant (a P-periodic state stays P-periodic: a run over period P equals, bit for
   bit, the same state replicated over period 2P), which a wrong plane index, copy
   direction or component fails.
+* ``OracleTfsf`` — the normal-incidence plane wave of ``KU/dim3/tfsf.py:14-242``.
+  Pinned by the scattered-field invariant (in vacuum the field outside the box stays at
+  round-off level), which a flipped coefficient or a correction one plane off fails.
+* ``drude_coefficients`` / ``OracleDrude`` — the auxiliary-current update of
+  ``KU/dim3/cpu/core/drude_ade.c:14-99`` with the coefficients of
+  ``KU/dim3/cpu/matter.py:192-202``.  Pinned by the discrete plasma oscillation of a
+  uniform periodic medium and its damping factor.
 """
 from __future__ import annotations
 
