@@ -69,18 +69,17 @@ class DesignBox(ctypes.Structure):
 _SIGS = {
     "mifdtd_last_error": (ctypes.c_char_p, []),
     "mifdtd_version": (_I, []),
-    "mifdtd_set_pad": (_I, [_I]),
-    "update_e": (None, [_I] * 3 + [_P] * 9),
-    "update_h": (None, [_I] * 3 + [_P] * 9),
     "mifdtd_update_e": (_I, [_I] * 4 + [_P] * 9 + [_P]),
     "mifdtd_update_h": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "mifdtd_update_e_f64": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "mifdtd_update_h_f64": (_I, [_I] * 4 + [_P] * 9 + [_P]),
     "mifdtd_update_e_lut": (_I, [_I] * 4 + [_P] * 10 + [_P]),
-    "update_x_e": (None, [_I] * 5 + [_P] * 10),
-    "update_x_h": (None, [_I] * 5 + [_P] * 8),
-    "update_y_e": (None, [_I] * 5 + [_P] * 10),
-    "update_y_h": (None, [_I] * 5 + [_P] * 8),
-    "update_z_e": (None, [_I] * 5 + [_P] * 10),
-    "update_z_h": (None, [_I] * 5 + [_P] * 8),
+    "mifdtd_cpml_face_x_e": (_I, [_I] * 6 + [_P] * 10 + [_P]),
+    "mifdtd_cpml_face_x_h": (_I, [_I] * 6 + [_P] * 8 + [_P]),
+    "mifdtd_cpml_face_y_e": (_I, [_I] * 6 + [_P] * 10 + [_P]),
+    "mifdtd_cpml_face_y_h": (_I, [_I] * 6 + [_P] * 8 + [_P]),
+    "mifdtd_cpml_face_z_e": (_I, [_I] * 6 + [_P] * 10 + [_P]),
+    "mifdtd_cpml_face_z_h": (_I, [_I] * 6 + [_P] * 8 + [_P]),
     "mifdtd_cpml_e": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
     "mifdtd_cpml_h": (_I, [_I] * 4 + [ctypes.POINTER(CpmlHalf)] + [_P] * 9 + [_P]),
     "mifdtd_source": (_I, [_P] + [_I] * 9 + [_F, _P, _F, _I, _P]),

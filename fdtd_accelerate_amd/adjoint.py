@@ -45,6 +45,11 @@ class DesignProblem:
         i0, j0, k0, di, dj, dk = self.box
         if not (0 < i0 and i0 + di < nx and 0 < j0 and j0 + dj < ny - 1 and 0 < k0 and k0 + dk < nz - 1):
             raise ValueError(f"design box {self.box} must lie inside the grid {n}")
+        # the adjoint's gradient has no term for the CPML correction of E (-/+CE*psi), so
+        # the box keeps off every E slab plane: [npml, n-1-npml) on each axis
+        if npml and not (i0 >= npml and i0 + di <= nx - 1 - npml and j0 >= npml and j0 + dj <= ny - 1 - npml
+                         and k0 >= npml and k0 + dk <= nz - 1 - npml):
+            raise ValueError(f"design box {self.box} overlaps a CPML slab (npml {npml}) of the grid {n}")
         self.sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e")
         # Ez soft line source on the plane x = src_plane: the line y = ny/2 along z over the
         # non-CPML interior

@@ -36,6 +36,13 @@ class BatchedYee:
         npmls = {s.cpml.npml for s in sims if s.cpml is not None}
         if len(npmls) > 1:
             raise ValueError(f"all instances must share one npml : {sorted(npmls)}")
+        # one set of b/a tables serves the whole batch: every instance's CPML (grading m,
+        # R, kappa, alpha, dt) must give the same float32 tables
+        tabs = [s.cpml for s in sims if s.cpml is not None]
+        for cp in tabs[1:]:
+            if any(not torch.equal(x, y) for x, y in zip((cp.bE, cp.aE, cp.bH, cp.aH),
+                                                          (tabs[0].bE, tabs[0].aE, tabs[0].bH, tabs[0].aH))):
+                raise ValueError("all instances must share the same CPML tables (grading, R, kappa, alpha)")
         ce = {s.fields.ce_on for s in sims}
         if len(ce) > 1:
             raise ValueError("CE arrays must be used by every instance or by none")

@@ -1,9 +1,13 @@
-"""Builds ``libmifdtd.so`` (all HIP kernels + the C ABI) in-tree for gfx950.
+"""Builds ``libmifdtd.so`` (all HIP kernels + the C ABI) in-tree for gfx950, and the
+KEMP core program libraries over it.
 
 ``python -m fdtd_accelerate_amd.build`` compiles every ``csrc/*.hip`` to an object in
 ``_build/`` (in parallel) and links the shared library next to this file.  Flags:
   -ffp-contract=off   keep the reference's mul/add rounding (no FMA contraction)
   -Xclang -target-feature -Xclang -packed-fp32-ops   no packed-FP32 VALU forms
+It then compiles ``csrc/kemp_program.c`` once per KEMP program (precision, coeff_use,
+pad), as K7/cpu/core.py:52-85 generates one per combination, into
+``kemp/libkemp_<precision>_<coeff>_pad<p>.so``, each linked to libmifdtd.so.
 """
 from __future__ import annotations
 
@@ -18,6 +22,18 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmifdtd.so")
+KEMP_DIR = os.path.join(PKG, "kemp")
+CC = os.environ.get("CC", shutil.which("gcc") or "gcc")
+
+# (precision_float, pads) of the reference's programs: single pads nz to 4 floats, double
+# to 2 (K7/cpu/fields.py:27-32 with the 16-byte SSE width).
+KEMP_PRECISIONS = {"single": (0, 1, 2, 3), "double": (0, 1)}
+KEMP_COEFFS = ("", "e", "h", "eh")
+
+
+def kemp_program_path(precision_float: str, coeff_use: str, pad: int) -> str:
+    """Path of the program library of (precision_float, coeff_use, pad)."""
+    return os.path.join(KEMP_DIR, f"libkemp_{precision_float}_{coeff_use or 'none'}_pad{pad}.so")
 INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 
 ARCH = os.environ.get("MIFDTD_ARCH", "gfx950")
@@ -54,9 +70,33 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    progs = build_kemp_programs()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources and {len(progs)} KEMP programs", file=sys.stderr)
     return LIB
+
+
+def _compile_program(args):
+    prec, coeff, pad = args
+    out = kemp_program_path(prec, coeff, pad)
+    cmd = [CC, "-O2", "-std=c99", "-fPIC", "-shared", "-fvisibility=hidden", f"-I{INCLUDE}",
+           f"-DKEMP_DOUBLE={int(prec == 'double')}", f"-DKEMP_CE={int('e' in coeff)}",
+           f"-DKEMP_CH={int('h' in coeff)}", f"-DKEMP_PAD={pad}",
+           os.path.join(CSRC, "kemp_program.c"), "-o", out + ".tmp",
+           f"-L{PKG}", "-l:libmifdtd.so", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"KEMP program {prec}/{coeff or 'none'}/pad{pad} failed:\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_kemp_programs() -> list:
+    """Compile every KEMP program library (needs libmifdtd.so built)."""
+    os.makedirs(KEMP_DIR, exist_ok=True)
+    jobs = [(p, c, pad) for p, pads in KEMP_PRECISIONS.items() for c in KEMP_COEFFS for pad in pads]
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        return list(ex.map(_compile_program, jobs))
 
 
 if __name__ == "__main__":

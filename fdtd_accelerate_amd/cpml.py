@@ -60,6 +60,7 @@ class Cpml:
     def __init__(self, fields, npml, apply_direction=("fb", "fb", "fb"), m=4, R=1e-6,
                  kappa=1.0, alpha=0.0):
         util.check_type("fields", fields, Fields)
+        fields.require_single("Cpml")
         util.check_type("npml", npml, int)
         util.check_type("apply_direction", apply_direction, (list, tuple), str)
         if len(apply_direction) != 3 or any(set(d) - set("fb") for d in apply_direction):
@@ -152,3 +153,40 @@ class Cpml:
         chs = f.chs if f.ch_on else (None, None, None)
         _lib.call("mifdtd_cpml_h", f.nx, f.ny, f.nz_pitch, f.pad, self._descH, *f.ptrs(f.ehs),
                   *f.ptrs(chs), _lib.stream_handle(f.stream))
+
+
+_FACE_E = {0: ("ey", "ez", "hy", "hz", 1, 2), 1: ("ex", "ez", "hx", "hz", 0, 2), 2: ("ex", "ey", "hx", "hy", 0, 1)}
+
+
+def check_face_psi(fields, axis, npml, psi1, psi2):
+    """psi of one face must have this engine's shape (``psi_shape``), not kuFDTD's
+    (Npml, Ny-1, Nz), (Nx-1, Npml, Nz), (Nx-1, Ny-1, Npml+1) (``KU/cpu/cpml.py:35-85``),
+    which the face kernels would index past its end."""
+    nx, ny, nzp = fields.ns_pitch
+    want = psi_shape(axis, nx, ny, nzp, npml)
+    for name, p in (("psi1", psi1), ("psi2", psi2)):
+        if not isinstance(p, torch.Tensor):
+            raise TypeError(f"argument '{name}' type must be a torch.Tensor : {type(p)} is given")
+        if tuple(p.shape) != want or p.dtype != torch.float32 or not p.is_contiguous():
+            raise ValueError(f"argument '{name}' must be a contiguous float32 tensor of shape {want} "
+                             f"(x: (npml, ny, nz_pitch), y: (nx, npml, nz_pitch), z: (nx, ny, npml)) : "
+                             f"{tuple(p.shape)} {p.dtype} is given")
+
+
+def face_update(fields, half, axis, side, npml, psi1, psi2, b, a):
+    """One CPML face of one half step through ``mifdtd_cpml_face_<x|y|z>_<e|h>``
+    (kuFDTD's per-face call, ``KU/cpu/cpml.py:131-203``, on this engine's layout)."""
+    util.check_value("half", half, ("e", "h"))
+    util.check_value("axis", axis, (0, 1, 2))
+    util.check_value("side", side, (0, 1))
+    check_face_psi(fields, axis, npml, psi1, psi2)
+    f1, f2, g1, g2, c1, c2 = _FACE_E[axis]
+    g = fields.get
+    args = [fields.nx, fields.ny, fields.nz_pitch, fields.pad, npml, side]
+    if half == "e":
+        ces = (fields.ces[c1].data_ptr(), fields.ces[c2].data_ptr()) if fields.ce_on else (None, None)
+        args += [g(f1).data_ptr(), g(f2).data_ptr(), g(g1).data_ptr(), g(g2).data_ptr(), *ces]
+    else:
+        args += [g(f1).data_ptr(), g(f2).data_ptr(), g(g1).data_ptr(), g(g2).data_ptr()]
+    args += [psi1.data_ptr(), psi2.data_ptr(), b.data_ptr(), a.data_ptr(), _lib.stream_handle(fields.stream)]
+    _lib.call(f"mifdtd_cpml_face_{'xyz'[axis]}_{half}", *args)

@@ -386,6 +386,16 @@ static int check_box(const AGeom &g, const mifdtd_design_box *d)
     if (d->i0 < 0 || d->j0 < 0 || d->k0 < 0 || d->di < 1 || d->dj < 1 || d->dk < 1 || d->i0 + d->di > g.nx ||
         d->j0 + d->dj > g.ny || d->k0 + d->dk > g.nz)
         return fail(MIFDTD_EINVAL, "design box outside the grid");
+    // The gradient accumulates dE'/dCE = K H only: an E cell of a CPML slab also carries
+    // the -/+CE*psi term, whose derivative is not formed, so the box must keep off every
+    // E slab plane (front [0, npml), back [n-1-npml, n-1) on each axis).
+    if (g.npml > 0) {
+        const int np_ = g.npml;
+        if (d->i0 < np_ || d->j0 < np_ || d->k0 < np_ || d->i0 + d->di > g.nx - 1 - np_ ||
+            d->j0 + d->dj > g.ny - 1 - np_ || d->k0 + d->dk > g.nz - 1 - np_)
+            return fail(MIFDTD_EINVAL, "design box (%d,%d,%d)+(%d,%d,%d) overlaps a CPML slab (npml %d)", d->i0, d->j0,
+                        d->k0, d->di, d->dj, d->dk, np_);
+    }
     return 0;
 }
 

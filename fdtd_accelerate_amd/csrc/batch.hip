@@ -91,6 +91,7 @@ int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE,
     std::vector<unsigned> first(ninst + 1, 0);
     const bool hc = inst[0].ce[0] != nullptr;
     bool pml = false;
+    int batch_npml = 0;  // the one npml of the shared 2*(npml+1)-entry tables
     int tsrc = 0, tprobe = 0;
     unsigned long long blocks = 0;
     for (int n = 0; n < ninst; n++) {
@@ -117,6 +118,10 @@ int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE,
         if (d.npml > 0) {
             const int np_ = d.npml;
             if (np_ > MIFDTD_MAX_NPML) return fail(MIFDTD_EINVAL, "instance %d: npml %d too large", n, np_);
+            if (batch_npml && np_ != batch_npml)
+                return fail(MIFDTD_EINVAL, "instance %d: npml %d differs from the batch's %d (the tables are shared)", n,
+                            np_, batch_npml);
+            batch_npml = np_;
             if (nx < 2 * np_ + 2 || ny < 2 * np_ + 2 || nz < 2 * np_ + 2)
                 return fail(MIFDTD_EINVAL, "instance %d: grid too small for npml %d", n, np_);
             if (!bE || !aE || !bH || !aH) return fail(MIFDTD_ENULL, "CPML tables are NULL");

@@ -758,28 +758,11 @@ __global__ __launch_bounds__(kBlock) void k_tfsf_faces(mifdtd_tfsf_patches P, fl
 // =========================================================================== C ABI
 using namespace mifdtd;
 
-static int g_pad = 0;
-static std::mutex g_pad_mu;
-
-static int current_pad()
-{
-    std::lock_guard<std::mutex> lk(g_pad_mu);
-    return g_pad;
-}
-
 extern "C" {
 
 const char *mifdtd_last_error(void) { return g_err.c_str(); }
 
 int mifdtd_version(void) { return 1; }
-
-int mifdtd_set_pad(int pad)
-{
-    if (pad < 0) return fail(MIFDTD_EINVAL, "pad %d < 0", pad);
-    std::lock_guard<std::mutex> lk(g_pad_mu);
-    g_pad = pad;
-    return 0;
-}
 
 int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey, float *ez,
                     const float *hx, const float *hy, const float *hz,
@@ -820,18 +803,6 @@ int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad, const float *ex, cons
                        const_cast<float *>(ez), hx, hy, hz, chx, chy, chz, to_stream(stream));
 }
 
-void update_e(int nx, int ny, int nz, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
-              float *cex, float *cey, float *cez)
-{
-    mifdtd_update_e(nx, ny, nz, current_pad(), ex, ey, ez, hx, hy, hz, cex, cey, cez, nullptr);
-}
-
-void update_h(int nx, int ny, int nz, float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
-              float *chx, float *chy, float *chz)
-{
-    mifdtd_update_h(nx, ny, nz, current_pad(), ex, ey, ez, hx, hy, hz, chx, chy, chz, nullptr);
-}
-
 int mifdtd_cpml_e(int nx, int ny, int nzp, int pad, const mifdtd_cpml_half *cp,
                   float *ex, float *ey, float *ez, const float *hx, const float *hy, const float *hz,
                   const float *cex, const float *cey, const float *cez, mifdtd_stream_t stream)
@@ -864,30 +835,32 @@ int mifdtd_cpml_h(int nx, int ny, int nzp, int pad, const mifdtd_cpml_half *cp,
     return 0;
 }
 
-// kuFDTD per-face forms (KU/cpu/core/cpml_non_kappa.c argument order, Ncore dropped).
-static int face_e(int axis, int nx, int ny, int nz, int npml, int backward, float *f1, float *f2,
-                  float *g1, float *g2, float *c1, float *c2, float *psi1, float *psi2, float *b, float *a)
+// Per-face forms (kuFDTD's argument order, Ncore dropped; pad and stream explicit).
+static int face_e(int axis, int nx, int ny, int nz, int pad, int npml, int backward, float *f1, float *f2,
+                  const float *g1, const float *g2, const float *c1, const float *c2, float *psi1, float *psi2,
+                  const float *b, const float *a, mifdtd_stream_t stream)
 {
-    const int pad = current_pad();
     if (int rc = check_cpml(nx, ny, nz, pad, npml, b, a)) return rc;
     if (backward != 0 && backward != 1) return fail(MIFDTD_EINVAL, "backward must be 0 or 1");
     if (!psi1 || !psi2) return fail(MIFDTD_ENULL, "psi is NULL");
     FaceSet fs{{nullptr, nullptr}, {nullptr, nullptr}};
     fs.psi1[backward] = psi1;
     fs.psi2[backward] = psi2;
-    float *ex = nullptr, *ey = nullptr, *ez = nullptr, *cex = nullptr, *cey = nullptr, *cez = nullptr;
+    float *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    const float *cex = nullptr, *cey = nullptr, *cez = nullptr;
     const float *hx = nullptr, *hy = nullptr, *hz = nullptr;
     if (axis == 0) { ey = f1; ez = f2; hy = g1; hz = g2; cey = c1; cez = c2; }
     if (axis == 1) { ex = f1; ez = f2; hx = g1; hz = g2; cex = c1; cez = c2; }
     if (axis == 2) { ex = f1; ey = f2; hx = g1; hy = g2; cex = c1; cey = c2; }
     if ((c1 == nullptr) != (c2 == nullptr)) return fail(MIFDTD_ENULL, "CE arrays must be both set or both NULL");
-    return cpml_e_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, c1 != nullptr, cex, cey, cez, b, a, nullptr);
+    return cpml_e_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, c1 != nullptr, cex, cey, cez, b, a,
+                       to_stream(stream));
 }
 
-static int face_h(int axis, int nx, int ny, int nz, int npml, int backward, float *f1, float *f2,
-                  float *g1, float *g2, float *psi1, float *psi2, float *b, float *a)
+static int face_h(int axis, int nx, int ny, int nz, int pad, int npml, int backward, const float *f1,
+                  const float *f2, float *g1, float *g2, float *psi1, float *psi2, const float *b, const float *a,
+                  mifdtd_stream_t stream)
 {
-    const int pad = current_pad();
     if (int rc = check_cpml(nx, ny, nz, pad, npml, b, a)) return rc;
     if (backward != 0 && backward != 1) return fail(MIFDTD_EINVAL, "backward must be 0 or 1");
     if (!psi1 || !psi2) return fail(MIFDTD_ENULL, "psi is NULL");
@@ -899,38 +872,45 @@ static int face_h(int axis, int nx, int ny, int nz, int npml, int backward, floa
     if (axis == 0) { ey = f1; ez = f2; hy = g1; hz = g2; }
     if (axis == 1) { ex = f1; ez = f2; hx = g1; hz = g2; }
     if (axis == 2) { ex = f1; ey = f2; hx = g1; hy = g2; }
-    return cpml_h_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, false, nullptr, nullptr, nullptr, b, a, nullptr);
+    return cpml_h_axis(axis, nx, ny, nz, pad, npml, fs, ex, ey, ez, hx, hy, hz, false, nullptr, nullptr, nullptr, b, a,
+                       to_stream(stream));
 }
 
-void update_x_e(int nx, int ny, int nz, int npml, int backward, float *ey, float *ez, float *hy, float *hz,
-                float *cey, float *cez, float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_x_e(int nx, int ny, int nzp, int pad, int npml, int backward, float *ey, float *ez,
+                         const float *hy, const float *hz, const float *cey, const float *cez, float *psi1,
+                         float *psi2, const float *rcmb, const float *rcma, mifdtd_stream_t stream)
 {
-    face_e(0, nx, ny, nz, npml, backward, ey, ez, hy, hz, cey, cez, psi1, psi2, rcmb, rcma);
+    return face_e(0, nx, ny, nzp, pad, npml, backward, ey, ez, hy, hz, cey, cez, psi1, psi2, rcmb, rcma, stream);
 }
-void update_x_h(int nx, int ny, int nz, int npml, int backward, float *ey, float *ez, float *hy, float *hz,
-                float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_x_h(int nx, int ny, int nzp, int pad, int npml, int backward, const float *ey, const float *ez,
+                         float *hy, float *hz, float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream)
 {
-    face_h(0, nx, ny, nz, npml, backward, ey, ez, hy, hz, psi1, psi2, rcmb, rcma);
+    return face_h(0, nx, ny, nzp, pad, npml, backward, ey, ez, hy, hz, psi1, psi2, rcmb, rcma, stream);
 }
-void update_y_e(int nx, int ny, int nz, int npml, int backward, float *ex, float *ez, float *hx, float *hz,
-                float *cex, float *cez, float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_y_e(int nx, int ny, int nzp, int pad, int npml, int backward, float *ex, float *ez,
+                         const float *hx, const float *hz, const float *cex, const float *cez, float *psi1,
+                         float *psi2, const float *rcmb, const float *rcma, mifdtd_stream_t stream)
 {
-    face_e(1, nx, ny, nz, npml, backward, ex, ez, hx, hz, cex, cez, psi1, psi2, rcmb, rcma);
+    return face_e(1, nx, ny, nzp, pad, npml, backward, ex, ez, hx, hz, cex, cez, psi1, psi2, rcmb, rcma, stream);
 }
-void update_y_h(int nx, int ny, int nz, int npml, int backward, float *ex, float *ez, float *hx, float *hz,
-                float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_y_h(int nx, int ny, int nzp, int pad, int npml, int backward, const float *ex, const float *ez,
+                         float *hx, float *hz, float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream)
 {
-    face_h(1, nx, ny, nz, npml, backward, ex, ez, hx, hz, psi1, psi2, rcmb, rcma);
+    return face_h(1, nx, ny, nzp, pad, npml, backward, ex, ez, hx, hz, psi1, psi2, rcmb, rcma, stream);
 }
-void update_z_e(int nx, int ny, int nz, int npml, int backward, float *ex, float *ey, float *hx, float *hy,
-                float *cex, float *cey, float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_z_e(int nx, int ny, int nzp, int pad, int npml, int backward, float *ex, float *ey,
+                         const float *hx, const float *hy, const float *cex, const float *cey, float *psi1,
+                         float *psi2, const float *rcmb, const float *rcma, mifdtd_stream_t stream)
 {
-    face_e(2, nx, ny, nz, npml, backward, ex, ey, hx, hy, cex, cey, psi1, psi2, rcmb, rcma);
+    return face_e(2, nx, ny, nzp, pad, npml, backward, ex, ey, hx, hy, cex, cey, psi1, psi2, rcmb, rcma, stream);
 }
-void update_z_h(int nx, int ny, int nz, int npml, int backward, float *ex, float *ey, float *hx, float *hy,
-                float *psi1, float *psi2, float *rcmb, float *rcma)
+int mifdtd_cpml_face_z_h(int nx, int ny, int nzp, int pad, int npml, int backward, const float *ex, const float *ey,
+                         float *hx, float *hy, float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream)
 {
-    face_h(2, nx, ny, nz, npml, backward, ex, ey, hx, hy, psi1, psi2, rcmb, rcma);
+    return face_h(2, nx, ny, nzp, pad, npml, backward, ex, ey, hx, hy, psi1, psi2, rcmb, rcma, stream);
 }
 
 int mifdtd_source(float *f, int nx, int ny, int nzp, int i0, int i1, int j0, int j1, int k0, int k1,

@@ -33,6 +33,7 @@ class Drude:
 
     def __init__(self, fields, pt0, pt1, plasma_freq, gamma=0.0):
         util.check_type("fields", fields, Fields)
+        fields.require_single("Drude")
         p0, p1 = [int(v) for v in pt0], [int(v) for v in pt1]
         for a, n in enumerate(fields.ns):
             if not 0 <= p0[a] <= p1[a] < n:

@@ -58,15 +58,16 @@ class Fields:
         util.check_type("ny", ny, int)
         util.check_type("nz", nz, int)
         util.check_value("coeff_use", coeff_use, ("", "e", "h", "eh"))
-        util.check_value("precision_float", precision_float, ("single",))
+        util.check_value("precision_float", precision_float, ("single", "double"))
         util.check_type("segment_nbytes", segment_nbytes, int)
         if segment_nbytes % 16 or segment_nbytes <= 0:
             raise ValueError(f"argument 'segment_nbytes' must be a positive multiple of 16 : {segment_nbytes}")
         if nx < 2 or ny < 2 or nz < 2:
             raise ValueError(f"grid ({nx}, {ny}, {nz}) is too small")
 
-        dtype = torch.float32
-        align = segment_nbytes // 4
+        dtype = torch.float32 if precision_float == "single" else torch.float64
+        esize = 4 if precision_float == "single" else 8
+        align = segment_nbytes // esize
         pad = int(np.ceil(nz / align) * align) - nz
         nz_pitch = nz + pad
         self.device = torch.device(device) if device is not None else _device()
@@ -81,7 +82,7 @@ class Fields:
         self.pad = pad
         self.slice_z = slice(None) if pad == 0 else slice(None, -pad)
         self.precision_float = precision_float
-        self.dtype = np.float32
+        self.dtype = np.float32 if precision_float == "single" else np.float64
         self.torch_dtype = dtype
         self.coeff_use = coeff_use
         self.ce_on = "e" in coeff_use
@@ -122,13 +123,19 @@ class Fields:
             return self.ces["xyz".index(str_f[2])]
         return self.chs["xyz".index(str_f[2])]
 
+    def require_single(self, what):
+        """The update objects other than Core run single precision only."""
+        if self.precision_float != "single":
+            raise ValueError(f"{what} supports precision_float 'single' only : {self.precision_float!r} is given "
+                             "(the 'double' programs cover the core update_e/update_h)")
+
     def append_instance(self, instance):
         util.append_instance(self.instance_list, instance)
 
     def _fill(self, dst_list, src_list, what):
         for d, s in zip(dst_list, src_list):
             if isinstance(s, np.ndarray):
-                s = torch.from_numpy(np.ascontiguousarray(s, dtype=np.float32))
+                s = torch.from_numpy(np.ascontiguousarray(s, dtype=self.dtype))
             elif not isinstance(s, torch.Tensor):
                 raise TypeError(f"argument '{what}' type must be a numpy.ndarray or torch.Tensor : {type(s)} is given")
             if tuple(s.shape) != (self.nx, self.ny, self.nz):
@@ -151,7 +158,7 @@ class Fields:
         unchanged).  Possible when each component has at most 256 distinct values;
         returns whether it was done.  The float arrays stay (the CPML kernels and host
         access use them); call again after changing CE."""
-        if not self.ce_on:
+        if not self.ce_on or self.precision_float != "single":
             return False
         lut = torch.zeros((3, 256), dtype=torch.float32, device=self.device)
         mats = []

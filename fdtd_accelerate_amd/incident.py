@@ -34,6 +34,7 @@ class IncidentDirect:
 
     def __init__(self, fields, str_f, pt0, pt1, tfunc, spatial_value=1.0, is_overwrite=False):
         util.check_type("fields", fields, Fields)
+        fields.require_single("IncidentDirect")
         util.check_value("str_f", str_f, _FIELD_NAMES)
         if not callable(tfunc):
             raise TypeError(f"argument 'tfunc' type must be a function : {type(tfunc)} is given")
@@ -91,6 +92,7 @@ class GetFields:
 
     def __init__(self, fields, str_f, pt0, pt1):
         util.check_type("fields", fields, Fields)
+        fields.require_single("GetFields")
         util.check_type("str_f", str_f, (str, list, tuple), str)
         str_fs = (str_f,) if isinstance(str_f, str) else tuple(str_f)
         for s in str_fs:
@@ -122,6 +124,7 @@ class SetFields:
 
     def __init__(self, fields, str_f, pt0, pt1, is_array=False, is_overwrite=True):
         util.check_type("fields", fields, Fields)
+        fields.require_single("SetFields")
         util.check_type("str_f", str_f, (str, list, tuple), str)
         util.check_type("is_array", is_array, bool)
         util.check_type("is_overwrite", is_overwrite, bool)

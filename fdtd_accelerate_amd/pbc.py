@@ -23,6 +23,7 @@ class Pbc:
 
     def __init__(self, fields, axes):
         util.check_type("fields", fields, Fields)
+        fields.require_single("Pbc")
         util.check_type("axes", axes, str)
         if not set(axes) & set("xyz") or set(axes) - set("xyz"):
             raise AssertionError(f"axes option is wrong: {axes!r} is given")

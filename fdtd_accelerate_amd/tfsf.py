@@ -41,6 +41,7 @@ class Tfsf:
     def __init__(self, fields, pt0, pt1, apply_direction, wavelength, propagation_direction=("normal", "x"),
                  polarization_angle=0.0):
         util.check_type("fields", fields, Fields)
+        fields.require_single("Tfsf")
         util.check_type("apply_direction", apply_direction, (list, tuple), str)
         if len(apply_direction) != 3 or any(set(d) - set("fb") for d in apply_direction):
             raise ValueError(f"argument 'apply_direction' must be three strings of 'f'/'b' : {apply_direction!r}")

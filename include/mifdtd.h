@@ -10,7 +10,8 @@
  * Functions returning int return 0 on success, a positive hipError_t from the
  * runtime, or a negative MIFDTD_E* code for rejected arguments; the message of the
  * last failure on the calling thread is available from mifdtd_last_error().
- * The void functions (the reference's own names and argument orders) report
+ * The void functions (the reference's own names and argument orders: s4_update_h/e
+ * here, update_e/update_h in the KEMP program libraries of mifdtd_kemp.h) report
  * failures only through mifdtd_last_error(), as the reference's void kernels have no
  * error channel (K7/cpu/src/core.c:11,59).
  *
@@ -38,28 +39,25 @@ const char *mifdtd_last_error(void);
 int mifdtd_version(void);
 
 /* ------------------------------------------------------------------ core ----
- * Replaces the KEMP ctypes core generated from K7/cpu/src/core.c:11 (update_e) and
- * :59 (update_h), bound at K7/cpu/core.py:86-105 (argtypes: 3 ints, 6 or 9 arrays),
- * and the OpenCL kernels K7/gpu/src/core.cl:3,32 (same argument order).
- * nz is nz_pitch, as in the reference.  The pad (masked lanes) that the reference
- * bakes into each generated program (K7/cpu/core.py:33-49) is set once with
- * mifdtd_set_pad().  cex..cez / chx..chz may be NULL, meaning the constant 0.5
- * (the coefficient-free programs, K7/cpu/core.py:63-65).
- * Semantics (masks as K7/cpu/src/core.c:31,41,51,79-80,90-91,101):
+ * The KEMP core programs.  The reference generates one shared library per
+ * (precision_float, coeff_use, pad) from K7/cpu/src/core.c:11 (update_e) and :59
+ * (update_h) (K7/cpu/core.py:52-85) and binds update_e/update_h with 3 ints and 6 or 9
+ * arrays (K7/cpu/core.py:89-105), the pad baked into the program's masks
+ * (K7/cpu/core.py:33-49).  The same programs exist here as one small library each
+ * (libkemp_<single|double>_<none|e|h|eh>_pad<p>.so, declared in mifdtd_kemp.h), each
+ * forwarding to the status-returning calls below with its own pad.  libmifdtd.so
+ * itself exports no update_e/update_h symbol.
+ * Semantics (masks as K7/cpu/src/core.c:31,41,51,79-80,90-91,101), nz = nz_pitch - pad:
  *   ex += cex*((hz[j+1]-hz)-(hy[k+1]-hy))  j<ny-1,          lanes k>=nz-1 -> 0
  *   ey += cey*((hx[k+1]-hx)-(hz[i+1]-hz))  i<nx-1,          lanes k>=nz-1 -> 0
  *   ez += cez*((hy[i+1]-hy)-(hx[j+1]-hx))  i<nx-1 && j<ny-1, pad lanes -> 0
  *   hx -= chx*((ez-ez[j-1])-(ey-ey[k-1]))  j>0,             lane 0 and pad -> 0
  *   hy -= chy*((ex-ex[k-1])-(ez-ez[i-1]))  i>0,             lane 0 and pad -> 0
  *   hz -= chz*((ey-ey[i-1])-(ex-ex[j-1]))  i>0 && j>0,      pad lanes -> 0
- */
-int mifdtd_set_pad(int pad);
-void update_e(int nx, int ny, int nz, float *ex, float *ey, float *ez,
-              float *hx, float *hy, float *hz, float *cex, float *cey, float *cez);
-void update_h(int nx, int ny, int nz, float *ex, float *ey, float *ez,
-              float *hx, float *hy, float *hz, float *chx, float *chy, float *chz);
-
-/* Status-returning, stream-taking forms of the two calls above (explicit pad). */
+ * A masked lane stores value*0 (the reference's SSE mask multiply), so signed zeros
+ * match.  NULL coefficient arrays mean the constant 0.5 (K7/cpu/core.py:63-65). */
+/* update_e / update_h of the 'single' programs, with the pad explicit, a status return
+ * and a stream. */
 int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad,
                     float *ex, float *ey, float *ez,
                     const float *hx, const float *hy, const float *hz,
@@ -70,6 +68,20 @@ int mifdtd_update_h(int nx, int ny, int nz_pitch, int pad,
                     float *hx, float *hy, float *hz,
                     const float *chx, const float *chy, const float *chz,
                     mifdtd_stream_t stream);
+
+/* The same for the 'double' programs (K7/cpu/core.py:52-54, DTYPE double, 2 lanes;
+ * K7/cpu/fields.py:25): float64 arrays, nz_pitch even (16-byte rows), 16-byte aligned
+ * pointers, pad in [0, nz_pitch-2]. */
+int mifdtd_update_e_f64(int nx, int ny, int nz_pitch, int pad,
+                        double *ex, double *ey, double *ez,
+                        const double *hx, const double *hy, const double *hz,
+                        const double *cex, const double *cey, const double *cez,
+                        mifdtd_stream_t stream);
+int mifdtd_update_h_f64(int nx, int ny, int nz_pitch, int pad,
+                        const double *ex, const double *ey, const double *ez,
+                        double *hx, double *hy, double *hz,
+                        const double *chx, const double *chy, const double *chz,
+                        mifdtd_stream_t stream);
 
 /* update_e with the CE arrays given as per-component uint8 material indices
  * (mat_c, same shape as the fields, 4-byte aligned) into lut (DEVICE, 3 x 256 float32:
@@ -84,12 +96,17 @@ int mifdtd_update_e_lut(int nx, int ny, int nz_pitch, int pad,
                         const unsigned char *mat_z, const float *lut, mifdtd_stream_t stream);
 
 /* ------------------------------------------------------------------ CPML ----
- * Replaces kuFDTD's non-kappa CPML C extension, KU/cpu/core/cpml_non_kappa.c:14
- * (update_x_e), :91 (update_x_h), :156 (update_y_e), :227 (update_y_h),
- * :294 (update_z_e), :365 (update_z_h), called from KU/cpu/cpml.py:131-203, and the
- * CUDA kernels KU/gpu/core/cpml_non_kapa.cu:17-130.  Argument order is kuFDTD's with
- * the CPU thread count dropped and nz = nz_pitch; the pad is the one set by
- * mifdtd_set_pad().  rcmb/rcma are DEVICE tables of 2*(npml+1) floats in kuFDTD's
+ * CPML face updates on the KEMP field layout.  The per-face arithmetic, the signs and
+ * the argument order follow kuFDTD's non-kappa CPML (KU/cpu/core/cpml_non_kappa.c:14
+ * update_x_e, :91 update_x_h, :156 update_y_e, :227 update_y_h, :294 update_z_e, :365
+ * update_z_h; KU/gpu/core/cpml_non_kapa.cu:17-130), with the CPU thread count dropped
+ * and the pad, a status return and a stream added.  They are NOT binary drop-ins for
+ * kuFDTD's buffers, hence the mifdtd_ prefix: kuFDTD's fields carry a ghost plane and
+ * row at index 0 and its psi arrays are (Npml, Ny-1, Nz), (Nx-1, Npml, Nz) and
+ * (Nx-1, Ny-1, Npml+1) with the front slab from plane 1 (KU/cpu/cpml.py:35-85,
+ * cpml_non_kappa.c:60); here the fields are KEMP's (no ghost layer, nz = nz_pitch) and
+ * the psi shapes and slab planes are those below.  The Python Cpml checks psi shapes
+ * before any call.  rcmb/rcma are DEVICE tables of 2*(npml+1) floats in kuFDTD's
  * layout (KU/cpu/cpml.py:107-126): E front entry p+1, E back npml+1+p; H front p+1,
  * H back npml+2+p.
  *   x: psi1->ey from hz, psi2->ez from hy   psi shape (npml, ny, nz_pitch)
@@ -98,27 +115,34 @@ int mifdtd_update_e_lut(int nx, int ny, int nz_pitch, int pad,
  *   E:  psi = b*psi + a*(H(+1)-H);  E_1 -= CE*psi1;  E_2 += CE*psi2
  *   H:  psi = b*psi + a*(E-E(-1));  H_1 += 0.5*psi1; H_2 -= 0.5*psi2
  * Slab planes: E front p, E back n_e-1-npml+p, H front p, H back n_e-npml+p
- * (p in [0,npml), n_e = nx, ny or nz); updates only cells the core writes.
- * Requires n_e >= 2*npml+2 on the axis and 1 <= npml <= MIFDTD_MAX_NPML. */
+ * (p in [0,npml), n_e = nx, ny or nz - pad); updates only cells the core writes.
+ * Requires n_e >= 2*npml+2 on the axis and 1 <= npml <= MIFDTD_MAX_NPML.
+ * cex..cez may be NULL (the constant 0.5). */
 #define MIFDTD_MAX_NPML 64
-void update_x_e(int nx, int ny, int nz, int npml, int backward,
-                float *ey, float *ez, float *hy, float *hz, float *cey, float *cez,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
-void update_x_h(int nx, int ny, int nz, int npml, int backward,
-                float *ey, float *ez, float *hy, float *hz,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
-void update_y_e(int nx, int ny, int nz, int npml, int backward,
-                float *ex, float *ez, float *hx, float *hz, float *cex, float *cez,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
-void update_y_h(int nx, int ny, int nz, int npml, int backward,
-                float *ex, float *ez, float *hx, float *hz,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
-void update_z_e(int nx, int ny, int nz, int npml, int backward,
-                float *ex, float *ey, float *hx, float *hy, float *cex, float *cey,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
-void update_z_h(int nx, int ny, int nz, int npml, int backward,
-                float *ex, float *ey, float *hx, float *hy,
-                float *psi1, float *psi2, float *rcmb, float *rcma);
+int mifdtd_cpml_face_x_e(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         float *ey, float *ez, const float *hy, const float *hz,
+                         const float *cey, const float *cez, float *psi1, float *psi2,
+                         const float *rcmb, const float *rcma, mifdtd_stream_t stream);
+int mifdtd_cpml_face_x_h(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         const float *ey, const float *ez, float *hy, float *hz,
+                         float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream);
+int mifdtd_cpml_face_y_e(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         float *ex, float *ez, const float *hx, const float *hz,
+                         const float *cex, const float *cez, float *psi1, float *psi2,
+                         const float *rcmb, const float *rcma, mifdtd_stream_t stream);
+int mifdtd_cpml_face_y_h(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         const float *ex, const float *ez, float *hx, float *hz,
+                         float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream);
+int mifdtd_cpml_face_z_e(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         float *ex, float *ey, const float *hx, const float *hy,
+                         const float *cex, const float *cey, float *psi1, float *psi2,
+                         const float *rcmb, const float *rcma, mifdtd_stream_t stream);
+int mifdtd_cpml_face_z_h(int nx, int ny, int nz_pitch, int pad, int npml, int backward,
+                         const float *ex, const float *ey, float *hx, float *hy,
+                         float *psi1, float *psi2, const float *rcmb, const float *rcma,
+                         mifdtd_stream_t stream);
 
 /* All CPML faces of one half step, in the reference order (x f, x b, y f, y b, z f,
  * z b): one launch per axis.  psi1[2*axis+side], psi2[2*axis+side] may be NULL for

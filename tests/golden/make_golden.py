@@ -1,4 +1,5 @@
-"""Writes tests/golden/kemp_core.npz: golden vectors of the reference's own KEMP core.
+"""Writes tests/golden/kemp_core.npz (float) and kemp_core_f64.npz (double): golden
+vectors of the reference's own KEMP core.
 
 Run in the build container (needs /root/reference): it compiles
 K7/cpu/src/core.c in place (``make -C oracle ref``), drives it through ctypes on seeded
@@ -7,6 +8,9 @@ reference's test (``K7/cpu/test/test_core.py:69-89``: ufunc e/h, nz 61..64, coef
 '', 'e', 'h', tmax 1 and 10) on smaller grids, plus alternating E/H steps; one case
 at the reference's own grid (40, 50, 64, tmax 10, coeff 'e') is stored as a SHA-256
 digest of the output with its input seed.
+
+The double file follows the same grid with the reference's 'double' programs (2-lane
+SSE2, nz padded to 2: nz 13 and 14 give pads 1 and 0).
 
     python tests/golden/make_golden.py
 """
@@ -22,17 +26,17 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle import oracle as O  # noqa: E402
 
 
-def rand_fields(rng, nx, ny, nz, nzp, n):
+def rand_fields(rng, nx, ny, nz, nzp, n, dtype=np.float32):
     out = []
     for _ in range(n):
-        a = np.zeros((nx, ny, nzp), np.float32)
-        a[:, :, :nz] = rng.random((nx, ny, nz), dtype=np.float32)
+        a = np.zeros((nx, ny, nzp), dtype)
+        a[:, :, :nz] = rng.random((nx, ny, nz), dtype=dtype)
         out.append(a)
     return out
 
 
 def run_ref(pad, ehs, ces, chs, ufunc, tmax):
-    ref = O.ref_core(np.float32, pad)
+    ref = O.ref_core(ehs[0].dtype, pad)
     nx, ny, nzp = ehs[0].shape
     a = [x.copy() for x in ehs]
     for _ in range(tmax):
@@ -43,10 +47,10 @@ def run_ref(pad, ehs, ces, chs, ufunc, tmax):
     return a
 
 
-def coeffs(rng, coeff_use, nx, ny, nz, nzp):
-    half = lambda: np.full((nx, ny, nzp), 0.5, np.float32)
-    ces = rand_fields(rng, nx, ny, nz, nzp, 3) if "e" in coeff_use else [half() for _ in range(3)]
-    chs = rand_fields(rng, nx, ny, nz, nzp, 3) if "h" in coeff_use else [half() for _ in range(3)]
+def coeffs(rng, coeff_use, nx, ny, nz, nzp, dtype=np.float32):
+    half = lambda: np.full((nx, ny, nzp), 0.5, dtype)
+    ces = rand_fields(rng, nx, ny, nz, nzp, 3, dtype) if "e" in coeff_use else [half() for _ in range(3)]
+    chs = rand_fields(rng, nx, ny, nz, nzp, 3, dtype) if "h" in coeff_use else [half() for _ in range(3)]
     for c in ces + chs:
         c[:, :, nz:] = 0.5
     return ces, chs
@@ -99,6 +103,43 @@ def main():
     print("wrote", case, "cases; big digest", h)
 
 
+def main_f64():
+    """Golden vectors of the reference's double programs (pads 0 and 1)."""
+    O.build(ref=True)
+    rec = {}
+    nx, ny = 5, 6
+    case = 0
+    f8 = np.float64
+    for nz in (13, 14, 15, 16):
+        nzp = (nz + 1) // 2 * 2
+        pad = nzp - nz
+        for ufunc, coeff_use, tmax in (("e", "", 1), ("e", "e", 1), ("h", "", 1), ("h", "h", 1),
+                                       ("eh", "eh", 5)):
+            rng = np.random.default_rng(5000 + case)
+            zeros = lambda: np.zeros((nx, ny, nzp), f8)
+            if ufunc == "e":
+                ehs = [zeros() for _ in range(3)] + rand_fields(rng, nx, ny, nz, nzp, 3, f8)
+            elif ufunc == "h":
+                ehs = rand_fields(rng, nx, ny, nz, nzp, 3, f8) + [zeros() for _ in range(3)]
+            else:
+                ehs = rand_fields(rng, nx, ny, nz, nzp, 6, f8)
+            ces, chs = coeffs(rng, coeff_use, nx, ny, nz, nzp, f8)
+            out = run_ref(pad, ehs, ces, chs, ufunc, tmax)
+            p = f"c{case}_"
+            rec[p + "meta"] = np.array([nx, ny, nz, nzp, pad, tmax, {"e": 0, "h": 1, "eh": 2}[ufunc],
+                                        int("e" in coeff_use), int("h" in coeff_use)], np.int64)
+            rec[p + "in"] = np.stack(ehs)
+            if "e" in coeff_use:
+                rec[p + "ce"] = np.stack(ces)
+            if "h" in coeff_use:
+                rec[p + "ch"] = np.stack(chs)
+            rec[p + "out"] = np.stack(out)
+            case += 1
+    rec["ncases"] = np.array(case)
+    np.savez_compressed(os.path.join(HERE, "kemp_core_f64.npz"), **rec)
+    print("wrote", case, "double cases")
+
+
 def big_inputs(seed):
     """Inputs of the digest case (shared with the tests)."""
     nx, ny, nz = 40, 50, 64
@@ -109,4 +150,7 @@ def big_inputs(seed):
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--f64"]:
+        main_f64()
+    else:
+        main()

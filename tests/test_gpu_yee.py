@@ -58,27 +58,99 @@ def test_core_golden_status_abi(golden):
             assert same_bits(host(g), r), (meta, c)
 
 
-def test_core_golden_kemp_void_abi(golden):
-    """The reference-named void entry points (update_e/update_h, nz = nz_pitch, pad set
-    once per program with mifdtd_set_pad) on the same vectors."""
-    L = _lib()
-    lib = L.load()
-    try:
-        for meta, ehs, ces, chs, out in golden_cases(golden):
-            L.call("mifdtd_set_pad", meta["pad"])
+def _bind_like_kemp(lib, ce_on, ch_on):
+    """The binding of K7/cpu/core.py:86-105: 3 c_int and 6 arrays, plus 3 coefficient
+    arrays on update_e (ce_on) / update_h (ch_on); restype None.  Device pointers take
+    the place of the reference's host ndpointer."""
+    carg = ctypes.c_void_p
+    argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [carg for _ in range(6)]
+    lib.update_e.argtypes = list(argtypes)
+    lib.update_e.restype = None
+    lib.update_h.argtypes = list(argtypes)
+    lib.update_h.restype = None
+    if ce_on:
+        lib.update_e.argtypes += [carg for _ in range(3)]
+    if ch_on:
+        lib.update_h.argtypes += [carg for _ in range(3)]
+    return lib
+
+
+def _run_programs(golden, precision):
+    """Each golden case through every KEMP program of its pad that can express it: a
+    program takes CE (CH) arrays iff its coeff_use holds 'e' ('h'); a coefficient-free
+    program stands for a case whose coefficients are all 0.5."""
+    from fdtd_accelerate_amd import core
+    from fdtd_accelerate_amd.build import kemp_program_path
+    runs = 0
+    for meta, ehs, ces, chs, out in golden_cases(golden):
+        half = dev(np.full(ehs[0].shape, 0.5, ehs[0].dtype))
+        for coeff_use in ("", "e", "h", "eh"):
+            if ("e" not in coeff_use and ces is not None) or ("h" not in coeff_use and chs is not None):
+                continue
+            core.program(precision, coeff_use, meta["pad"])  # checks the program id
+            lib = _bind_like_kemp(ctypes.CDLL(kemp_program_path(precision, coeff_use, meta["pad"])),
+                                  "e" in coeff_use, "h" in coeff_use)
             d = [dev(x) for x in ehs]
-            half = dev(np.full(ehs[0].shape, 0.5, np.float32))
+            # keep the coefficient tensors alive while their pointers are in use
             dce = [dev(x) for x in ces] if ces is not None else [half] * 3
             dch = [dev(x) for x in chs] if chs is not None else [half] * 3
+            e_args = [meta["nx"], meta["ny"], meta["nzp"]] + [t.data_ptr() for t in d]
+            h_args = list(e_args)
+            if "e" in coeff_use:
+                e_args += [t.data_ptr() for t in dce]
+            if "h" in coeff_use:
+                h_args += [t.data_ptr() for t in dch]
             for _ in range(meta["tmax"]):
                 if meta["ufunc"] in ("e", "eh"):
-                    lib.update_e(meta["nx"], meta["ny"], meta["nzp"], *[t.data_ptr() for t in d + dce])
+                    lib.update_e(*e_args)
                 if meta["ufunc"] in ("h", "eh"):
-                    lib.update_h(meta["nx"], meta["ny"], meta["nzp"], *[t.data_ptr() for t in d + dch])
+                    lib.update_h(*h_args)
             for c, (g, r) in enumerate(zip(d, out)):
-                assert same_bits(host(g), r), (meta, c)
-    finally:
-        L.call("mifdtd_set_pad", 0)
+                assert same_bits(host(g), r), (precision, coeff_use, meta, c)
+            runs += 1
+    return runs
+
+
+def test_core_golden_kemp_programs_single(golden):
+    """The reference-named void entry points of every 'single' KEMP program (coeff_use
+    '', 'e', 'h', 'eh' x pad 0..3), bound with the reference's argtypes, on the golden
+    vectors of the reference's own compiled core."""
+    assert _run_programs(golden, "single") == 52
+
+
+def test_core_golden_kemp_programs_double():
+    """The same for the 'double' programs (pads 0 and 1) against the reference's double
+    core (tests/golden/kemp_core_f64.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "kemp_core_f64.npz"))
+    assert _run_programs(g, "double") == 52
+
+
+def test_core_double_fields_vs_oracle():
+    """Fields(precision_float='double') + Core (mifdtd_update_e/h_f64) against the float64
+    oracle over 20 steps, with CE arrays and a 64-byte GPU pitch (pad 5)."""
+    from fdtd_accelerate_amd import Core, Fields
+    nx, ny, nz = 9, 10, 35
+    f = Fields(nx, ny, nz, coeff_use="e", precision_float="double")
+    assert f.pad == 5 and f.nz_pitch == 40
+    rng = np.random.default_rng(77)
+    eh = [rng.standard_normal((nx, ny, nz)) for _ in range(6)]
+    ce = [rng.random((nx, ny, nz)) for _ in range(3)]
+    f.set_ehs(*eh)
+    f.set_ces(*ce)
+    Core(f)
+    pad = f.pad
+    pz = lambda a: np.pad(a, ((0, 0), (0, 0), (0, pad)))
+    ref = [pz(x) for x in eh]
+    rce = [pz(x) for x in ce]
+    for _ in range(20):
+        f.update_e()
+        f.update_h()
+        O.update_e(ref, rce, pad)
+        O.update_h(ref, None, pad)
+    for name, r in zip(("ex", "ey", "ez", "hx", "hy", "hz"), ref):
+        g = host(f.get(name))
+        assert g.dtype == np.float64 and same_bits(g, r), name
 
 
 def test_core_reference_digest(golden):
@@ -207,10 +279,10 @@ def test_cpml_step_vs_oracle(ns, npml, faces):
     assert np.abs(osim.psiE[(0, 0)][0]).max() > 0 if (0, 0) in osim.psiE else True
 
 
-def test_cpml_kufdtd_face_entry_points():
-    """update_{x,y,z}_{e,h} (kuFDTD argument order) one face at a time vs the oracle."""
+def test_cpml_face_entry_points():
+    """mifdtd_cpml_face_{x,y,z}_{e,h} (kuFDTD's argument order, KEMP layout) one face at a
+    time vs the oracle."""
     L = _lib()
-    lib = L.load()
     nx, ny, nz, npml = 18, 20, 24, 5
     rng = np.random.default_rng(11)
     ehs = [rng.standard_normal((nx, ny, nz)).astype(np.float32) for _ in range(6)]
@@ -221,7 +293,6 @@ def test_cpml_kufdtd_face_entry_points():
     dc = [dev(x) for x in ces]
     tb = {k: dev(v) for k, v in dict(bE=bE, aE=aE, bH=bH, aH=aH).items()}
     ref = [x.copy() for x in ehs]
-    L.call("mifdtd_set_pad", 0)
     ex, ey, ez, hx, hy, hz = [t.data_ptr() for t in d]
     cex, cey, cez = [t.data_ptr() for t in dc]
     for axis in range(3):
@@ -234,12 +305,10 @@ def test_cpml_kufdtd_face_entry_points():
             for _ in range(3):
                 e_args = {0: (ey, ez, hy, hz, cey, cez), 1: (ex, ez, hx, hz, cex, cez), 2: (ex, ey, hx, hy, cex, cey)}[axis]
                 h_args = {0: (ey, ez, hy, hz), 1: (ex, ez, hx, hz), 2: (ex, ey, hx, hy)}[axis]
-                fe = getattr(lib, "update_%s_e" % "xyz"[axis])
-                fh = getattr(lib, "update_%s_h" % "xyz"[axis])
-                fe(nx, ny, nz, npml, side, *e_args, gpe[0].data_ptr(), gpe[1].data_ptr(),
-                   tb["bE"].data_ptr(), tb["aE"].data_ptr())
-                fh(nx, ny, nz, npml, side, *h_args, gph[0].data_ptr(), gph[1].data_ptr(),
-                   tb["bH"].data_ptr(), tb["aH"].data_ptr())
+                L.call("mifdtd_cpml_face_%s_e" % "xyz"[axis], nx, ny, nz, 0, npml, side, *e_args,
+                       gpe[0].data_ptr(), gpe[1].data_ptr(), tb["bE"].data_ptr(), tb["aE"].data_ptr(), None)
+                L.call("mifdtd_cpml_face_%s_h" % "xyz"[axis], nx, ny, nz, 0, npml, side, *h_args,
+                       gph[0].data_ptr(), gph[1].data_ptr(), tb["bH"].data_ptr(), tb["aH"].data_ptr(), None)
                 O.cpml_e(axis, side, ref, ces, pe[0], pe[1], bE, aE, 0, npml)
                 O.cpml_h(axis, side, ref, None, ph[0], ph[1], bH, aH, 0, npml)
             for c, (g, r) in enumerate(zip(d, ref)):

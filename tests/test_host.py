@@ -100,8 +100,52 @@ def test_argument_errors_without_gpu():
     P.n = _lib.TFSF_MAX_PATCH + 1
     rc = lib.mifdtd_tfsf_faces(ctypes.byref(P), (ctypes.c_void_p * 6)(*[16] * 6), 16, 20, 8, 8, 8, None)
     assert rc == -1 and "patch count" in _lib.last_error()
+    rc = lib.mifdtd_update_e_f64(4, 4, 7, 0, *([16] * 6), None, None, None, None)
+    assert rc == -2 and "multiple of 2" in _lib.last_error()
+    rc = lib.mifdtd_update_h_f64(4, 4, 6, 5, *([16] * 6), None, None, None, None)
+    assert rc == -1 and "pad" in _lib.last_error()
+    rc = lib.mifdtd_cpml_face_x_e(20, 20, 20, 0, 4, 2, *([16] * 10), None)
+    assert rc == -1 and "backward" in _lib.last_error()
     with pytest.raises(RuntimeError):
-        _lib.call("mifdtd_set_pad", -1)
+        _lib.call("mifdtd_update_e", 4, 4, 6, 0, *([None] * 9), None)
+
+
+def test_kemp_programs_export_reference_symbols():
+    """Every KEMP core program library (precision x coeff_use x pad) loads without a GPU,
+    exports update_e/update_h with the reference's arity and names no other program."""
+    from fdtd_accelerate_amd import core
+    from fdtd_accelerate_amd.build import KEMP_COEFFS, KEMP_PRECISIONS
+    n = 0
+    for prec, pads in KEMP_PRECISIONS.items():
+        for coeff in KEMP_COEFFS:
+            for pad in pads:
+                lib = core.program(prec, coeff, pad)
+                assert len(lib.update_e.argtypes) == 3 + (9 if "e" in coeff else 6)
+                assert len(lib.update_h.argtypes) == 3 + (9 if "h" in coeff else 6)
+                n += 1
+    assert n == 24
+    with pytest.raises(ValueError):
+        core.program("double", "e", 2)
+    with pytest.raises(ValueError):
+        core.program("half", "e", 0)
+    # the main library exports no reference-named core symbol (the programs do)
+    from fdtd_accelerate_amd import _lib
+    import ctypes as C
+    main = C.CDLL(_lib.LIB_PATH)
+    for name in ("update_e", "update_h", "mifdtd_set_pad", "update_x_e"):
+        assert not hasattr(main, name), name
+
+
+def test_kemp_program_rejects_on_cpu_through_last_error():
+    """A rejected call through a program records its reason (the void ABI's only error
+    channel): nz_pitch 6 is not a multiple of 4 for a single program."""
+    from fdtd_accelerate_amd import _lib, core
+    prog = core.program("single", "", 2)
+    prog.update_e(4, 4, 6, *([None] * 6))
+    assert "multiple of 4" in _lib.last_error()
+    prog = core.program("double", "eh", 1)
+    prog.update_h(4, 4, 7, *([None] * 9))
+    assert "multiple of 2" in _lib.last_error()
 
 
 def test_fields_requires_gpu():
@@ -111,8 +155,10 @@ def test_fields_requires_gpu():
         pytest.skip("GPU present")
     with pytest.raises(RuntimeError):
         Fields(8, 8, 8)
-    with pytest.raises(ValueError):
+    with pytest.raises(RuntimeError):
         Fields(8, 8, 8, precision_float="double")
+    with pytest.raises(ValueError):
+        Fields(8, 8, 8, precision_float="quadruple")
     with pytest.raises(TypeError):
         Fields(8.0, 8, 8)
 
@@ -146,3 +192,46 @@ def test_paint_ce_at_yee_points():
     assert ces[2][13, 12, 10] == np.float32(0.5)         # Ez at (13.5, 12.5, 10): distance ~4.06
     n_in = int((ces[0] == np.float32(0.125)).sum())
     assert abs(n_in - 4 / 3 * np.pi * 3.2 ** 3) < 0.25 * 4 / 3 * np.pi * 3.2 ** 3
+
+
+def test_adjoint_design_box_off_the_cpml():
+    """The adjoint gradient has no term for the CPML correction of E, so a design box
+    touching a CPML slab is rejected by the C entry point and by DesignProblem."""
+    from fdtd_accelerate_amd import _lib
+    lib = _lib.load()
+    lam = (ctypes.c_void_p * 6)(*([16] * 6))
+    ce = (ctypes.c_void_p * 3)(None, None, None)
+    for box, ok in (((10, 10, 10, 19, 19, 19), True), ((9, 10, 10, 19, 19, 19), False),
+                    ((10, 10, 10, 19, 19, 20), False), ((10, 19, 10, 19, 19, 19), False)):
+        b = _lib.DesignBox(*box)
+        # 40^3, npml 10: E slab planes are [0, 10) and [29, 39) on each axis
+        rc = lib.mifdtd_adjoint_step(40, 40, 40, 0, lam, ce, 10, 16, 16, 16, 16, 16, 16, None, 0,
+                                     ctypes.byref(b), None if ok else 16, 16, None)
+        if ok:
+            assert rc == -3 and "curl" in _lib.last_error()  # passes the box check, then needs curl
+        else:
+            assert rc == -1 and "overlaps a CPML slab" in _lib.last_error(), box
+    from fdtd_accelerate_amd.adjoint import DesignProblem
+    with pytest.raises((ValueError, RuntimeError)) as e:
+        DesignProblem(n=(40, 40, 40), npml=10, box=(8, 12, 12, 10, 10, 10))
+    assert e.type is ValueError or "GPU" in str(e.value)
+
+
+def test_kufdtd_shaped_psi_is_rejected():
+    """A face psi allocated with kuFDTD's shapes (KU/cpu/cpml.py:35-85) is refused before
+    any device call, since the face kernels index this engine's shapes."""
+    import torch
+    from fdtd_accelerate_amd.cpml import check_face_psi, psi_shape
+
+    class F:  # a Fields stand-in: only the pitch is read
+        ns_pitch = (30, 32, 36)
+    nx, ny, nz, npml = 30, 32, 36, 6
+    kufdtd = [(npml, ny - 1, nz), (nx - 1, npml, nz), (nx - 1, ny - 1, npml + 1)]
+    for axis in range(3):
+        bad = torch.zeros(kufdtd[axis])
+        with pytest.raises(ValueError, match="must be a contiguous float32 tensor"):
+            check_face_psi(F, axis, npml, bad, bad)
+        good = torch.zeros(psi_shape(axis, nx, ny, nz, npml))
+        check_face_psi(F, axis, npml, good, good)
+        with pytest.raises(ValueError):
+            check_face_psi(F, axis, npml, good, good.double())

@@ -39,6 +39,20 @@ def test_oracle_matches_reference_golden(golden):
     assert n == int(golden["ncases"]) == 20
 
 
+def test_oracle_matches_reference_golden_double():
+    """The float64 oracle against the reference's 'double' programs (pads 0 and 1)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "kemp_core_f64.npz"))
+    n = 0
+    for meta, ehs, ces, chs, out in golden_cases(g):
+        assert ehs[0].dtype == np.float64
+        got = _oracle_run(meta, ehs, ces, chs)
+        for c, (x, r) in enumerate(zip(got, out)):
+            assert np.array_equal(x, r) and x.tobytes() == r.tobytes(), (meta, c)
+        n += 1
+    assert n == int(g["ncases"]) == 20
+
+
 def test_oracle_matches_reference_digest(golden):
     from tests.golden.make_golden import big_inputs
     ehs, ces = big_inputs(golden["big_seed"])
