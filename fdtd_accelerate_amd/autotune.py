@@ -32,13 +32,11 @@ def _load():
     return _table
 
 
-def lds_fits(T, CX, table, npml=0):
+def lds_fits(T, CX, table):
     """Whether the sweep's LDS (E and H row exchange 2 x 2 x T x 2 x 256 floats, the k-1
-    carry (CX+1)(T+1)2 floats, the 3 x 256 CE table, the four z-face CPML tables
-    8 (npml+1) floats, ~0.4 KiB of source/probe tables) fits the 160 KiB of one gfx950
-    work-group."""
-    zt = 8 * (npml + 1) if npml else 0
-    return 4 * ((768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2 + zt) + 512 <= 160 * 1024
+    carry (CX+1)(T+1)2 floats, the 3 x 256 CE table, ~0.4 KiB of source/probe tables)
+    fits the 160 KiB of one gfx950 work-group."""
+    return 4 * ((768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2) + 512 <= 160 * 1024
 
 
 def heuristic(nx, ny, nzp, npml):
@@ -69,7 +67,7 @@ def tune_fused(sim, candidates=((7, 64), (9, 64), (9, 96), (11, 64), (11, 128), 
     best = None
     results = {}
     for T, CX in candidates:
-        if not lds_fits(T, CX, f.ce_mat is not None, sim.cpml.npml if sim.cpml is not None else 0):
+        if not lds_fits(T, CX, f.ce_mat is not None):
             continue
         st = FusedStepper(sim, rows_per_thread=T, planes_per_chunk=CX)
         st.run(1, None)

@@ -69,7 +69,6 @@ struct FGeom {
     int kb, ke;  // z range of the interior box (multiples of 32)
     int zb0;     // first z segment start: kb rounded down to 256 (segments sit on 1 KiB)
     int nseg;    // z segments of 256 floats from zb0 covering [kb, ke)
-    int cseg;    // segments per row of the uniform-CE map (0: no map)
     int nblocks;
 };
 
@@ -80,37 +79,17 @@ struct Ctx {  // buffers of one launch (restrict-qualified at the kernel boundar
     const unsigned char *mx, *my, *mz;          // CE material indices (kCeTable)
 };
 
-// The z faces of the CPML, applied inside the sweep (with the CPML on, the sweep box spans
-// all of z; only lanes whose quad holds a z-slab cell run this code): psi of the z faces
-// in the packed layout (faces 4 and 5, one npml-float run per row) and the four b/a tables,
-// staged in LDS per block.  psi E is read from step n and written to step n+1 for owned
-// cells only (the recomputed halo row and seed plane belong to the shell pass); psi H is
-// updated in place by the owning lane.
-struct FZ {
-    int npml;
-    const float *pe;    // psi E of step n (packed)
-    float *pe_out;      // psi E of step n+1
-    float *ph;          // psi H (packed, in place)
-    size_t f1, b1, sZ;  // float offsets of psi1 of the z front / back face; psi2 = psi1 + sZ
-    const float *bE, *aE, *bH, *aH;
-};
-
 // How CE reaches the E update: the constant 0.5, three float arrays, or three uint8
 // material-index arrays into a 3 x 256 float table held in LDS (same float values, so
 // the products are identical; 3 B per cell instead of 12).
 enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2 };
 
 // CE of the quad at element rb + lc (rb uniform, lc the lane's element offset)
-// With CE as material indices, useg >= 0 says that the whole 256-cell segment of this row
-// holds one index per component (bits 0-7 x, 8-15 y, 16-23 z; shift selects it): the
-// value comes from the table and the index bytes are not read.
 template <int CEM>
-__device__ __forceinline__ Q4 load_ce(const float *cf, const unsigned char *mt, const float *tab, size_t rb, unsigned lc,
-                                      int useg = -1, int shift = 0)
+__device__ __forceinline__ Q4 load_ce(const float *cf, const unsigned char *mt, const float *tab, size_t rb, unsigned lc)
 {
     if (CEM == kCeConst) return splat4(0.5f);
     if (CEM == kCeArrays) return ld4(cf + rb + lc);
-    if (useg >= 0) return splat4(tab[(useg >> shift) & 255]);
     const uchar4 t = *reinterpret_cast<const uchar4 *>(mt + rb + lc);
     Q4 q;
     q.v[0] = tab[t.x];
@@ -156,14 +135,11 @@ __device__ __forceinline__ size_t row_base(const FGeom &g, int i, int j, int m)
 // (all of step n, zero outside the grid), hx_nx/hy_nx = hx/hy of the element after the
 // segment for its last lane.  All memory loads of the row are issued first.  src: the
 // block's source table (LDS) or null.
-// ZP: z faces of the CPML on lanes whose quad holds a z-slab cell (zq), psi E stored when
-// zst (owned cell); zt: the b/a tables in LDS (bE, aE, bH, aH, 2*(npml+1) floats each).
-template <int CEM, bool ZP>
+template <int CEM>
 __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const float *tab, const FSrc *src, int i, int j,
                                           int m, unsigned l4, unsigned lc, bool end_lane, const Q4 hc[3],
                                           const Q4 &hux, const Q4 &huz, const Q4 hN[3], float hx_nx, float hy_nx,
-                                          Q4 &exn, Q4 &eyn, Q4 &ezn, const FZ &z, const float *zt, bool zq, bool zst,
-                                          int useg)
+                                          Q4 &exn, Q4 &eyn, Q4 &ezn)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int k0 = g.zb0 + m * 256 + (int)l4;
@@ -171,9 +147,9 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
     const Q4 e0 = ld4(c.ex + rb + lc);
     const Q4 e1 = ld4(c.ey + rb + lc);
     const Q4 e2 = ld4(c.ez + rb + lc);
-    const Q4 c0 = load_ce<CEM>(c.cex, c.mx, tab, rb, lc, useg, 0);
-    const Q4 c1 = load_ce<CEM>(c.cey, c.my, tab + 256, rb, lc, useg, 8);
-    const Q4 c2 = load_ce<CEM>(c.cez, c.mz, tab + 512, rb, lc, useg, 16);
+    const Q4 c0 = load_ce<CEM>(c.cex, c.mx, tab, rb, lc);
+    const Q4 c1 = load_ce<CEM>(c.cey, c.my, tab + 256, rb, lc);
+    const Q4 c2 = load_ce<CEM>(c.cez, c.mz, tab + 512, rb, lc);
     // k+1 neighbours of H(i,j)
     float hy1 = __shfl_down(hc[1].v[0], 1);
     float hx1 = __shfl_down(hc[0].v[0], 1);
@@ -213,33 +189,6 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
         }
     }
     ezn = e;
-    // z faces (after the core update, before the sources: the reference's order).  In the
-    // box every z-slab cell is a valid ex / ey cell (j < ny-1, i < nx-1, k < nz-1).
-    if (ZP && zq) {
-        const int np_ = z.npml, T2 = 2 * (np_ + 1);
-        const size_t zr = ((size_t)i * g.ny + j) * np_;
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int k = k0 + l;
-            const bool back = k >= nz - 1 - np_ && k < nz - 1;
-            if (k < np_ || back) {
-                const int p = back ? k - (nz - 1 - np_) : k;
-                const int t = back ? np_ + 1 + p : p + 1;
-                const float b = zt[t], a = zt[T2 + t];
-                const size_t o = (back ? z.b1 : z.f1) + zr + p;
-                const float y1 = l < 3 ? hc[1].v[l + 1] : hy1;
-                const float x1 = l < 3 ? hc[0].v[l + 1] : hx1;
-                const float t1 = b * z.pe[o] + a * (y1 - hc[1].v[l]);   // psi1 -> ex from hy
-                const float t2 = b * z.pe[o + z.sZ] + a * (x1 - hc[0].v[l]);  // psi2 -> ey from hx
-                exn.v[l] = exn.v[l] - c0.v[l] * t1;
-                eyn.v[l] = eyn.v[l] + c1.v[l] * t2;
-                if (zst) {
-                    z.pe_out[o] = t1;
-                    z.pe_out[o + z.sZ] = t2;
-                }
-            }
-        }
-    }
     if (src) {
         apply_src(*src, 0, i, j, k0, exn);
         apply_src(*src, 1, i, j, k0, eyn);
@@ -250,11 +199,10 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
 // H_new of row j of plane i for this lane's quad: hc = H(i,j) of step n, exn/eyn/ezn =
 // E_new(i,j), exb/ezb = E_new(i,j-1) (ex, ez), eyp/ezp = E_new(i-1,j) (ey, ez),
 // exm/eym = E_new(i,j) at k0-1 (ex, ey).
-template <bool ZP>
 __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i, int j, int m, unsigned l4,
                                           const Q4 hc[3], const Q4 &exn, const Q4 &eyn, const Q4 &ezn, const Q4 &exb,
                                           const Q4 &ezb, const Q4 &eyp, const Q4 &ezp, float exm, float eym, Q4 &hxo,
-                                          Q4 &hyo, Q4 &hzo, const FZ &z, const float *zt, bool zq, bool zst)
+                                          Q4 &hyo, Q4 &hzo)
 {
     const int nz = g.nz;
     const int k0 = g.zb0 + m * 256 + (int)l4;
@@ -287,32 +235,6 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
         for (int l = 0; l < 4; ++l) {
             const float v = hzo.v[l] - 0.5f * ((eyn.v[l] - eyp.v[l]) - (exn.v[l] - exb.v[l]));
             hzo.v[l] = (k0 + l < nz) ? v : v * 0.f;
-        }
-    }
-    // z faces: valid hx / hy cells of the slab are 0 < k < nz (i > 0, j > 0 in the box)
-    if (ZP && zq) {
-        const int np_ = z.npml, T2 = 2 * (np_ + 1);
-        const size_t zr = ((size_t)i * g.ny + j) * np_;
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int k = k0 + l;
-            const bool back = k >= nz - np_ && k < nz;
-            if ((k > 0 && k < np_) || back) {
-                const int p = back ? k - (nz - np_) : k;
-                const int t = back ? np_ + 2 + p : p + 1;
-                const float b = zt[2 * T2 + t], a = zt[3 * T2 + t];
-                const size_t o = (back ? z.b1 : z.f1) + zr + p;
-                const float ymv = l > 0 ? eyn.v[l - 1] : eym;
-                const float xmv = l > 0 ? exn.v[l - 1] : exm;
-                const float t1 = b * z.ph[o] + a * (eyn.v[l] - ymv);   // psi1 -> hx from ey
-                const float t2 = b * z.ph[o + z.sZ] + a * (exn.v[l] - xmv);  // psi2 -> hy from ex
-                hxo.v[l] = hxo.v[l] + 0.5f * t1;
-                hyo.v[l] = hyo.v[l] - 0.5f * t2;
-                if (zst) {
-                    z.ph[o] = t1;
-                    z.ph[o + z.sZ] = t2;
-                }
-            }
         }
     }
     if (src) {
@@ -348,17 +270,16 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
 // plane's loads issue together.  The LDS exchange is ordered by the barrier and the
 // parity.  Source and probe tables are copied to LDS once per block and consulted only
 // by blocks whose rows they touch.
-template <int CEM, bool ZP, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_fused(
-    FGeom g, FZ zf,
+template <int CEM>
+__global__ __launch_bounds__(1024, 4) void k_fused(
+    FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     float *__restrict__ oex, float *__restrict__ oey, float *__restrict__ oez,
     float *__restrict__ ohx, float *__restrict__ ohy, float *__restrict__ ohz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my,
-    const unsigned char *__restrict__ mz, const float *__restrict__ lut, const int *__restrict__ ceseg, FSrc src,
-    FProbe prb)
+    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb)
 {
     const Ctx c{ex, ey, ez, hx, hy, hz, oex, oey, oez, ohx, ohy, ohz, cex, cey, cez, mx, my, mz};
     extern __shared__ float lds[];
@@ -369,7 +290,6 @@ __global__ __launch_bounds__(MAXT) void k_fused(
     float *ebuf = lds + (CEM == kCeTable ? 768 : 0);  // [2][T][2][256]: ex, ez of rows w = 0..T-1
     float *hbuf = ebuf + 2 * T * 512;                 // [2][T][2][256]: hx, hz of H(i+1) of rows w = 1..T
     float *cbuf = hbuf + 2 * T * 512;                 // [cx+1][T+1][2]: ex, ey at the segment's last cell
-    float *zt = cbuf + (size_t)(g.cx + 1) * (T + 1) * 2;  // ZP: [4][2*(npml+1)] bE, aE, bH, aH
 
     // XCD-aware block order: consecutive logical blocks (neighbouring row blocks of a
     // chunk, which share halo rows) land on the same XCD.
@@ -397,13 +317,6 @@ __global__ __launch_bounds__(MAXT) void k_fused(
     }
     if (CEM == kCeTable)
         for (int t = threadIdx.x; t < 768; t += blockDim.x) tab[t] = lut[t];
-    if (ZP) {
-        const int T2 = 2 * (zf.npml + 1);
-        for (int t = threadIdx.x; t < 4 * T2; t += blockDim.x) {
-            const int q = t / T2, r = t - q * T2;
-            zt[t] = (q == 0 ? zf.bE : q == 1 ? zf.aE : q == 2 ? zf.bH : zf.aH)[r];
-        }
-    }
     __syncthreads();
     const FSrc *sp = src_hit ? &s_src : nullptr;
 
@@ -430,9 +343,6 @@ __global__ __launch_bounds__(MAXT) void k_fused(
         // first cell of a box starting inside the row: E_new(kb-1) from the output
         // (stored by the shell E pass before this launch)
         const bool carry_mem = m == 0 && g.kb > 0 && own && start_lane;
-        // this lane's quad holds a z-slab cell of the CPML (E or H)
-        const int kq = base + (int)l4;
-        const bool zq = ZP && (kq < zf.npml || (kq + 3 >= g.nz - 1 - zf.npml && kq < g.nz));
         Q4 eyp = zero4(), ezp = zero4();  // E_new(i-1, j)
         Q4 hc[3], hux = zero4(), huz = zero4();
         float hx_nx = 0.f, hy_nx = 0.f;  // hx, hy of the element after the segment
@@ -452,14 +362,10 @@ __global__ __launch_bounds__(MAXT) void k_fused(
                 huz = ld4(hz + ru + lc);
             }
         }
-        // uniform-CE words of this row: plane i's is loaded one plane ahead (scalar loads)
-        const bool useg_on = CEM == kCeTable && g.cseg > 0 && live;
-        int useg = useg_on ? ceseg[((size_t)ib * g.ny + j) * g.cseg + m] : -1;
         for (int i = ib; i < i1; ++i) {
             const bool seed = i < i0;
             const int par = it & 1;
             ++it;
-            const int useg_next = useg_on && i + 1 < i1 ? ceseg[((size_t)(i + 1) * g.ny + j) * g.cseg + m] : -1;
             Q4 hN[3];
             hN[0] = hN[1] = hN[2] = zero4();
             float hxN_nx = 0.f, hyN_nx = 0.f;
@@ -494,8 +400,7 @@ __global__ __launch_bounds__(MAXT) void k_fused(
             }
             Q4 exn = zero4(), eyn = zero4(), ezn = zero4();
             if (live)
-                compute_E<CEM, ZP>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn,
-                                   ezn, zf, zt, zq, own && !seed && act, useg);
+                compute_E<CEM>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn, ezn);
             if (w < T) {
                 float *es = ebuf + (size_t)((par * T + w) * 2) * 256;
                 *reinterpret_cast<float4 *>(es + l4) = make_float4(exn.v[0], exn.v[1], exn.v[2], exn.v[3]);
@@ -527,8 +432,7 @@ __global__ __launch_bounds__(MAXT) void k_fused(
                 const Q4 exb = Q4{{xb[l4], xb[l4 + 1], xb[l4 + 2], xb[l4 + 3]}};
                 const Q4 ezb = Q4{{xb[256 + l4], xb[256 + l4 + 1], xb[256 + l4 + 2], xb[256 + l4 + 3]}};
                 Q4 hxo, hyo, hzo;
-                compute_H<ZP>(g, sp, i, j, m, l4, hc, exn, eyn, ezn, exb, ezb, eyp, ezp, exm, eym, hxo, hyo, hzo, zf, zt,
-                              zq, act);
+                compute_H(g, sp, i, j, m, l4, hc, exn, eyn, ezn, exb, ezb, eyp, ezp, exm, eym, hxo, hyo, hzo);
                 const size_t rbase = row_base(g, i, j, m);
                 if (act) {
                     st4(oex + rbase + l4, exn);
@@ -555,7 +459,6 @@ __global__ __launch_bounds__(MAXT) void k_fused(
                     }
                 }
             }
-            useg = useg_next;
             eyp = eyn;
             ezp = ezn;
             hc[0] = hN[0];
@@ -707,8 +610,8 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         const int np_ = d->npml;
         g.xb = np_ + 1; g.xe = nx - 1 - np_;
         g.yb = np_ + 1; g.ye = ny - 1 - np_;
-        g.kb = 0; g.ke = nzp;  // z faces run inside the sweep
-        if (g.xb >= g.xe || g.yb >= g.ye) {  // no interior: the shell is the grid
+        g.kb = 32 * ((np_ + 1 + 31) / 32); g.ke = 32 * ((nz - 1 - np_) / 32);
+        if (g.xb >= g.xe || g.yb >= g.ye || g.kb >= g.ke) {  // no interior: the shell is the grid
             g.xb = g.xe = nx;
             g.yb = g.ye = ny;
             g.kb = g.ke = nzp;
@@ -723,14 +626,11 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     g.nchunks = sweep ? (g.xe - g.xb + CX - 1) / CX : 0;
     g.zb0 = 256 * (g.kb / 256);
     g.nseg = sweep ? (g.ke - g.zb0 + 255) / 256 : 0;
-    // the uniform-CE map has one word per 256-cell segment from z = 0 (the sweep's
-    // segments start at zb0 = 0 here)
-    g.cseg = (tbl && d->ce_seg && g.zb0 == 0) ? (nzp + 255) / 256 : 0;
     const long long nbl = (long long)g.nrb * g.nchunks;
     if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
     g.nblocks = (int)nbl;
     const size_t lds_bytes = sizeof(float) * ((cem == kCeTable ? 768 : 0) + (size_t)2 * 2 * T * 2 * 256 +
-                                              (size_t)(CX + 1) * (T + 1) * 2 + (pml ? 8 * (size_t)(d->npml + 1) : 0));
+                                              (size_t)(CX + 1) * (T + 1) * 2);
     if (lds_bytes + sizeof(FSrc) + sizeof(FProbe) > kMaxLds)
         return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS (> %zu)", T, CX, lds_bytes,
                     kMaxLds);
@@ -752,20 +652,13 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         add(g.xe, nx, 0, ny, 0, Q);
         add(g.xb, g.xe, 0, g.yb, 0, Q);
         add(g.xb, g.xe, g.ye, ny, 0, Q);
+        add(g.xb, g.xe, g.yb, g.ye, 0, g.kb / 4);
+        add(g.xb, g.xe, g.yb, g.ye, g.ke / 4, Q);
         if (total >= (1ull << 32) - 256) return fail(MIFDTD_ERANGE, "CPML shell too large");
         sb.start[sb.n] = (unsigned)total;
     }
     SGeom sg{};
     sg.nx = nx; sg.ny = ny; sg.nzp = nzp; sg.nz = nz;
-    FZ zf{};
-    if (pml) {
-        zf.npml = d->npml;
-        zf.ph = d->psiH;
-        zf.sZ = scp.sZ;
-        zf.f1 = 4 * scp.sX + 4 * scp.sY;  // face 4 (z front) psi1; face 5 psi1 two faces on
-        zf.b1 = zf.f1 + 2 * scp.sZ;
-        zf.bE = d->bE; zf.aE = d->aE; zf.bH = d->bH; zf.aH = d->aH;
-    }
 
     FSrc src{};
     SSrc ssrc{};
@@ -841,22 +734,13 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         }
         if (sweep) {
             if (sweep_ms) (void)hipEventRecord(marks[2 * (size_t)s], st);
-            zf.pe = pin;
-            zf.pe_out = pout;
-#define MIFDTD_FUSED_LAUNCH(CEV, ZPV)                                                                              \
-    (nthr <= 640 ? k_fused<CEV, ZPV, 640> : k_fused<CEV, ZPV, 1024>)<<<g.nblocks, nthr, lds_bytes, st>>>(g, zf, in[0], in[1], in[2], in[3], in[4], in[5], out[0],    \
-                                                          out[1], out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], \
-                                                          d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2],         \
-                                                          d->ce_lut, d->ce_seg, src, prb)
-            if (pml) {
-                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, true);
-                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, true);
-                else MIFDTD_FUSED_LAUNCH(kCeConst, true);
-            } else {
-                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, false);
-                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, false);
-                else MIFDTD_FUSED_LAUNCH(kCeConst, false);
-            }
+#define MIFDTD_FUSED_LAUNCH(CEV)                                                                                 \
+    k_fused<CEV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
+                                                     out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
+                                                     d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, src, prb)
+            if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable);
+            else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays);
+            else MIFDTD_FUSED_LAUNCH(kCeConst);
 #undef MIFDTD_FUSED_LAUNCH
             if (sweep_ms) (void)hipEventRecord(marks[2 * (size_t)s + 1], st);
             MIFDTD_CHECK_LAUNCH("fused sweep launch");

@@ -51,28 +51,6 @@ def staggered_arrays(count, shape, dtype, device, zero=True):
     return [arena[k * step:k * step + n].view(shape) for k in range(count)]
 
 
-SEGMENT = 256  # z cells per segment of the fused sweep (64 lanes x 4)
-
-
-def segment_map(mats):
-    """(nx, ny, ceil(nz_pitch/256)) int32: for each 256-cell z segment of each row, the
-    three component indices packed as x | y << 8 | z << 16 when every cell of the
-    segment holds the same index per component, else -1."""
-    nx, ny, nzp = mats[0].shape
-    nseg = -(-nzp // SEGMENT)
-    word = torch.zeros((nx, ny, nseg), dtype=torch.int32, device=mats[0].device)
-    uniform = torch.ones((nx, ny, nseg), dtype=torch.bool, device=mats[0].device)
-    for c, m in enumerate(mats):
-        if nseg * SEGMENT != nzp:  # a partial last segment: repeat its last cell
-            m = torch.cat([m, m[:, :, -1:].expand(nx, ny, nseg * SEGMENT - nzp)], dim=2)
-        v = m.view(nx, ny, nseg, SEGMENT)
-        lo = v.amin(dim=3)
-        uniform &= lo == v.amax(dim=3)
-        word |= lo.to(torch.int32) << (8 * c)
-        del v, lo, m
-    return torch.where(uniform, word, torch.full_like(word, -1)).contiguous()
-
-
 class Fields:
     def __init__(self, nx, ny, nz, coeff_use="", precision_float="single",
                  segment_nbytes=64, device=None, stream=None):
@@ -117,7 +95,6 @@ class Fields:
         self.chs = [self._half() for _ in range(3)] if self.ch_on else None
         self.ce_mat = None  # per-component uint8 indices into ce_lut (compress_ce)
         self.ce_lut = None
-        self.ce_seg = None  # per 256-cell z segment: the uniform index triple or -1 (compress_ce)
         self.instance_list = []
 
     def _half(self):
@@ -173,7 +150,7 @@ class Fields:
         if not self.ce_on:
             raise AttributeError("The Fields instance has no ce arrays. You should add 'e' in the option 'coeff_use'")
         self._fill(self.ces, [cex, cey, cez], "ce")
-        self.ce_mat = self.ce_lut = self.ce_seg = None
+        self.ce_mat = self.ce_lut = None
 
     def compress_ce(self):
         """Let the core E update read the CE arrays as uint8 material indices into a
@@ -189,12 +166,11 @@ class Fields:
             for c, ce in enumerate(self.ces):
                 vals, inv = torch.unique(ce, return_inverse=True)
                 if vals.numel() > 256:
-                    self.ce_mat = self.ce_lut = self.ce_seg = None
+                    self.ce_mat = self.ce_lut = None
                     return False
                 lut[c, :vals.numel()] = vals
                 mats.append(inv.to(torch.uint8).contiguous())
                 del inv
-            self.ce_seg = segment_map(mats)
         self.ce_mat, self.ce_lut = mats, lut
         return True
 

@@ -81,7 +81,7 @@ class FusedStepper:
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0)
         self.T = rows_per_thread or T
         self.CX = planes_per_chunk or CX
-        if not lds_fits(self.T, self.CX, f.ce_mat is not None, cp.npml if cp else 0):
+        if not lds_fits(self.T, self.CX, f.ce_mat is not None):
             raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} need more than 64 KiB of LDS")
         if cp is not None:
             cp.alt_psiE()
@@ -108,7 +108,6 @@ class FusedStepper:
             for c in range(3):
                 d.ce_mat[c] = f.ce_mat[c].data_ptr()
             d.ce_lut = f.ce_lut.data_ptr()
-            d.ce_seg = f.ce_seg.data_ptr() if f.ce_seg is not None else None
         cp = sim.cpml
         if cp is not None:
             d.npml = cp.npml
@@ -165,17 +164,16 @@ class FusedStepper:
         return sweep_ms.value
 
     def sweep_box(self):
-        """(xb, xe, yb, ye, kb, ke): the box the single-pass sweep updates, as the C side
-        sets it: the planes and rows with no x / y CPML term, over all of z (the z faces
-        run inside the sweep); the whole grid without CPML."""
+        """(xb, xe, yb, ye, kb, ke): the interior box the single-pass sweep updates (the
+        cells with no CPML term; the whole grid without CPML), as the C side sets it."""
         f = self.sim.fields
-        nx, ny = f.nx, f.ny
+        nx, ny, nz = f.nx, f.ny, f.nz
         cp = self.sim.cpml
         if cp is None:
             return 0, nx, 0, ny, 0, f.nz_pitch
         p = cp.npml
-        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 0, f.nz_pitch)
-        if box[0] >= box[1] or box[2] >= box[3]:
+        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32))
+        if box[0] >= box[1] or box[2] >= box[3] or box[4] >= box[5]:
             return nx, nx, ny, ny, f.nz_pitch, f.nz_pitch
         return box
 

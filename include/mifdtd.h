@@ -269,11 +269,6 @@ typedef struct {
      * when ce_mat[0] is set, ce[] is ignored.  Index arrays must be 4-byte aligned. */
     const unsigned char *ce_mat[3];
     const float *ce_lut;
-    /* fused only, with ce_mat (or NULL): one int per 256-cell z segment of each row,
-     * (nx, ny, ceil(nz_pitch/256)) C order: >= 0 when the segment holds one material index
-     * per component (bits 0-7 x, 8-15 y, 16-23 z), -1 otherwise.  The sweep then takes CE
-     * from the table without reading the segment's index bytes (same values). */
-    const int *ce_seg;
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values

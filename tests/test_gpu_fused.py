@@ -299,46 +299,3 @@ def test_slab_engine_switch():
     for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
         assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
     assert same_values(a.series(), b.series())
-
-
-@pytest.mark.parametrize("ns,npml,T,CX,steps", [
-    ((26, 22, 600), 5, 9, 8, 4),
-    ((30, 33, 520), 12, 4, 64, 3),
-    ((30, 28, 257), 0, 9, 16, 3),
-])
-def test_fused_uniform_segments_match_inplace(ns, npml, T, CX, steps):
-    """A blocky medium (a few boxes in vacuum) with CE as material indices: most 256-cell
-    z segments hold one index per component and take CE from the table without reading
-    the index bytes, the others read them; with CPML the z faces run inside the sweep.
-    Equal to the in-place engine in every field, psi and probe sample."""
-    from fdtd_accelerate_amd.sources import gaussian_pulse
-    sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", T, CX)]
-    nx, ny, nz = ns
-    eps = np.ones((nx, ny, nz), np.float32)
-    eps[3:9, 4:12, 100:140] = 4.0
-    eps[10:nx - 2, 2:7, 300:nz - 1] = 9.0
-    eps[nx // 2, ny // 2, 5:260] = 2.25
-    ce = (0.5 / eps).astype(np.float32)
-    tf = gaussian_pulse(5.0, 3.0, 8.0)
-    for s in sims:
-        _fill(s, 23)
-        s.fields.set_ces(ce, np.roll(ce, 1, axis=0), np.roll(ce, 1, axis=1))
-        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
-        s.add_source("ex", (2, 3, 1), (nx - 3, 3, 8), tf, 0.5)
-        s.add_probe("ey", (nx // 2, ny // 2 + 1, nz // 2))
-        s.add_probe("hx", (nx // 2, ny // 2, 3))
-        s.add_probe("hy", (nx // 2, ny // 2, nz - 2))
-    assert sims[1].fields.compress_ce()
-    seg = sims[1].fields.ce_seg.cpu().numpy()
-    assert (seg >= 0).any() and (seg < 0).any(), "the medium must mix uniform and non-uniform segments"
-    series = [s.run(steps) for s in sims]
-    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
-        a = sims[0].field(name).cpu().numpy()
-        b = sims[1].field(name).cpu().numpy()
-        assert same_values(a, b), (name, np.abs(a - b).max())
-    assert same_values(series[0], series[1])
-    if npml:
-        for key in sims[0].cpml.psiE:
-            for c in range(2):
-                assert same_values(sims[0].cpml.psiE[key][c].cpu().numpy(), sims[1].cpml.psiE[key][c].cpu().numpy()), ("psiE", key, c)
-                assert same_values(sims[0].cpml.psiH[key][c].cpu().numpy(), sims[1].cpml.psiH[key][c].cpu().numpy()), ("psiH", key, c)
