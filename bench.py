@@ -32,7 +32,11 @@ source + H update + CPML-H (+ probe gather).
 Prints one JSON line: Gcell-updates/s over K timed steps (inputs resident in HBM),
 the dominant kernel's roofline (algorithmic bytes / its average launch time, from
 HIP events around each launch on the kernel's stream) and a CPU baseline (the
-oracle, timed on a bounded sample on this host).
+build's CPU oracle, timed on a bounded sample on this host).  The default run also
+carries the metric's other two legs as keys of the same line: "batched" (the
+64-instance uneven batch against the same instances run one after another) and
+"adjoint" (the 160^3 design problem: backward / forward time and the gradient's error
+against the float64 references); --plain-only skips them.
 """
 import argparse
 import json
@@ -57,6 +61,9 @@ MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
 # The 4th-order E stage (k_s4_e, CE arrays), same method, keyed by (n, npml);
 # profiles/k_s4_symplectic_cpml10_512cube_hbm_counters.txt.
 MEASURED_S4_TRAFFIC_GB = {(512, 10): 8.17}
+# The batched step (both launches of one step, all 64 instances), same method, keyed by
+# the batch's size key; filled from profiles/ when measured.
+MEASURED_BATCH_TRAFFIC_GB = {}
 
 
 def parse():
@@ -78,6 +85,8 @@ def parse():
     p.add_argument("--no-ce-lut", action="store_true",
                    help="kemp and fused engines: read CE as float arrays instead of uint8 indices + table")
     p.add_argument("--cpu-sample-n", type=int, default=256)
+    p.add_argument("--plain-only", action="store_true",
+                   help="default workload: time the 1024^3 step only (skip the batched and adjoint legs)")
     return p.parse_args()
 
 
@@ -165,8 +174,11 @@ def time_fused(sim, steps, warmup):
     kern_ms = st.run(steps, None, time_kernel=True)
     cells = sim.fields.nx * sim.fields.ny * sim.fields.nz
     sweep = st.sweep_cells()
-    name = f"k_fused (interior sweep {sweep} cells, {st.bytes_per_cell:g} B/cell)"
-    return ms / steps, wall, kern_ms, name, st.bytes_per_cell * sweep / cells
+    # algorithmic bytes on the per-cell basis of the plain Yee step: read 6 fields and 3 CE,
+    # write 6 fields = 60 B (the sweep box holds no CPML cell); the kernel itself reads CE
+    # as 3 B of material indices, so it moves 51 B of those 60
+    name = f"k_fused (interior sweep {sweep} cells, 60 B/cell algorithmic, {st.bytes_per_cell:g} B/cell read/written)"
+    return ms / steps, wall, kern_ms, name, 60.0 * sweep / cells
 
 
 def time_slab(sim, steps, warmup, slab_planes=None):
@@ -285,6 +297,9 @@ def run_batch(args):
     seq_kemp_ms = max(a.elapsed_time(b), (time.perf_counter() - t0) * 1e3) / ksteps
     seq_best = min(seq_slab_ms, seq_kemp_ms)
     sizes = sorted(s.fields.nx for s in sims)
+    step_bytes = sum(60.0 * s.fields.nx * s.fields.ny * s.fields.nz +
+                     32.0 * (2 * s.cpml.npml * (s.fields.ny * s.fields.nz + s.fields.nx * s.fields.nz + s.fields.nx * s.fields.ny)
+                             if s.cpml is not None else 0) for s in sims)
     return {
         "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
@@ -297,9 +312,11 @@ def run_batch(args):
                        "inplace_engine_ms_per_step": round(seq_kemp_ms, 4),
                        "inplace_steps_timed": ksteps},
         "tuning": None if tuned is None else {"best": list(tuned[0]), "ms": {f"{k[0]}/{k[1]}": round(v, 4) for k, v in tuned[1].items()}},
-        "roofline": {"bound": "hbm", "kernel": "k_batch (E + H passes)", "achieved": round((84.0 * cells) / (ms * 1e-3) / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round((84.0 * cells) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_cell": 84.0, "avg_launch_ms": round(ms / 2, 4)},
+        "roofline": {"bound": "hbm", "kernel": bt.kernel_name, "basis": "60 B per cell + 32 B per CPML face-slab cell per step",
+                     "achieved": round(step_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_BATCH_TRAFFIC_GB.get(bt.key()), "traffic_unit": "GB per step",
+                     "algorithmic_gb": round(step_bytes / 1e9, 3), "ms_per_step": round(ms, 4)},
         "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
     }
 
@@ -414,14 +431,21 @@ def run_drude(args):
     }
 
 
+ADJ_GOLD = os.path.join(ROOT, "tests", "golden", "adjoint_design160.npz")
+
+
 def run_adjoint(args):
     """The 160^3 design problem: J(eps) forward (800 steps, checkpoints every 50) and dJ/deps on the
-    64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both."""
+    64^3 design box by the adjoint (recompute + adjoint kernels); GPU times of both.  With the
+    committed float64 reference data, the gradient's error against the float64 central
+    differences at its 32 voxels and against the float64 gradient over the box."""
     import numpy as np
     import torch
     from fdtd_accelerate_amd.adjoint import DesignProblem
     p = DesignProblem()
-    eps = np.random.default_rng(4).uniform(1.0, 4.0, p.box[3:]).astype(np.float32)
+    gold = np.load(ADJ_GOLD) if os.path.exists(ADJ_GOLD) else None
+    seed = int(gold["eps_seed"]) if gold is not None else 4
+    eps = np.random.default_rng(seed).uniform(1.0, 4.0, p.box[3:]).astype(np.float32)
     p.forward(eps)  # warm-up (allocations, first launches)
     p.backward()
     torch.cuda.synchronize()
@@ -438,6 +462,14 @@ def run_adjoint(args):
     wall = time.perf_counter() - wall0
     fwd, bwd = a.elapsed_time(b), b.elapsed_time(c)
     n = p.n[0] * p.n[1] * p.n[2]
+    check = None
+    if gold is not None:
+        gh = g.cpu().numpy().astype(np.float64)
+        fd = gold["fd"]
+        ref = gold["grad_ref"].astype(np.float64)
+        check = {"fd_voxels": int(len(fd)), "fd_max_rel": float(np.max(np.abs(gh[tuple(gold["voxels"].T)] - fd) / np.abs(fd))),
+                 "grad_rel_l2_vs_fp64": float(np.linalg.norm(gh - ref) / np.linalg.norm(ref)),
+                 "J_rel_vs_fp64": float(abs(J - float(gold["J64"])) / float(gold["J64"]))}
     return {
         "metric": METRIC, "value": round(n * p.nsteps / (fwd * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
         "steps": p.nsteps, "warmup": 1, "ms_per_step": round(fwd / p.nsteps, 4), "higher_is_better": True,
@@ -445,7 +477,7 @@ def run_adjoint(args):
         "config": {"workload": "adjoint gradient: 160^3 Yee, CPML 10, 64^3 design box eps_r U[1,4], Ez line source on x=20 (y=80, all interior z), "
                                "J = sum of |E|^2 on x=140 over steps 601-800, checkpoints every 50", "cells": n},
         "forward_ms": round(fwd, 3), "backward_ms": round(bwd, 3), "backward_over_forward": round(bwd / fwd, 3),
-        "J": J, "grad_norm": float(g.norm()),
+        "J": J, "grad_norm": float(g.norm()), "check": check,
         "roofline": None,
         "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
     }
@@ -493,7 +525,7 @@ def cpu_baseline(args):
         if el > 10.0 or steps >= 1000:
             break
     gcells = ns ** 3 * steps / el / 1e9
-    return {"value": round(gcells, 4), "unit": "Gcell/s", "cores": cores, "kind": "port",
+    return {"value": round(gcells, 4), "unit": "Gcell/s", "cores": cores, "kind": "port", "label": "build CPU oracle",
             "sample": f"oracle float32 Yee+CPML({npml}) {ns}^3, 12 spheres, {steps} steps, {el:.1f} s, OpenMP {cores} threads"}
 
 
@@ -567,11 +599,39 @@ def main():
         "wall_s": round(wall, 3),
         "device": torch.cuda.get_device_name(0),
     }
-    if not args.no_cpu_baseline:
-        del sim
+    del sim
+    torch.cuda.empty_cache()
+    if not args.plain_only and args.n == 1024:
+        out["batched"] = batched_leg(args)
         torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_leg(args)
+        torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(args):
+    """The metric's batched leg: the 64-instance uneven batch (300 steps) against the
+    same instances run one after another."""
+    import copy
+    b = copy.copy(args)
+    b.steps, b.warmup = 300, 5
+    r = run_batch(b)
+    print(f"batched leg: {r['ms_per_step']} ms/step, {r['batched_vs_sequential']}x sequential", file=sys.stderr, flush=True)
+    return {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+            "config": r["config"], "batched_vs_sequential": r["batched_vs_sequential"], "sequential": r["sequential"],
+            "roofline": r["roofline"]}
+
+
+def adjoint_leg(args):
+    """The metric's adjoint leg: the 160^3 design problem, backward / forward time and the
+    gradient's error against the float64 references."""
+    r = run_adjoint(args)
+    print(f"adjoint leg: backward/forward {r['backward_over_forward']}", file=sys.stderr, flush=True)
+    return {"forward_ms": r["forward_ms"], "backward_ms": r["backward_ms"],
+            "backward_over_forward": r["backward_over_forward"], "forward_gcells": r["value"],
+            "config": r["config"], "check": r["check"]}
 
 
 if __name__ == "__main__":

@@ -30,6 +30,8 @@ _TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_ba
 
 
 class BatchedYee:
+    kernel_name = "k_batch (E pass + H pass of every instance)"
+
     def __init__(self, sims, use_graph=None, remap_rows=None, stream=None):
         if not sims:
             raise ValueError("need at least one simulation")

@@ -7,6 +7,7 @@
  * oracle_body.h and are instantiated for float (_f32) and double (_f64).
  */
 #include <stddef.h>
+#include <xmmintrin.h>
 
 #define REAL float
 #define SFX(name) name##_f32
@@ -19,3 +20,12 @@
 #include "oracle_body.h"
 #undef REAL
 #undef SFX
+
+/* Flush-to-zero and denormals-are-zero on the calling thread's MXCSR (threads created
+ * afterwards inherit it): the FTZ variant of the CPU baseline timing. */
+void ora_set_ftz(int on)
+{
+    unsigned m = _mm_getcsr();
+    m = on ? (m | 0x8040u) : (m & ~0x8040u);
+    _mm_setcsr(m);
+}
