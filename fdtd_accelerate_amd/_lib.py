@@ -96,6 +96,7 @@ _SIGS = {
     "mifdtd_batch_sizes": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_uint)]),
     "mifdtd_batch_run": (_I, [_P, _I, _P, _P, _I, _I, _P]),
     "mifdtd_batch_destroy": (_I, [_P]),
+    "mifdtd_batch_run_static": (_I, [ctypes.POINTER(InstanceDesc), _I, _P, _P, _P, _P, _I, _P, _P, _I, _P]),
     "mifdtd_adjoint_step": (_I, [_I] * 4 + [ctypes.POINTER(_P * 6), ctypes.POINTER(_P * 3), _I, _P, _P,
                                             _P, _P, _P, _P, _P, _I, ctypes.POINTER(DesignBox), _P, _P, _P]),
     "mifdtd_design_curl": (_I, [_I] * 4 + [_P] * 3 + [ctypes.POINTER(DesignBox), _P, _P]),

@@ -29,6 +29,63 @@ _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 _TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_batch.json")
 
 
+def instance_descs(sims):
+    """``mifdtd_instance`` descriptors of the simulations (a ctypes array) and the shared
+    CPML tables (bE, aE, bH, aH device pointers, or Nones without CPML)."""
+    ref = next((s.cpml for s in sims if s.cpml is not None), None)
+    descs = (_lib.InstanceDesc * len(sims))()
+    for n, s in enumerate(sims):
+        f = s.fields
+        d = descs[n]
+        d.nx, d.ny, d.nz_pitch, d.pad = f.nx, f.ny, f.nz_pitch, f.pad
+        for c in range(6):
+            d.e[c] = f.ehs[c].data_ptr()
+        if f.ce_on:
+            for c in range(3):
+                d.ce[c] = f.ces[c].data_ptr()
+        if s.cpml is not None:
+            d.npml = s.cpml.npml
+            d.psiE = s.cpml.psiE_buf.data_ptr()
+            d.psiH = s.cpml.psiH_buf.data_ptr()
+        d.nsrc = len(s.sources)
+        for k, src in enumerate(s.sources):
+            d.src_comp[k] = _COMP[src.str_f]
+            box = (src.pt0[0], src.pt1[0], src.pt0[1], src.pt1[1], src.pt0[2], src.pt1[2])
+            for m in range(6):
+                d.src_box[k][m] = box[m]
+            d.src_sv[k] = None if src.sv is None else src.sv.data_ptr()
+            d.src_svs[k] = src.svs
+            d.src_overwrite[k] = int(src.is_overwrite)
+        d.nprobe = len(s.probe_points)
+        for k, (c, (i, j, kk)) in enumerate(s.probe_points):
+            d.probe_comp[k] = c
+            d.probe_ijk[k][0], d.probe_ijk[k][1], d.probe_ijk[k][2] = i, j, kk
+    tabs = (None,) * 4 if ref is None else (ref.bE.data_ptr(), ref.aE.data_ptr(), ref.bH.data_ptr(), ref.aH.data_ptr())
+    return descs, tabs
+
+
+def source_values(sims, nsteps, device):
+    """Device (nsteps, sum of sources) float32 values: column n is the n-th source of the
+    batch (instances in order), row k its tfunc(tstep + k) cast to float32."""
+    total = sum(len(s.sources) for s in sims)
+    tv = np.zeros((max(nsteps, 1), max(total, 1)), np.float32)
+    col = 0
+    for s in sims:
+        for src in s.sources:
+            for k in range(nsteps):
+                tv[k, col] = np.float32(src.tfunc(src.tstep + k))
+            col += 1
+    return torch.from_numpy(tv).to(device)
+
+
+def advance(sims, nsteps):
+    """Move the sources' and simulations' step counters on after nsteps batched steps."""
+    for s in sims:
+        for src in s.sources:
+            src.tstep += nsteps
+        s.tstep += nsteps
+
+
 class BatchedYee:
     kernel_name = "k_batch (E pass + H pass of every instance)"
 
@@ -61,35 +118,7 @@ class BatchedYee:
         self.sims = list(sims)
         self.device = sims[0].fields.device
         self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
-        ref = next((s.cpml for s in sims if s.cpml is not None), None)
-        descs = (_lib.InstanceDesc * len(sims))()
-        for n, s in enumerate(sims):
-            f = s.fields
-            d = descs[n]
-            d.nx, d.ny, d.nz_pitch, d.pad = f.nx, f.ny, f.nz_pitch, f.pad
-            for c in range(6):
-                d.e[c] = f.ehs[c].data_ptr()
-            if f.ce_on:
-                for c in range(3):
-                    d.ce[c] = f.ces[c].data_ptr()
-            if s.cpml is not None:
-                d.npml = s.cpml.npml
-                d.psiE = s.cpml.psiE_buf.data_ptr()
-                d.psiH = s.cpml.psiH_buf.data_ptr()
-            d.nsrc = len(s.sources)
-            for k, src in enumerate(s.sources):
-                d.src_comp[k] = _COMP[src.str_f]
-                box = (src.pt0[0], src.pt1[0], src.pt0[1], src.pt1[1], src.pt0[2], src.pt1[2])
-                for m in range(6):
-                    d.src_box[k][m] = box[m]
-                d.src_sv[k] = None if src.sv is None else src.sv.data_ptr()
-                d.src_svs[k] = src.svs
-                d.src_overwrite[k] = int(src.is_overwrite)
-            d.nprobe = len(s.probe_points)
-            for k, (c, (i, j, kk)) in enumerate(s.probe_points):
-                d.probe_comp[k] = c
-                d.probe_ijk[k][0], d.probe_ijk[k][1], d.probe_ijk[k][2] = i, j, kk
-        tabs = (None,) * 4 if ref is None else (ref.bE.data_ptr(), ref.aE.data_ptr(), ref.bH.data_ptr(), ref.aH.data_ptr())
+        descs, tabs = instance_descs(self.sims)
         h = ctypes.c_void_p()
         _lib.call("mifdtd_batch_create", descs, len(sims), *tabs, ctypes.byref(h))
         self._h = h
@@ -118,14 +147,7 @@ class BatchedYee:
             pass
 
     def _tvals(self, nsteps):
-        tv = np.zeros((max(nsteps, 1), max(self.total_src, 1)), np.float32)
-        col = 0
-        for s in self.sims:
-            for src in s.sources:
-                for k in range(nsteps):
-                    tv[k, col] = np.float32(src.tfunc(src.tstep + k))
-                col += 1
-        return torch.from_numpy(tv).to(self.device)
+        return source_values(self.sims, nsteps, self.device)
 
     def run(self, nsteps, collect=True):
         """Advance every instance nsteps steps; returns the per-instance probe series
@@ -140,10 +162,7 @@ class BatchedYee:
         torch.cuda.current_stream(self.device).wait_stream(self.stream)
         tv.record_stream(self.stream)
         buf.record_stream(self.stream)
-        for s in self.sims:
-            for src in s.sources:
-                src.tstep += nsteps
-            s.tstep += nsteps
+        advance(self.sims, nsteps)
         if not collect:
             return None
         host = buf.cpu().numpy()[:nsteps]
@@ -170,10 +189,7 @@ class BatchedYee:
                   self.remap_rows, self.stream.cuda_stream)
         b.record(self.stream)
         torch.cuda.synchronize()
-        for s in self.sims:
-            for src in s.sources:
-                src.tstep += nsteps
-            s.tstep += nsteps
+        advance(self.sims, nsteps)
         return a.elapsed_time(b) / max(nsteps, 1)
 
 

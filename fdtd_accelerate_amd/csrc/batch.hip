@@ -13,6 +13,7 @@
 
 #include "step_quad.hpp"
 
+#include <mutex>
 #include <vector>
 
 namespace mifdtd {
@@ -70,25 +71,19 @@ __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, co
 
 using namespace mifdtd;
 
-struct mifdtd_batch {
+// What the launches of a batch need besides the device descriptors.
+struct BMeta {
     int ninst = 0;
     bool hc = false, pml = false;
     unsigned total_blocks = 0;
     int total_src = 0, total_probe = 0;
-    BInst *d_inst = nullptr;
-    unsigned *d_first = nullptr;
 };
 
-extern "C" {
-
-int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE, const float *aE, const float *bH,
-                        const float *aH, mifdtd_batch **out)
+// Validate the instances and fill h[0..ninst) and first[0..ninst] (host arrays).
+static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, const float *aE, const float *bH,
+                       const float *aH, BInst *h, unsigned *first, BMeta &m)
 {
-    if (!out) return fail(MIFDTD_ENULL, "out is NULL");
-    *out = nullptr;
     if (!inst || ninst < 1) return fail(MIFDTD_EINVAL, "need at least one instance");
-    std::vector<BInst> h(ninst);
-    std::vector<unsigned> first(ninst + 1, 0);
     const bool hc = inst[0].ce[0] != nullptr;
     bool pml = false;
     int batch_npml = 0;  // the one npml of the shared 2*(npml+1)-entry tables
@@ -183,13 +178,124 @@ int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE,
         if (blocks >= (1ull << 31)) return fail(MIFDTD_ERANGE, "batch exceeds 2^31 blocks");
     }
     first[ninst] = (unsigned)blocks;
+    m.ninst = ninst;
+    m.hc = hc;
+    m.pml = pml;
+    m.total_blocks = (unsigned)blocks;
+    m.total_src = tsrc;
+    m.total_probe = tprobe;
+    return 0;
+}
+
+static int batch_launches(const BMeta &b, const BInst *d_inst, const unsigned *d_first, int nsteps, const float *tvals,
+                          float *probe_series, unsigned R, hipStream_t st)
+{
+    const unsigned grid = b.total_blocks;
+    for (int s = 0; s < nsteps; s++) {
+        const float *tv = b.total_src ? tvals + (size_t)s * b.total_src : nullptr;
+        float *pr = b.total_probe ? probe_series + (size_t)s * b.total_probe : nullptr;
+#define MIFDTD_BATCH_LAUNCH(HC, PML, EH) \
+    k_batch<HC, PML, EH><<<grid, SB, 0, st>>>(d_inst, d_first, b.ninst, tv, pr, R)
+        if (b.hc && b.pml) MIFDTD_BATCH_LAUNCH(true, true, true);
+        else if (b.hc) MIFDTD_BATCH_LAUNCH(true, false, true);
+        else if (b.pml) MIFDTD_BATCH_LAUNCH(false, true, true);
+        else MIFDTD_BATCH_LAUNCH(false, false, true);
+        MIFDTD_CHECK_LAUNCH("batch E launch");
+        if (b.pml) MIFDTD_BATCH_LAUNCH(false, true, false);
+        else MIFDTD_BATCH_LAUNCH(false, false, false);
+        MIFDTD_CHECK_LAUNCH("batch H launch");
+#undef MIFDTD_BATCH_LAUNCH
+    }
+    return 0;
+}
+
+static int check_run_args(const BMeta &b, int nsteps, const float *tvals, float *probe_series, int remap_rows)
+{
+    if (nsteps < 0) return fail(MIFDTD_EINVAL, "nsteps < 0");
+    if (b.total_src && !tvals) return fail(MIFDTD_ENULL, "tvals is NULL");
+    if (b.total_probe && !probe_series) return fail(MIFDTD_ENULL, "probe_series is NULL");
+    if (remap_rows < 0 || remap_rows > 4096) return fail(MIFDTD_EINVAL, "remap_rows %d outside [0,4096]", remap_rows);
+    return 0;
+}
+
+// ------------------------------------------------------------ allocation-free run
+// mifdtd_batch_run_static keeps the descriptors in this file-scope device array instead
+// of device memory from hipMalloc: one copy per GPU, held until the process ends, shared
+// by every call on that GPU (overlapping calls on one GPU race on it).  Host code reaches
+// it only through the address hipGetSymbolAddress returns for the current device (the
+// array's name is a host-side stand-in, not a device address).
+constexpr int kStaticMaxInst = MIFDTD_BATCH_STATIC_MAX_INST;
+constexpr size_t kStaticFirstOff = ((size_t)kStaticMaxInst * sizeof(BInst) + 255) & ~(size_t)255;
+constexpr size_t kStaticBytes = kStaticFirstOff + sizeof(unsigned) * (kStaticMaxInst + 1);
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// The current device's address of `scratch` (looked up once per device).
+static int scratch_ptr(unsigned char **out)
+{
+    int dev = 0;
+    if (hipError_t e = hipGetDevice(&dev)) return hip_status(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(MIFDTD_ERANGE, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) return hip_status(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(MIFDTD_ENULL, "hipGetSymbolAddress(scratch) gave NULL");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+    return 0;
+}
+
+// Host bytes reach the scratch array as kernel arguments of k_put (2 KiB per launch), so
+// no host buffer has to outlive the call and no staging memory is involved.
+constexpr int kPutWords = 512;
+struct PutChunk {
+    unsigned w[kPutWords];
+};
+
+__global__ __launch_bounds__(256) void k_put(unsigned *__restrict__ dst, PutChunk c, int nw)
+{
+    for (int t = threadIdx.x; t < nw; t += 256) dst[t] = c.w[t];
+}
+
+static int put_bytes(unsigned char *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const unsigned *w = static_cast<const unsigned *>(src);
+    const size_t nw = (bytes + 3) / 4;  // whole words: the sources are word-sized structs and arrays
+    for (size_t o = 0; o < nw; o += kPutWords) {
+        PutChunk c;
+        const int n = (int)(nw - o < (size_t)kPutWords ? nw - o : (size_t)kPutWords);
+        for (int t = 0; t < n; t++) c.w[t] = w[o + t];
+        k_put<<<1, 256, 0, st>>>(reinterpret_cast<unsigned *>(dst) + o, c, n);
+        MIFDTD_CHECK_LAUNCH("descriptor put");
+    }
+    return 0;
+}
+
+struct mifdtd_batch {
+    BMeta m;
+    BInst *d_inst = nullptr;
+    unsigned *d_first = nullptr;
+};
+
+extern "C" {
+
+int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE, const float *aE, const float *bH,
+                        const float *aH, mifdtd_batch **out)
+{
+    if (!out) return fail(MIFDTD_ENULL, "out is NULL");
+    *out = nullptr;
+    if (!inst || ninst < 1) return fail(MIFDTD_EINVAL, "need at least one instance");
+    std::vector<BInst> h(ninst);
+    std::vector<unsigned> first(ninst + 1, 0);
+    BMeta m;
+    if (int rc = build_batch(inst, ninst, bE, aE, bH, aH, h.data(), first.data(), m)) return rc;
     auto *bt = new mifdtd_batch;
-    bt->ninst = ninst;
-    bt->hc = hc;
-    bt->pml = pml;
-    bt->total_blocks = (unsigned)blocks;
-    bt->total_src = tsrc;
-    bt->total_probe = tprobe;
+    bt->m = m;
     int rc = hip_status(hipMalloc(&bt->d_inst, sizeof(BInst) * ninst), "batch descriptor alloc");
     if (!rc) rc = hip_status(hipMalloc(&bt->d_first, sizeof(unsigned) * (ninst + 1)), "batch prefix alloc");
     if (!rc) rc = hip_status(hipMemcpy(bt->d_inst, h.data(), sizeof(BInst) * ninst, hipMemcpyHostToDevice), "batch upload");
@@ -209,30 +315,9 @@ int mifdtd_batch_create(const mifdtd_instance *inst, int ninst, const float *bE,
 int mifdtd_batch_sizes(const mifdtd_batch *b, int *total_src, int *total_probe, unsigned *total_blocks)
 {
     if (!b) return fail(MIFDTD_ENULL, "batch is NULL");
-    if (total_src) *total_src = b->total_src;
-    if (total_probe) *total_probe = b->total_probe;
-    if (total_blocks) *total_blocks = b->total_blocks;
-    return 0;
-}
-
-static int batch_launches(mifdtd_batch *b, int nsteps, const float *tvals, float *probe_series, unsigned R, hipStream_t st)
-{
-    const unsigned grid = b->total_blocks;
-    for (int s = 0; s < nsteps; s++) {
-        const float *tv = b->total_src ? tvals + (size_t)s * b->total_src : nullptr;
-        float *pr = b->total_probe ? probe_series + (size_t)s * b->total_probe : nullptr;
-#define MIFDTD_BATCH_LAUNCH(HC, PML, EH) \
-    k_batch<HC, PML, EH><<<grid, SB, 0, st>>>(b->d_inst, b->d_first, b->ninst, tv, pr, R)
-        if (b->hc && b->pml) MIFDTD_BATCH_LAUNCH(true, true, true);
-        else if (b->hc) MIFDTD_BATCH_LAUNCH(true, false, true);
-        else if (b->pml) MIFDTD_BATCH_LAUNCH(false, true, true);
-        else MIFDTD_BATCH_LAUNCH(false, false, true);
-        MIFDTD_CHECK_LAUNCH("batch E launch");
-        if (b->pml) MIFDTD_BATCH_LAUNCH(false, true, false);
-        else MIFDTD_BATCH_LAUNCH(false, false, false);
-        MIFDTD_CHECK_LAUNCH("batch H launch");
-#undef MIFDTD_BATCH_LAUNCH
-    }
+    if (total_src) *total_src = b->m.total_src;
+    if (total_probe) *total_probe = b->m.total_probe;
+    if (total_blocks) *total_blocks = b->m.total_blocks;
     return 0;
 }
 
@@ -240,19 +325,16 @@ int mifdtd_batch_run(mifdtd_batch *b, int nsteps, const float *tvals, float *pro
                      int remap_rows, mifdtd_stream_t stream)
 {
     if (!b) return fail(MIFDTD_ENULL, "batch is NULL");
-    if (nsteps < 0) return fail(MIFDTD_EINVAL, "nsteps < 0");
-    if (b->total_src && !tvals) return fail(MIFDTD_ENULL, "tvals is NULL");
-    if (b->total_probe && !probe_series) return fail(MIFDTD_ENULL, "probe_series is NULL");
-    if (remap_rows < 0 || remap_rows > 4096) return fail(MIFDTD_EINVAL, "remap_rows %d outside [0,4096]", remap_rows);
+    if (int rc = check_run_args(b->m, nsteps, tvals, probe_series, remap_rows)) return rc;
     if (nsteps == 0) return 0;
     hipStream_t st = to_stream(stream);
-    if (!use_graph) return batch_launches(b, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
+    if (!use_graph) return batch_launches(b->m, b->d_inst, b->d_first, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
     if (!st) return fail(MIFDTD_EINVAL, "graph capture needs a non-null stream");
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int rc = hip_status(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc) return rc;
-    const int lrc = batch_launches(b, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
+    const int lrc = batch_launches(b->m, b->d_inst, b->d_first, nsteps, tvals, probe_series, (unsigned)remap_rows, st);
     rc = hip_status(hipStreamEndCapture(st, &graph), "end capture");
     if (lrc) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -275,6 +357,29 @@ int mifdtd_batch_destroy(mifdtd_batch *b)
     if (b->d_first && !rc) rc = hip_status(hipFree(b->d_first), "batch free");
     delete b;
     return rc;
+}
+
+int mifdtd_batch_run_static(const mifdtd_instance *inst, int ninst, const float *bE, const float *aE, const float *bH,
+                            const float *aH, int nsteps, const float *tvals, float *probe_series, int remap_rows,
+                            mifdtd_stream_t stream)
+{
+    static_assert(sizeof(BInst) % 4 == 0, "descriptors are copied as words");
+    if (ninst > kStaticMaxInst)
+        return fail(MIFDTD_ERANGE, "%d instances: the allocation-free run holds at most %d (MIFDTD_BATCH_STATIC_MAX_INST)",
+                    ninst, kStaticMaxInst);
+    static thread_local BInst h[kStaticMaxInst];
+    static thread_local unsigned first[kStaticMaxInst + 1];
+    BMeta m;
+    if (int rc = build_batch(inst, ninst, bE, aE, bH, aH, h, first, m)) return rc;
+    if (int rc = check_run_args(m, nsteps, tvals, probe_series, remap_rows)) return rc;
+    if (nsteps == 0) return 0;
+    unsigned char *base = nullptr;
+    if (int rc = scratch_ptr(&base)) return rc;
+    hipStream_t st = to_stream(stream);
+    if (int rc = put_bytes(base, h, sizeof(BInst) * ninst, st)) return rc;
+    if (int rc = put_bytes(base + kStaticFirstOff, first, sizeof(unsigned) * (ninst + 1), st)) return rc;
+    return batch_launches(m, reinterpret_cast<const BInst *>(base), reinterpret_cast<const unsigned *>(base + kStaticFirstOff),
+                          nsteps, tvals, probe_series, (unsigned)remap_rows, st);
 }
 
 }  // extern "C"

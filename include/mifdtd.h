@@ -345,6 +345,23 @@ int mifdtd_batch_run(mifdtd_batch *b, int nsteps, const float *tvals, float *pro
                      int use_graph, int remap_rows, mifdtd_stream_t stream);
 int mifdtd_batch_destroy(mifdtd_batch *b);
 
+/* The batched run without any device allocation: the same validation, kernels and
+ * results as mifdtd_batch_create + mifdtd_batch_run (use_graph 0) + mifdtd_batch_destroy,
+ * with the instance descriptors held in a static device array of the library instead of
+ * memory from hipMalloc.  No reference counterpart: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  At most MIFDTD_BATCH_STATIC_MAX_INST instances per call (rejected with
+ * MIFDTD_ERANGE beyond).  Each GPU holds one such array until the process ends; every
+ * call on a GPU rewrites it (nothing has to start at zero), so calls that overlap on one
+ * GPU (other streams, other host threads) race.  tvals: DEVICE (nsteps, sum of the
+ * instances' nsrc); probe_series: DEVICE (nsteps, sum of nprobe); columns as for
+ * mifdtd_batch_run. */
+#define MIFDTD_BATCH_STATIC_MAX_INST 128
+int mifdtd_batch_run_static(const mifdtd_instance *inst, int ninst,
+                            const float *bE, const float *aE, const float *bH, const float *aH,
+                            int nsteps, const float *tvals, float *probe_series, int remap_rows,
+                            mifdtd_stream_t stream);
+
 /* ---------------------------------------------------------------- adjoint ----
  * Discrete adjoint of one in-place Yee + CPML step (update_e, mifdtd_cpml_e, sources,
  * update_h, mifdtd_cpml_h with CH = 0.5 and the packed psi layout), for gradients with

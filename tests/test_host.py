@@ -235,3 +235,12 @@ def test_kufdtd_shaped_psi_is_rejected():
         check_face_psi(F, axis, npml, good, good)
         with pytest.raises(ValueError):
             check_face_psi(F, axis, npml, good, good.double())
+
+
+def test_static_batch_rejects_beyond_its_limit_without_gpu():
+    """The allocation-free batched run refuses more instances than its static array holds,
+    before any device work."""
+    from fdtd_accelerate_amd import _lib
+    descs = (_lib.InstanceDesc * 129)()
+    rc = _lib.load().mifdtd_batch_run_static(descs, 129, None, None, None, None, 1, None, None, 0, None)
+    assert rc == -4 and "at most 128" in _lib.last_error()
