@@ -18,6 +18,8 @@
 namespace mifdtd {
 
 constexpr int S4B = 256;
+constexpr int kS4Rows = 8;     // owned rows per work-group of the z-marching stages (T+3 waves)
+constexpr int kS4Planes = 32;  // planes per z chunk
 
 struct S4Geom {
     int nx, ny, nz;
@@ -159,6 +161,273 @@ __global__ __launch_bounds__(S4B) void k_s4_e(S4Geom g, float dl,
     if (rx) st4(ex + id, e0);
     if (ry) st4(ey + id, e1);
     if (rz) st4(ez + id, e2);
+}
+
+// ------------------------------------------------------------------ z-marching stages
+// The same stage updates with the neighbour rows and planes read once per work-group:
+// a work-group owns one x segment of 64 quads (a lane per quad) of T rows [j0, j0+T) and
+// marches the planes of a z chunk [k0, k1).  Wave w holds row j0-1+w (E stage) or
+// j0-2+w (H stage), so that the T owned rows have the y halo rows (E: j-1, j+1, j+2;
+// H: j-2, j-1, j+1) as waves of the same work-group; per plane each wave publishes the
+// two components its neighbours need (E: hz, hx; H: ez, ex) in LDS (double-buffered, one
+// barrier per plane).  The z neighbours (E: hy, hx at k-1 .. k+2; H: ey, ex at k-2 ..
+// k+1) stay in a four-plane register window per owned lane, and the x neighbours come
+// from the adjacent lanes (the segment's edge lanes load theirs).  Arithmetic, guards
+// and stores are those of k_s4_e / k_s4_h, so the values are identical.
+struct S4ZGeom {
+    int nx, ny, nz;
+    int Qx;   // quads per x row
+    int nxs;  // x segments of 64 quads per row
+    int T;    // owned rows per work-group
+    int nyt;  // row tiles
+    int cz;   // planes per z chunk
+    int nzc;  // z chunks
+};
+
+// x neighbours of this lane's quad q of f (x = i0 .. i0+3): f[i0-2], f[i0-1], f[i0+4],
+// f[i0+5], zero outside the row.  Called by every lane of the wave (shuffles).
+struct XN {
+    float m2, m1, p4, p5;
+};
+
+__device__ __forceinline__ XN xneigh(const float *f, size_t id, const Q4 &q, int lane, int qi, int Qx)
+{
+    XN n;
+    n.m2 = __shfl_up(q.v[2], 1);
+    n.m1 = __shfl_up(q.v[3], 1);
+    n.p4 = __shfl_down(q.v[0], 1);
+    n.p5 = __shfl_down(q.v[1], 1);
+    if (lane == 0) {
+        n.m2 = qi > 0 ? f[id - 2] : 0.f;
+        n.m1 = qi > 0 ? f[id - 1] : 0.f;
+    }
+    if (qi + 1 >= Qx) {
+        n.p4 = n.p5 = 0.f;
+    } else if (lane == 63) {
+        n.p4 = f[id + 4];
+        n.p5 = f[id + 5];
+    }
+    return n;
+}
+
+__device__ __forceinline__ Q4 lds4(const float *p) { return Q4{{p[0], p[1], p[2], p[3]}}; }
+
+__device__ __forceinline__ void put4(float *p, const Q4 &q)
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+}
+
+template <bool HC>
+__global__ __launch_bounds__(1024) void k_s4_e_zm(S4ZGeom g, float dl, float *__restrict__ ex, float *__restrict__ ey,
+                                                  float *__restrict__ ez, const float *__restrict__ hx,
+                                                  const float *__restrict__ hy, const float *__restrict__ hz,
+                                                  const float *__restrict__ cx, const float *__restrict__ cy,
+                                                  const float *__restrict__ cz, float ce0)
+{
+    extern __shared__ float zl[];  // [2][T+3][2][256]: hz, hx of plane k, rows j0-1 .. j0+T+1
+    const int T = g.T;
+    unsigned b = blockIdx.x;
+    const int xs = (int)(b % (unsigned)g.nxs);
+    b /= (unsigned)g.nxs;
+    const int yt = (int)(b % (unsigned)g.nyt);
+    const int zc = (int)(b / (unsigned)g.nyt);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int j0 = yt * T, k0 = zc * g.cz, k1 = min(nz, k0 + g.cz);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int j = j0 - 1 + w;
+    const int qi = xs * 64 + lane;
+    const bool qact = qi < g.Qx;
+    const int i0 = 4 * (qact ? qi : g.Qx - 1);  // inactive lanes address the row's last quad
+    const bool rowok = j >= 0 && j < ny;
+    const size_t sz = (size_t)nx * ny;
+    const size_t rowoff = (size_t)(rowok ? j : 0) * nx + i0;
+    const bool own = w >= 1 && w <= T && j < ny;
+    const float c27 = 27.f;
+    const Q4 z4 = splat4(0.f);
+    Q4 hyw[4], hxw[4];  // planes k-1 .. k+2 of this lane's quad (owned rows)
+    if (own) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int kk = k0 - 1 + m;
+            const bool in = kk >= 0 && kk < nz;
+            hyw[m] = in ? ld4(hy + (size_t)kk * sz + rowoff) : z4;
+            hxw[m] = in ? ld4(hx + (size_t)kk * sz + rowoff) : z4;
+        }
+    }
+    for (int k = k0; k < k1; ++k) {
+        float *buf = zl + (size_t)((k - k0) & 1) * (T + 3) * 512;
+        const size_t idk = (size_t)k * sz + rowoff;
+        const Q4 hzk = rowok ? ld4(hz + idk) : z4;
+        Q4 hxk, e0, e1, e2, c0, c1, c2;
+        if (own) {
+            const bool in2 = k + 2 < nz;
+            hyw[3] = in2 ? ld4(hy + idk + 2 * sz) : z4;
+            hxw[3] = in2 ? ld4(hx + idk + 2 * sz) : z4;
+            hxk = hxw[1];
+            e0 = ld4(ex + idk);
+            e1 = ld4(ey + idk);
+            e2 = ld4(ez + idk);
+            c0 = HC ? ld4(cx + idk) : splat4(ce0);
+            c1 = HC ? ld4(cy + idk) : splat4(ce0);
+            c2 = HC ? ld4(cz + idk) : splat4(ce0);
+        } else {
+            hxk = rowok ? ld4(hx + idk) : z4;
+        }
+        put4(buf + (size_t)(w * 2) * 256 + 4 * lane, hzk);
+        put4(buf + (size_t)(w * 2 + 1) * 256 + 4 * lane, hxk);
+        XN nz_{}, ny_{};
+        if (own) {
+            nz_ = xneigh(hz, idk, hzk, lane, qi, g.Qx);
+            ny_ = xneigh(hy, idk, hyw[1], lane, qi, g.Qx);
+        }
+        __syncthreads();
+        if (own && qact && j <= ny - 2 && k <= nz - 2) {
+            const bool rx = j > 0 && j < ny - 2 && k > 0 && k < nz - 2;
+            const bool ry = k > 0 && k < nz - 2;
+            const bool rz = j > 0 && j < ny - 2;
+            const Q4 hzjm = lds4(buf + (size_t)((w - 1) * 2) * 256 + 4 * lane);
+            const Q4 hzjp = lds4(buf + (size_t)((w + 1) * 2) * 256 + 4 * lane);
+            const Q4 hzj2 = lds4(buf + (size_t)((w + 2) * 2) * 256 + 4 * lane);
+            const Q4 hxjm = lds4(buf + (size_t)((w - 1) * 2 + 1) * 256 + 4 * lane);
+            const Q4 hxjp = lds4(buf + (size_t)((w + 1) * 2 + 1) * 256 + 4 * lane);
+            const Q4 hxj2 = lds4(buf + (size_t)((w + 2) * 2 + 1) * 256 + 4 * lane);
+            // x windows f[i0-2 .. i0+5] of hz and hy at plane k
+            const float wz[8] = {nz_.m2, nz_.m1, hzk.v[0], hzk.v[1], hzk.v[2], hzk.v[3], nz_.p4, nz_.p5};
+            const float wy[8] = {ny_.m2, ny_.m1, hyw[1].v[0], hyw[1].v[1], hyw[1].v[2], hyw[1].v[3], ny_.p4, ny_.p5};
+            const int ib = 4 * qi;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int i = ib + l;
+                if (i > nx - 2) continue;
+                if (rx)
+                    e0.v[l] += dl * c0.v[l] * (c27 * (hzjp.v[l] - hzk.v[l] - hyw[2].v[l] + hyw[1].v[l])
+                                               - (hzj2.v[l] - hzjm.v[l] - hyw[3].v[l] + hyw[0].v[l]));
+                if (i > 0 && i < nx - 2 && ry)
+                    e1.v[l] += dl * c1.v[l] * (c27 * (hxw[2].v[l] - hxw[1].v[l] - wz[l + 3] + hzk.v[l])
+                                               - (hxw[3].v[l] - hxw[0].v[l] - wz[l + 4] + wz[l + 1]));
+                if (i > 0 && i < nx - 2 && rz)
+                    e2.v[l] += dl * c2.v[l] * (c27 * (wy[l + 3] - hyw[1].v[l] - hxjp.v[l] + hxw[1].v[l])
+                                               - (wy[l + 4] - wy[l + 1] - hxj2.v[l] + hxjm.v[l]));
+            }
+            if (rx) st4(ex + idk, e0);
+            if (ry) st4(ey + idk, e1);
+            if (rz) st4(ez + idk, e2);
+        }
+        if (own) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                hyw[m] = hyw[m + 1];
+                hxw[m] = hxw[m + 1];
+            }
+        }
+    }
+}
+
+template <bool HC>
+__global__ __launch_bounds__(1024) void k_s4_h_zm(S4ZGeom g, float cl, const float *__restrict__ ex,
+                                                  const float *__restrict__ ey, const float *__restrict__ ez,
+                                                  float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+                                                  const float *__restrict__ cx, const float *__restrict__ cy,
+                                                  const float *__restrict__ cz, float ch0)
+{
+    extern __shared__ float zl[];  // [2][T+3][2][256]: ez, ex of plane k, rows j0-2 .. j0+T
+    const int T = g.T;
+    unsigned b = blockIdx.x;
+    const int xs = (int)(b % (unsigned)g.nxs);
+    b /= (unsigned)g.nxs;
+    const int yt = (int)(b % (unsigned)g.nyt);
+    const int zc = (int)(b / (unsigned)g.nyt);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int j0 = yt * T, k0 = zc * g.cz, k1 = min(nz, k0 + g.cz);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int j = j0 - 2 + w;
+    const int qi = xs * 64 + lane;
+    const bool qact = qi < g.Qx;
+    const int i0 = 4 * (qact ? qi : g.Qx - 1);
+    const bool rowok = j >= 0 && j < ny;
+    const size_t sz = (size_t)nx * ny;
+    const size_t rowoff = (size_t)(rowok ? j : 0) * nx + i0;
+    const bool own = w >= 2 && w <= T + 1 && j < ny;
+    const float c27 = 27.f;
+    const Q4 z4 = splat4(0.f);
+    Q4 eyw[4], exw[4];  // planes k-2 .. k+1 of this lane's quad (owned rows)
+    if (own) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int kk = k0 - 2 + m;
+            const bool in = kk >= 0 && kk < nz;
+            eyw[m] = in ? ld4(ey + (size_t)kk * sz + rowoff) : z4;
+            exw[m] = in ? ld4(ex + (size_t)kk * sz + rowoff) : z4;
+        }
+    }
+    for (int k = k0; k < k1; ++k) {
+        float *buf = zl + (size_t)((k - k0) & 1) * (T + 3) * 512;
+        const size_t idk = (size_t)k * sz + rowoff;
+        const Q4 ezk = rowok ? ld4(ez + idk) : z4;
+        Q4 exk, h0, h1, h2, c0, c1, c2;
+        if (own) {
+            const bool in1 = k + 1 < nz;
+            eyw[3] = in1 ? ld4(ey + idk + sz) : z4;
+            exw[3] = in1 ? ld4(ex + idk + sz) : z4;
+            exk = exw[2];
+            h0 = ld4(hx + idk);
+            h1 = ld4(hy + idk);
+            h2 = ld4(hz + idk);
+            c0 = HC ? ld4(cx + idk) : splat4(ch0);
+            c1 = HC ? ld4(cy + idk) : splat4(ch0);
+            c2 = HC ? ld4(cz + idk) : splat4(ch0);
+        } else {
+            exk = rowok ? ld4(ex + idk) : z4;
+        }
+        put4(buf + (size_t)(w * 2) * 256 + 4 * lane, ezk);
+        put4(buf + (size_t)(w * 2 + 1) * 256 + 4 * lane, exk);
+        XN nz_{}, ny_{};
+        if (own) {
+            nz_ = xneigh(ez, idk, ezk, lane, qi, g.Qx);
+            ny_ = xneigh(ey, idk, eyw[2], lane, qi, g.Qx);
+        }
+        __syncthreads();
+        if (own && qact && j <= ny - 2 && k <= nz - 2) {
+            const bool rx = j > 1 && k > 1;
+            const bool ry = k > 1;
+            const bool rz = j > 1;
+            const Q4 ezjm = lds4(buf + (size_t)((w - 1) * 2) * 256 + 4 * lane);
+            const Q4 ezjp = lds4(buf + (size_t)((w + 1) * 2) * 256 + 4 * lane);
+            const Q4 ezj2 = lds4(buf + (size_t)((w - 2) * 2) * 256 + 4 * lane);
+            const Q4 exjm = lds4(buf + (size_t)((w - 1) * 2 + 1) * 256 + 4 * lane);
+            const Q4 exjp = lds4(buf + (size_t)((w + 1) * 2 + 1) * 256 + 4 * lane);
+            const Q4 exj2 = lds4(buf + (size_t)((w - 2) * 2 + 1) * 256 + 4 * lane);
+            const float wz[8] = {nz_.m2, nz_.m1, ezk.v[0], ezk.v[1], ezk.v[2], ezk.v[3], nz_.p4, nz_.p5};
+            const float wy[8] = {ny_.m2, ny_.m1, eyw[2].v[0], eyw[2].v[1], eyw[2].v[2], eyw[2].v[3], ny_.p4, ny_.p5};
+            const int ib = 4 * qi;
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                const int i = ib + l;
+                if (i > nx - 2) continue;
+                if (rx && j < ny - 1 && k < nz - 1)
+                    h0.v[l] -= cl * c0.v[l] * (c27 * (ezk.v[l] - ezjm.v[l] - eyw[2].v[l] + eyw[1].v[l])
+                                               - (ezjp.v[l] - ezj2.v[l] - eyw[3].v[l] + eyw[0].v[l]));
+                if (i > 1 && ry)
+                    h1.v[l] -= cl * c1.v[l] * (c27 * (exw[2].v[l] - exw[1].v[l] - ezk.v[l] + wz[l + 1])
+                                               - (exw[3].v[l] - exw[0].v[l] - wz[l + 3] + wz[l]));
+                if (i > 1 && rz)
+                    h2.v[l] -= cl * c2.v[l] * (c27 * (eyw[2].v[l] - wy[l + 1] - exw[2].v[l] + exjm.v[l])
+                                               - (wy[l + 3] - wy[l] - exjp.v[l] + exj2.v[l]));
+            }
+            if (rx) st4(hx + idk, h0);
+            if (ry) st4(hy + idk, h1);
+            if (rz) st4(hz + idk, h2);
+        }
+        if (own) {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                eyw[m] = eyw[m + 1];
+                exw[m] = exw[m + 1];
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ CPML
@@ -328,20 +597,27 @@ static int s4_core(bool h_half, int nx, int ny, int nz, float w, float *ex, floa
     if (hc && !(c0 && c1 && c2)) return fail(MIFDTD_ENULL, "coefficient arrays must be all set or all NULL");
     if (hc && (!aligned16(c0) || !aligned16(c1) || !aligned16(c2)))
         return fail(MIFDTD_EALIGN, "coefficient pointer not 16-byte aligned");
-    S4Geom g;
-    g.nx = nx;
-    g.ny = ny;
-    g.nz = nz;
-    g.Qx = (unsigned)(nx / 4);
-    g.nquads = g.Qx * (unsigned)ny * (unsigned)nz;
-    const unsigned grid = (g.nquads + S4B - 1) / S4B;
+    S4ZGeom z;
+    z.nx = nx;
+    z.ny = ny;
+    z.nz = nz;
+    z.Qx = nx / 4;
+    z.nxs = (z.Qx + 63) / 64;
+    z.T = kS4Rows;
+    z.nyt = (ny + z.T - 1) / z.T;
+    z.cz = kS4Planes;
+    z.nzc = (nz + z.cz - 1) / z.cz;
+    const unsigned long long nb = (unsigned long long)z.nxs * z.nyt * z.nzc;
+    if (nb >= (1ull << 31)) return fail(MIFDTD_ERANGE, "4th-order grid too large");
+    const unsigned threads = 64u * (unsigned)(z.T + 3);
+    const size_t lds = sizeof(float) * 2 * (size_t)(z.T + 3) * 512;
     if (h_half) {
-        if (hc) k_s4_h<true><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
-        else k_s4_h<false><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        if (hc) k_s4_h_zm<true><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        else k_s4_h_zm<false><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
         MIFDTD_CHECK_LAUNCH("s4 update_h launch");
     } else {
-        if (hc) k_s4_e<true><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
-        else k_s4_e<false><<<grid, S4B, 0, st>>>(g, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        if (hc) k_s4_e_zm<true><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        else k_s4_e_zm<false><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
         MIFDTD_CHECK_LAUNCH("s4 update_e launch");
     }
     return 0;
