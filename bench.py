@@ -58,9 +58,9 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
                        ("kemp", 1024, 12, "uint8 index + table"): 53.24,
                        ("fused", 1024, 12, "uint8 index + table"): 51.83}
-# The 4th-order E stage (k_s4_e, CE arrays), same method, keyed by (n, npml);
-# profiles/k_s4_symplectic_cpml10_512cube_hbm_counters.txt.
-MEASURED_S4_TRAFFIC_GB = {(512, 10): 8.17}
+# The 4th-order E stage (k_s4_e_zm, CE as uint8 material indices), same method, keyed by
+# (n, npml); filled from profiles/ when measured.
+MEASURED_S4_TRAFFIC_GB = {}
 # The batched step (both launches of one step, all 64 instances), same method, keyed by
 # the batch's size key; filled from profiles/ when measured.
 MEASURED_BATCH_TRAFFIC_GB = {}
@@ -242,10 +242,11 @@ def run_s4(args):
         "config": {"workload": f"4th-order symplectic FP32 {n}^3, {nsph} spheres, CPML {npml}, S=0.9*S_max={sim.S:.4f}, Ez point source",
                    "cells": cells},
         "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "roofline": {"bound": "hbm", "kernel": "k_s4_e", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_s4_e_zm (E stage)", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_S4_TRAFFIC_GB.get((n, npml)), "traffic_unit": "GB per launch",
-                     "bytes_per_cell": 48.0, "avg_launch_ms": round(kern_ms, 4)},
+                     "bytes_per_cell": 48.0, "bytes_basis": "read E, H, CE as float32, write E (the kernel reads CE as "
+                     "3 B of material indices: 39 B moved)", "avg_launch_ms": round(kern_ms, 4)},
         "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
     }
 

@@ -104,6 +104,7 @@ _SIGS = {
     "s4_update_e": (None, [_I] * 3 + [_F] + [_P] * 9),
     "mifdtd_s4_update_h": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),
     "mifdtd_s4_update_e": (_I, [_I] * 3 + [_F] + [_P] * 9 + [_F, _P]),
+    "mifdtd_s4_update_e_lut": (_I, [_I] * 3 + [_F] + [_P] * 10 + [_P]),
     "mifdtd_s4_cpml_e": (_I, [_I] * 3 + [ctypes.POINTER(CpmlHalf), _F] + [_P] * 9 + [_F, _P]),
     "mifdtd_s4_cpml_h": (_I, [_I] * 3 + [ctypes.POINTER(CpmlHalf), _F] + [_P] * 9 + [_F, _P]),
 }

@@ -217,15 +217,36 @@ __device__ __forceinline__ void put4(float *p, const Q4 &q)
     *reinterpret_cast<float4 *>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
 }
 
-template <bool HC>
+// CE of the E stage: the constant ce0, three float arrays, or three uint8 material-index
+// arrays into a 3 x 256 table of the float values (staged in LDS; the same products).
+enum S4Ce { kS4CeConst = 0, kS4CeArrays = 1, kS4CeTable = 2 };
+
+template <int CEM>
+__device__ __forceinline__ Q4 s4_ce(const float *c, const unsigned char *m, const float *tab, size_t id, float ce0)
+{
+    if (CEM == kS4CeConst) return splat4(ce0);
+    if (CEM == kS4CeArrays) return ld4(c + id);
+    const uchar4 t = *reinterpret_cast<const uchar4 *>(m + id);
+    return Q4{{tab[t.x], tab[t.y], tab[t.z], tab[t.w]}};
+}
+
+template <int CEM>
 __global__ __launch_bounds__(1024) void k_s4_e_zm(S4ZGeom g, float dl, float *__restrict__ ex, float *__restrict__ ey,
                                                   float *__restrict__ ez, const float *__restrict__ hx,
                                                   const float *__restrict__ hy, const float *__restrict__ hz,
                                                   const float *__restrict__ cx, const float *__restrict__ cy,
-                                                  const float *__restrict__ cz, float ce0)
+                                                  const float *__restrict__ cz, float ce0,
+                                                  const unsigned char *__restrict__ mx,
+                                                  const unsigned char *__restrict__ my,
+                                                  const unsigned char *__restrict__ mz, const float *__restrict__ lut)
 {
-    extern __shared__ float zl[];  // [2][T+3][2][256]: hz, hx of plane k, rows j0-1 .. j0+T+1
+    extern __shared__ float zl[];  // [2][T+3][2][256]: hz, hx of plane k, rows j0-1 .. j0+T+1; [768] CE table
     const int T = g.T;
+    float *tab = zl + (size_t)2 * (T + 3) * 512;
+    if (CEM == kS4CeTable) {
+        for (int t = threadIdx.x; t < 768; t += blockDim.x) tab[t] = lut[t];
+        __syncthreads();
+    }
     unsigned b = blockIdx.x;
     const int xs = (int)(b % (unsigned)g.nxs);
     b /= (unsigned)g.nxs;
@@ -268,9 +289,9 @@ __global__ __launch_bounds__(1024) void k_s4_e_zm(S4ZGeom g, float dl, float *__
             e0 = ld4(ex + idk);
             e1 = ld4(ey + idk);
             e2 = ld4(ez + idk);
-            c0 = HC ? ld4(cx + idk) : splat4(ce0);
-            c1 = HC ? ld4(cy + idk) : splat4(ce0);
-            c2 = HC ? ld4(cz + idk) : splat4(ce0);
+            c0 = s4_ce<CEM>(cx, mx, tab, idk, ce0);
+            c1 = s4_ce<CEM>(cy, my, tab + 256, idk, ce0);
+            c2 = s4_ce<CEM>(cz, mz, tab + 512, idk, ce0);
         } else {
             hxk = rowok ? ld4(hx + idk) : z4;
         }
@@ -584,6 +605,23 @@ static int s4_check(int nx, int ny, int nz)
     return 0;
 }
 
+static int s4_zgeom(int nx, int ny, int nz, S4ZGeom &z, unsigned &nblocks)
+{
+    z.nx = nx;
+    z.ny = ny;
+    z.nz = nz;
+    z.Qx = nx / 4;
+    z.nxs = (z.Qx + 63) / 64;
+    z.T = kS4Rows;
+    z.nyt = (ny + z.T - 1) / z.T;
+    z.cz = kS4Planes;
+    z.nzc = (nz + z.cz - 1) / z.cz;
+    const unsigned long long nb = (unsigned long long)z.nxs * z.nyt * z.nzc;
+    if (nb >= (1ull << 31)) return fail(MIFDTD_ERANGE, "4th-order grid too large");
+    nblocks = (unsigned)nb;
+    return 0;
+}
+
 static int s4_core(bool h_half, int nx, int ny, int nz, float w, float *ex, float *ey, float *ez, float *hx, float *hy,
                    float *hz, const float *c0, const float *c1, const float *c2, float cconst, hipStream_t st)
 {
@@ -598,26 +636,21 @@ static int s4_core(bool h_half, int nx, int ny, int nz, float w, float *ex, floa
     if (hc && (!aligned16(c0) || !aligned16(c1) || !aligned16(c2)))
         return fail(MIFDTD_EALIGN, "coefficient pointer not 16-byte aligned");
     S4ZGeom z;
-    z.nx = nx;
-    z.ny = ny;
-    z.nz = nz;
-    z.Qx = nx / 4;
-    z.nxs = (z.Qx + 63) / 64;
-    z.T = kS4Rows;
-    z.nyt = (ny + z.T - 1) / z.T;
-    z.cz = kS4Planes;
-    z.nzc = (nz + z.cz - 1) / z.cz;
-    const unsigned long long nb = (unsigned long long)z.nxs * z.nyt * z.nzc;
-    if (nb >= (1ull << 31)) return fail(MIFDTD_ERANGE, "4th-order grid too large");
+    unsigned nb = 0;
+    if (int rc = s4_zgeom(nx, ny, nz, z, nb)) return rc;
     const unsigned threads = 64u * (unsigned)(z.T + 3);
     const size_t lds = sizeof(float) * 2 * (size_t)(z.T + 3) * 512;
     if (h_half) {
-        if (hc) k_s4_h_zm<true><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
-        else k_s4_h_zm<false><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        if (hc) k_s4_h_zm<true><<<nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        else k_s4_h_zm<false><<<nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
         MIFDTD_CHECK_LAUNCH("s4 update_h launch");
     } else {
-        if (hc) k_s4_e_zm<true><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
-        else k_s4_e_zm<false><<<(unsigned)nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst);
+        if (hc)
+            k_s4_e_zm<kS4CeArrays><<<nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst,
+                                                                       nullptr, nullptr, nullptr, nullptr);
+        else
+            k_s4_e_zm<kS4CeConst><<<nb, threads, lds, st>>>(z, w, ex, ey, ez, hx, hy, hz, c0, c1, c2, cconst,
+                                                                      nullptr, nullptr, nullptr, nullptr);
         MIFDTD_CHECK_LAUNCH("s4 update_e launch");
     }
     return 0;
@@ -712,6 +745,31 @@ int mifdtd_s4_update_e(int nx, int ny, int nz, float dl, float *ex, float *ey, f
 {
     return s4_core(false, nx, ny, nz, dl, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
                    const_cast<float *>(hz), cex, cey, cez, ce0, to_stream(stream));
+}
+
+int mifdtd_s4_update_e_lut(int nx, int ny, int nz, float dl, float *ex, float *ey, float *ez, const float *hx,
+                           const float *hy, const float *hz, const unsigned char *mat_x, const unsigned char *mat_y,
+                           const unsigned char *mat_z, const float *lut, mifdtd_stream_t stream)
+{
+    if (int rc = s4_check(nx, ny, nz)) return rc;
+    const float *ptrs[6] = {ex, ey, ez, hx, hy, hz};
+    for (const float *p : ptrs) {
+        if (!p) return fail(MIFDTD_ENULL, "field pointer is NULL");
+        if (!aligned16(p)) return fail(MIFDTD_EALIGN, "field pointer not 16-byte aligned");
+    }
+    if (!mat_x || !mat_y || !mat_z || !lut) return fail(MIFDTD_ENULL, "material index or table is NULL");
+    const unsigned char *mats[3] = {mat_x, mat_y, mat_z};
+    for (const unsigned char *m : mats)
+        if (reinterpret_cast<uintptr_t>(m) & 3u) return fail(MIFDTD_EALIGN, "material index array not 4-byte aligned");
+    S4ZGeom z;
+    unsigned nb = 0;
+    if (int rc = s4_zgeom(nx, ny, nz, z, nb)) return rc;
+    const unsigned threads = 64u * (unsigned)(z.T + 3);
+    const size_t lds = sizeof(float) * (2 * (size_t)(z.T + 3) * 512 + 768);
+    k_s4_e_zm<kS4CeTable><<<nb, threads, lds, to_stream(stream)>>>(z, dl, ex, ey, ez, hx, hy, hz, nullptr, nullptr,
+                                                                   nullptr, 0.f, mat_x, mat_y, mat_z, lut);
+    MIFDTD_CHECK_LAUNCH("s4 update_e (CE table) launch");
+    return 0;
 }
 
 int mifdtd_s4_cpml_e(int nx, int ny, int nz, const mifdtd_cpml_half *cp, float dl, float *ex, float *ey, float *ez,

@@ -84,6 +84,7 @@ class S4Sim:
         self.ehs = [torch.zeros(shape, dtype=torch.float32, device=self.device) for _ in range(6)]
         self.c0 = float(np.float32(self.S / 24.0))
         self.ces = None
+        self.ce_mat = self.ce_lut = None  # CE as uint8 material indices + table (set_ce)
         self.chs = None
         self.cl = [float(np.float32(c)) for c in S4_C]
         self.dl = [float(np.float32(d)) for d in S4_D]
@@ -134,7 +135,20 @@ class S4Sim:
         self.set_ce(ce, ce, ce)
 
     def set_ce(self, cex, cey, cez):
+        """CE arrays; when each component holds at most 256 distinct values the E stages
+        read them as uint8 material indices into an exact table (39 instead of 48 bytes
+        per cell, the same values; the CPML keeps the float arrays)."""
         self.ces = [self._fortran(c) for c in (cex, cey, cez)]
+        self.ce_mat = self.ce_lut = None
+        lut = torch.zeros((3, 256), dtype=torch.float32, device=self.device)
+        mats = []
+        for c, ce in enumerate(self.ces):
+            vals, inv = torch.unique(ce, return_inverse=True)
+            if vals.numel() > 256:
+                return
+            lut[c, :vals.numel()] = vals
+            mats.append(inv.to(torch.uint8).contiguous())
+        self.ce_mat, self.ce_lut = mats, lut
 
     def set_ch(self, chx, chy, chz):
         self.chs = [self._fortran(c) for c in (chx, chy, chz)]
@@ -187,7 +201,11 @@ class S4Sim:
         ce = [c.data_ptr() for c in self.ces] if self.ces is not None else [None] * 3
         if events is not None:
             events[0].record(self.stream)
-        _lib.call("mifdtd_s4_update_e", self.nx, self.ny, self.nz, self.dl[s], *e, *ce, self.c0, st)
+        if self.ce_mat is not None:
+            _lib.call("mifdtd_s4_update_e_lut", self.nx, self.ny, self.nz, self.dl[s], *e,
+                      *[m.data_ptr() for m in self.ce_mat], self.ce_lut.data_ptr(), st)
+        else:
+            _lib.call("mifdtd_s4_update_e", self.nx, self.ny, self.nz, self.dl[s], *e, *ce, self.c0, st)
         if events is not None:
             events[1].record(self.stream)
         if self.npml:

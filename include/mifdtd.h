@@ -424,6 +424,17 @@ int mifdtd_s4_update_e(int nx, int ny, int nz, float dl,
                        const float *cex, const float *cey, const float *cez, float ce0,
                        mifdtd_stream_t stream);
 
+/* mifdtd_s4_update_e with CE given as per-component uint8 material indices (Fortran
+ * (nx, ny, nz), 4-byte aligned) into lut (DEVICE, 3 x 256 float32: row c holds the
+ * distinct CE values of component c): the same results, 39 instead of 48 bytes per cell.
+ * No reference counterpart (a storage form of the CE arrays of S4/PyCUDA/100-base.py:84-86
+ * for media with at most 256 distinct values per component). */
+int mifdtd_s4_update_e_lut(int nx, int ny, int nz, float dl,
+                           float *ex, float *ey, float *ez,
+                           const float *hx, const float *hy, const float *hz,
+                           const unsigned char *mat_x, const unsigned char *mat_y,
+                           const unsigned char *mat_z, const float *lut, mifdtd_stream_t stream);
+
 /* CPML for the 4th-order path (no reference counterpart: the reference's 4th-order
  * path has no absorbing boundary).  Face, table and sign conventions of the Yee CPML
  * above (kuFDTD), driven by the 4th-order difference of the stage update:
