@@ -59,11 +59,11 @@ MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
                        ("kemp", 1024, 12, "uint8 index + table"): 53.24,
                        ("fused", 1024, 12, "uint8 index + table"): 51.83}
 # The 4th-order E stage (k_s4_e_zm, CE as uint8 material indices), same method, keyed by
-# (n, npml); filled from profiles/ when measured.
-MEASURED_S4_TRAFFIC_GB = {}
-# The batched step (both launches of one step, all 64 instances), same method, keyed by
-# the batch's size key; filled from profiles/ when measured.
-MEASURED_BATCH_TRAFFIC_GB = {}
+# (n, npml); profiles/k_s4_zmarch_symplectic_cpml10_512cube_hbm_counters.txt.
+MEASURED_S4_TRAFFIC_GB = {(512, 10): 5.81}
+# The batched step (both launches of one step, all instances), same method, keyed by
+# (instances, cube edges drawn from); profiles/k_batch_uneven_64cubes_kernel_trace.txt.
+MEASURED_BATCH_TRAFFIC_GB = {(64, "64,96,128,160,192"): 18.38}
 
 
 def parse():
@@ -316,7 +316,7 @@ def run_batch(args):
         "roofline": {"bound": "hbm", "kernel": bt.kernel_name, "basis": "60 B per cell + 32 B per CPML face-slab cell per step",
                      "achieved": round(step_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_BATCH_TRAFFIC_GB.get(bt.key()), "traffic_unit": "GB per step",
+                     "traffic": MEASURED_BATCH_TRAFFIC_GB.get((args.instances, args.sizes)), "traffic_unit": "GB per step",
                      "algorithmic_gb": round(step_bytes / 1e9, 3), "ms_per_step": round(ms, 4)},
         "wall_s": round(wall, 3), "device": torch.cuda.get_device_name(0),
     }
