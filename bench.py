@@ -75,6 +75,8 @@ def parse():
     p.add_argument("--npml", type=int, default=12)
     p.add_argument("--spheres", type=int, default=200)
     p.add_argument("--engine", default="fused", choices=("fused", "kemp", "slab"))
+    p.add_argument("--pairs", action="store_true",
+                   help="fused engine: advance steps in pairs (two-step interior sweep; slower on MI355X)")
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf", "drude"))
@@ -150,7 +152,7 @@ def time_kemp(sim, steps, warmup):
     return ms / steps, wall, kern_ms, "update_e", 48.0
 
 
-def time_fused(sim, steps, warmup):
+def time_fused(sim, steps, warmup, pairs=False):
     """Fused engine: events bracket the whole run (all launches of the K steps); a second
     run of K steps brackets, inside the C loop, every interior-sweep launch (the dominant
     kernel) with timing-only HIP events on the fields' stream.
@@ -158,7 +160,7 @@ def time_fused(sim, steps, warmup):
     cells / grid cells), so that bytes x cells / sweep time is the sweep's rate."""
     import torch
     from fdtd_accelerate_amd.fused import FusedStepper
-    st = FusedStepper(sim)
+    st = FusedStepper(sim, step_pairs=pairs)
     st.run(warmup, None)
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
@@ -177,6 +179,11 @@ def time_fused(sim, steps, warmup):
     # algorithmic bytes on the per-cell basis of the plain Yee step: read 6 fields and 3 CE,
     # write 6 fields = 60 B (the sweep box holds no CPML cell); the kernel itself reads CE
     # as 3 B of material indices, so it moves 51 B of those 60
+    if st.pairs:
+        # one launch advances its box two steps: state and CE read once, state written once
+        name = (f"k_fused2 (two-step interior sweep {sweep} cells, 60 B/cell per launch = 30 B/cell per step "
+                f"algorithmic, {st.bytes_per_cell:g} B/cell per launch read/written)")
+        return ms / steps, wall, kern_ms, name, 30.0 * sweep / cells
     name = f"k_fused (interior sweep {sweep} cells, 60 B/cell algorithmic, {st.bytes_per_cell:g} B/cell read/written)"
     return ms / steps, wall, kern_ms, name, 60.0 * sweep / cells
 
@@ -566,8 +573,10 @@ def main():
     if args.engine == "slab":
         ms, wall, kern_ms, kname, kbytes = time_slab(sim, args.steps, args.warmup, args.slab_planes)
     else:
-        runner = time_fused if args.engine == "fused" else time_kemp
-        ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
+        if args.engine == "fused":
+            ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup, args.pairs)
+        else:
+            ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
     n = args.n
     cells = n ** 3
     ce_storage = "uint8 index + table" if getattr(sim.fields, "ce_mat", None) is not None else "float32"
@@ -590,12 +599,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point source",
-                   "engine": args.engine, "cells": cells,
+                   "engine": args.engine + (" (step pairs)" if args.pairs else ""), "cells": cells,
                    "ce_storage": ce_storage},
         "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine, n, args.npml, ce_storage)), "traffic_unit": "GB per launch",
+                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine + ("-pairs" if args.pairs else ""), n, args.npml, ce_storage)), "traffic_unit": "GB per launch",
                      "bytes_per_cell": round(kbytes, 3), "avg_launch_ms": round(kern_ms, 4)},
         "wall_s": round(wall, 3),
         "device": torch.cuda.get_device_name(0),

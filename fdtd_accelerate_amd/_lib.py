@@ -45,7 +45,7 @@ class FusedDesc(ctypes.Structure):
         ("src_sv", _P * MAX_SRC), ("src_svs", _F * MAX_SRC), ("src_overwrite", _I * MAX_SRC),
         ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
         ("rows_per_thread", _I), ("planes_per_chunk", _I),
-        ("ce_mat", _P * 3), ("ce_lut", _P),
+        ("ce_mat", _P * 3), ("ce_lut", _P), ("e_mid", _P * 6),
     ]
 
 

@@ -39,6 +39,15 @@ def lds_fits(T, CX, table):
     return 4 * ((768 if table else 0) + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2) + 512 <= 160 * 1024
 
 
+def lds_fits_pairs(T, CX, table):
+    """Whether the two-step sweep's LDS (T+3 rows: H and E row exchanges of both levels,
+    4 x (T+3) x 2 x 256 floats; the k-1 carries of both levels per plane of the chunk,
+    (CX+3) x (T+3) x 4 floats; the CE table; ~0.7 KiB of source/probe tables) fits the
+    160 KiB of one gfx950 work-group."""
+    R = T + 3
+    return 4 * ((768 if table else 0) + 4 * R * 512 + 4 * (-(-R // 4)) + (CX + 3) * R * 4) + 1024 <= 160 * 1024
+
+
 def heuristic(nx, ny, nzp, npml):
     """T owned rows per block (y halo overhead 1/T; the block has T+1 waves), CX planes
     per chunk (seed-plane overhead 1/CX), keeping >= 1024 blocks.  T = 9 measured best
@@ -53,7 +62,30 @@ def heuristic(nx, ny, nzp, npml):
     return T, max(1, cx)
 
 
-def fused_tiles(nx, ny, nzp, npml):
+def heuristic_pairs(nx, ny, nzp, npml):
+    """(T, CX) of the two-step sweep: T = 9 owned rows (12 waves, the kernel's launch
+    bound; 9.95 ms per step at 1024^3 against 11.8 for T = 5), CX = 64 planes, halved
+    while fewer than 1024 blocks."""
+    T = min(9, max(1, ny // 8))
+    nrb = -(-ny // T)
+    cx = min(nx, 64)
+    while cx > 8 and nrb * (-(-nx // cx)) < 1024:
+        cx = -(-cx // 2)
+    while cx > 1 and not lds_fits_pairs(T, cx, True):
+        cx -= 1
+    return T, max(1, cx)
+
+
+def fused_pairs(nx, ny, nzp, npml):
+    """Whether the fused engine advances steps in pairs by default on this grid: never
+    (at 1024^3 with CPML 12 a pair takes 27.3 ms against 25.1 ms for two single steps on
+    MI355X; the two-step sweep halves the bytes but is issue-bound)."""
+    return False
+
+
+def fused_tiles(nx, ny, nzp, npml, pairs=False):
+    if pairs:
+        return heuristic_pairs(nx, ny, nzp, npml)
     t = _load().get((nx, ny, nzp, npml))
     return t if t is not None else heuristic(nx, ny, nzp, npml)
 

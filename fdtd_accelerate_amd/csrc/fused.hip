@@ -28,6 +28,9 @@
 //    is carried in LDS and the segment's last lane loads the k+1 element;
 //  * block ids are remapped so that consecutive row blocks of a chunk run on the same
 //    XCD, sharing L2 for the halo rows.
+// Step pairs (a third buffer set given): the two-step sweep k_fused2 below advances a box
+// two cells further from the CPML slabs by two steps per pass, between shell passes of
+// one step each (host code of mifdtd_fused_run_timed).
 // Nothing a launch reads from memory is written by another thread of the same launch.
 // Per-cell arithmetic order equals the in-place path (core, x, y, z CPML, sources)
 // under -ffp-contract=off: values are bit-identical to it; only the sign of zeros
@@ -470,17 +473,494 @@ __global__ __launch_bounds__(1024, 4) void k_fused(
     }
 }
 
+// ------------------------------------------------------------------ two-step sweep
+// Two time steps of the interior box in one pass (state n read from one buffer set, state
+// n+2 written to another; state n+1 of the box never leaves the chip).  Used with CPML,
+// between shell passes that advance the CPML shell one step at a time (see the host code):
+// the box keeps two planes / rows of CPML-free cells to the CPML slabs in x and y, so the
+// halo cells it recomputes at step n+1 need no CPML term, and its z bounds are the 128-byte
+// line bounds of the one-step box (at the box's z faces the step n+1 values of the cell
+// below come from the shell's step-n+1 buffer, those of step n+2 from the output).
+//
+// Block: T owned rows of an x chunk [i0, i1); wave w streams row j = j0-2+w:
+//   E1 (step n+1 E) on rows j0-2 .. j0+T, H1 on j0-1 .. j0+T, E2 (step n+2) on j0-1 .. j0+T-1,
+//   H2 on the owned rows j0 .. j0+T-1.
+// March over planes p = i0-2 .. i1 within a 256-float z segment; at plane p the block forms
+// E1(p) and H1(p), then E2(p-1) and H2(p-1) (level 2 lags one plane, needing H1(p) as its
+// i+1 neighbour).  Row neighbours go through LDS: H0(p+1) (hx, hz) for the row below,
+// published for the next plane, H1(p-1) (hx, hz) for the row below, E1(p) (ex, ez) and
+// E2(p-1) (ex, ez) for the row above, within the plane (two barriers per plane).  z
+// neighbours: DPP wave shifts (lane i takes lane i+1's or i-1's value); the
+// segment's first cell takes k-1 from the previous segment's last cell (LDS carry per plane
+// and row) or, on the box's first segment, from memory; the k+1 neighbour of the segment's
+// last cell is the first cell of the next segment, whose E1 and H1 (hx, hy) the wave forms
+// as an extra element from step-n data.  H1(p-1) is published at plane p (before the first
+// barrier) for the row below's E2(p-1); every exchange buffer is single: each is rewritten
+// only after a barrier that follows its last read.
+// Per-cell arithmetic is the in-place path's (core curl, then sources; no masks apply in
+// the box or its halo), so the values equal two in-place steps bit for bit.
+struct F2Lds {
+    float *tab, *x1, *x2, *x3, *x4, *xe, *cb;
+};
+
+__device__ __forceinline__ Q4 lds_q(const float *p, unsigned l4)
+{
+    const float4 t = *reinterpret_cast<const float4 *>(p + l4);
+    return Q4{{t.x, t.y, t.z, t.w}};
+}
+
+// Whole-wave lane shifts (DPP wave_shl:1 / wave_shr:1 of the 64-lane wave): lane i takes
+// lane i+1's (next) or lane i-1's (prev) value; the lane at the end takes 0.
+__device__ __forceinline__ float lane_next(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_at(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ void lds_put(float *p, unsigned l4, const Q4 &q)
+{
+    *reinterpret_cast<float4 *>(p + l4) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+}
+
+// E update of a quad from H: e += c * curl (KEMP order), k+1 neighbours of the last element
+// given (hy1 = hy[k+1], hx1 = hx[k+1]); hu = (hx, hz) of the row above, hN = H of the next plane.
+__device__ __forceinline__ void e_curl(const Q4 e[3], const Q4 c[3], const Q4 h[3], const Q4 &hux, const Q4 &huz,
+                                       const Q4 hN[3], float hy1, float hx1, Q4 o[3])
+{
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const float y1 = l < 3 ? h[1].v[l + 1] : hy1;
+        const float x1 = l < 3 ? h[0].v[l + 1] : hx1;
+        o[0].v[l] = e[0].v[l] + c[0].v[l] * ((huz.v[l] - h[2].v[l]) - (y1 - h[1].v[l]));
+        o[1].v[l] = e[1].v[l] + c[1].v[l] * ((x1 - h[0].v[l]) - (hN[2].v[l] - h[2].v[l]));
+        o[2].v[l] = e[2].v[l] + c[2].v[l] * ((hN[1].v[l] - h[1].v[l]) - (hux.v[l] - h[0].v[l]));
+    }
+}
+
+// H update of a quad from E: h -= 0.5 * curl; eb = (ex, ez) of the row below, ep = (ey, ez)
+// of the previous plane, xm / ym = ex / ey at k-1 of the first element.
+__device__ __forceinline__ void h_curl(const Q4 h[3], const Q4 e[3], const Q4 &exb, const Q4 &ezb, const Q4 &eyp,
+                                       const Q4 &ezp, float xm, float ym, Q4 o[3])
+{
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const float ymv = l > 0 ? e[1].v[l - 1] : ym;
+        const float xmv = l > 0 ? e[0].v[l - 1] : xm;
+        o[0].v[l] = h[0].v[l] - 0.5f * ((e[2].v[l] - ezb.v[l]) - (e[1].v[l] - ymv));
+        o[1].v[l] = h[1].v[l] - 0.5f * ((e[0].v[l] - xmv) - (e[2].v[l] - ezp.v[l]));
+        o[2].v[l] = h[2].v[l] - 0.5f * ((e[1].v[l] - eyp.v[l]) - (e[0].v[l] - exb.v[l]));
+    }
+}
+
+// Probe records of one quad (component c, values v) at step-table p.
+__device__ __forceinline__ void probe_quad(const FProbe &p, int c, int i, int j, int k0, const Q4 &v)
+{
+    for (int n = 0; n < p.n; ++n) {
+        if (p.comp[n] == c && p.i[n] == i && p.j[n] == j) {
+#pragma unroll
+            for (int l = 0; l < 4; ++l)
+                if (p.k[n] == k0 + l) p.out[n] = v.v[l];
+        }
+    }
+}
+
+// Raw CE of a quad as the sweep carries it from one plane to the next: nothing (the
+// constant 0.5), four floats, or four packed uint8 material indices into the LDS table.
+template <int CEM>
+struct CeRaw {
+    Q4 f[3];
+    unsigned t[3];
+};
+
+template <int CEM>
+__device__ __forceinline__ void ce_raw_load(CeRaw<CEM> &r, const float *cex, const float *cey, const float *cez,
+                                            const unsigned char *mx, const unsigned char *my, const unsigned char *mz,
+                                            size_t off)
+{
+    if constexpr (CEM == kCeArrays) {
+        r.f[0] = ld4(cex + off);
+        r.f[1] = ld4(cey + off);
+        r.f[2] = ld4(cez + off);
+    } else if constexpr (CEM == kCeTable) {
+        r.t[0] = *reinterpret_cast<const unsigned *>(mx + off);
+        r.t[1] = *reinterpret_cast<const unsigned *>(my + off);
+        r.t[2] = *reinterpret_cast<const unsigned *>(mz + off);
+    }
+}
+
+template <int CEM>
+__device__ __forceinline__ Q4 ce_raw_get(const CeRaw<CEM> &r, const float *tab, int c)
+{
+    if constexpr (CEM == kCeArrays) return r.f[c];
+    if constexpr (CEM == kCeTable) {
+        const float *t = tab + 256 * c;
+        const unsigned v = r.t[c];
+        return Q4{{t[v & 255u], t[(v >> 8) & 255u], t[(v >> 16) & 255u], t[v >> 24]}};
+    }
+    return splat4(0.5f);
+}
+
+template <int CEM>
+__global__ __launch_bounds__(768) void k_fused2(
+    FGeom g,
+    const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
+    const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
+    const float *__restrict__ mex, const float *__restrict__ mey,
+    float *__restrict__ oex, float *__restrict__ oey, float *__restrict__ oez,
+    float *__restrict__ ohx, float *__restrict__ ohy, float *__restrict__ ohz,
+    const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
+    const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my,
+    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src1, FSrc src2, FProbe prb1,
+    FProbe prb2)
+{
+    extern __shared__ float lds[];
+    __shared__ FSrc s_src[2];
+    __shared__ FProbe s_prb[2];
+    const int T = g.T;
+    const int R = T + 3;  // rows streamed per block
+    F2Lds L;
+    L.tab = lds;
+    L.x1 = lds + (CEM == kCeTable ? 768 : 0);  // [R][2][256] H0(p+1): hx, hz
+    L.x2 = L.x1 + (size_t)R * 512;             // [R][2][256] E1(p): ex, ez
+    L.x3 = L.x2 + (size_t)R * 512;             // [R][2][256] H1(p-1): hx, hz
+    L.x4 = L.x3 + (size_t)R * 512;             // [R][2][256] E2(p-1): ex, ez
+    L.xe = L.x4 + (size_t)R * 512;             // [R] ez of the extra element, step n+1
+    L.cb = L.xe + R;                           // [cx+3][R][4] z carries: E1 ex, ey; E2 ex, ey
+
+    int bid = blockIdx.x;
+    {
+        const int nb = g.nblocks, nb8 = nb & ~7;
+        if (bid < nb8) bid = (bid & 7) * (nb8 >> 3) + (bid >> 3);
+    }
+    const int rb = bid % g.nrb;
+    const int chunk = bid / g.nrb;
+    const int j0 = g.yb + rb * T;
+    const int nown = min(T, g.ye - j0);
+    const int i0 = g.xb + chunk * g.cx;
+    const int i1 = min(g.xe, i0 + g.cx);
+    const int ib = i0 - 2;
+    bool src_hit = false, prb_hit = false;
+    for (int n = 0; n < kMaxSrc; ++n)
+        src_hit |= n < src1.n && src1.i1[n] >= ib && src1.i0[n] <= i1 && src1.j1[n] >= j0 - 2 && src1.j0[n] <= j0 + nown;
+    for (int n = 0; n < kMaxProbe; ++n)
+        prb_hit |= n < prb1.n && prb1.i[n] >= i0 && prb1.i[n] < i1 && prb1.j[n] >= j0 && prb1.j[n] <= j0 + nown - 1;
+    if (threadIdx.x == 0) {
+        if (src_hit) {
+            s_src[0] = src1;
+            s_src[1] = src2;
+        }
+        if (prb_hit) {
+            s_prb[0] = prb1;
+            s_prb[1] = prb2;
+        }
+    }
+    if (CEM == kCeTable)
+        for (int t = threadIdx.x; t < 768; t += blockDim.x) L.tab[t] = lut[t];
+    __syncthreads();
+    const FSrc *sp1 = src_hit ? &s_src[0] : nullptr;
+    const FSrc *sp2 = src_hit ? &s_src[1] : nullptr;
+
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const unsigned l4 = 4u * (unsigned)lane;
+    const int j = j0 - 2 + w;
+    const bool live = w <= nown + 2;           // E1 rows j0-2 .. j0+nown
+    const bool h1row = live && w >= 1;         // H1 rows j0-1 .. j0+nown
+    const bool e2row = w >= 1 && w <= nown + 1;  // E2 rows j0-1 .. j0+nown-1
+    const bool own = w >= 2 && w <= nown + 1;  // H2 and stored rows j0 .. j0+nown-1
+    const bool top = w == nown + 2;            // loads H0 of the row above itself
+    const Q4 z4 = zero4();
+    const float *tab = L.tab;
+    const float *ctab[3] = {L.tab, L.tab + 256, L.tab + 512};
+    float *cbw = L.cb + (size_t)w * 4;         // this row's carries; plane q at + q*R*4
+
+    for (int m = 0; m < g.nseg; ++m) {
+        const int base = g.zb0 + m * 256;
+        const int lo = max(g.kb - base, 0), hi = min(g.ke - base, 256);
+        const bool act = (int)l4 >= lo && (int)l4 < hi;
+        const unsigned lc = (int)l4 < lo ? (unsigned)lo : (int)l4 >= hi ? (unsigned)(hi - 4) : l4;
+        const bool start_lane = (int)l4 == lo;
+        const bool end_lane = (int)l4 + 4 == hi;
+        const int endl = (hi >> 2) - 1;  // index of the end lane
+        const int kend = base + hi;      // the extra element: first cell after the segment's box part
+        const int k0 = base + (int)l4;
+        // state at plane ib (live rows)
+        Q4 hc[3], hux = z4, huz = z4;
+        float hcx[3] = {0.f, 0.f, 0.f};  // H0 of the extra element (hx, hy, hz)
+        hc[0] = hc[1] = hc[2] = z4;
+        if (live) {
+            const size_t r = row_base(g, ib, j, m);
+            hc[0] = ld4(hx + r + lc);
+            hc[1] = ld4(hy + r + lc);
+            hc[2] = ld4(hz + r + lc);
+            const size_t rx_ = ((size_t)ib * g.ny + j) * g.nzp + kend;
+            hcx[0] = hx[rx_];
+            hcx[1] = hy[rx_];
+            hcx[2] = hz[rx_];
+            const size_t ru = row_base(g, ib, j + 1, m);
+            hux = ld4(hx + ru + lc);
+            huz = ld4(hz + ru + lc);
+        }
+        Q4 e1q[3] = {z4, z4, z4};      // E1(p-1)
+        CeRaw<CEM> crq;                // CE of plane p-1
+        Q4 h1q[3] = {z4, z4, z4};      // H1(p-1)
+        float h1xq[2] = {0.f, 0.f};    // H1(p-1) of the extra element: hx, hy
+        Q4 e2p[2] = {z4, z4};          // E2(p-2): ey, ez
+        float ez1xq = 0.f;             // ez of the extra element at p-1
+        for (int p = ib; p <= i1; ++p) {
+            const int q = p - ib;
+            const bool do_h1 = h1row && p > ib;
+            const bool do_e2 = e2row && p >= ib + 2;
+            const bool do_h2 = own && p >= ib + 3;
+            // ---------------------------------------------- phase 1: loads, E1(p)
+            Q4 hN[3];
+            hN[0] = hN[1] = hN[2] = z4;
+            float hNx[3] = {0.f, 0.f, 0.f};
+            Q4 e1[3];
+            CeRaw<CEM> cr;
+            float e1x[3] = {0.f, 0.f, 0.f};
+            e1[0] = e1[1] = e1[2] = z4;
+            // k-1 values of the segment's first cell: previous segment's carries (read before
+            // this plane overwrites them) or, on the first segment, the memory values
+            float c1x = 0.f, c1y = 0.f, c2x = 0.f, c2y = 0.f;
+            if (m == 0) {
+                if (do_h1) {
+                    const size_t r = ((size_t)p * g.ny + j) * g.nzp + (g.kb - 1);
+                    c1x = mex[r];
+                    c1y = mey[r];
+                }
+                if (do_h2) {
+                    const size_t r = ((size_t)(p - 1) * g.ny + j) * g.nzp + (g.kb - 1);
+                    c2x = oex[r];
+                    c2y = oey[r];
+                }
+            } else if (start_lane) {
+                const float *cs = cbw + (size_t)q * R * 4;
+                c1x = cs[0];
+                c1y = cs[1];
+                if (q >= 1) {
+                    const float *cs2 = cbw + (size_t)(q - 1) * R * 4;
+                    c2x = cs2[2];
+                    c2y = cs2[3];
+                }
+            }
+            if (live) {
+                const size_t rn = row_base(g, p + 1, j, m);
+                hN[0] = ld4(hx + rn + lc);
+                hN[1] = ld4(hy + rn + lc);
+                hN[2] = ld4(hz + rn + lc);
+                const size_t cxn = ((size_t)(p + 1) * g.ny + j) * g.nzp + kend;
+                hNx[0] = hx[cxn];
+                hNx[1] = hy[cxn];
+                hNx[2] = hz[cxn];
+                const size_t r = row_base(g, p, j, m);
+                Q4 e0[3];
+                e0[0] = ld4(ex + r + lc);
+                e0[1] = ld4(ey + r + lc);
+                e0[2] = ld4(ez + r + lc);
+                ce_raw_load<CEM>(cr, cex, cey, cez, mx, my, mz, r + lc);
+                // the extra element's step-n data
+                const size_t cx0 = ((size_t)p * g.ny + j) * g.nzp + kend;
+                float e0x[3] = {ex[cx0], ey[cx0], ez[cx0]};
+                float cxx[3];
+                if (CEM == kCeConst) {
+                    cxx[0] = cxx[1] = cxx[2] = 0.5f;
+                } else if (CEM == kCeArrays) {
+                    cxx[0] = cex[cx0];
+                    cxx[1] = cey[cx0];
+                    cxx[2] = cez[cx0];
+                } else {
+                    cxx[0] = ctab[0][mx[cx0]];
+                    cxx[1] = ctab[1][my[cx0]];
+                    cxx[2] = ctab[2][mz[cx0]];
+                }
+                const float hy_k1 = hy[cx0 + 1], hx_k1 = hx[cx0 + 1];
+                const float hz_u = hz[cx0 + g.nzp], hx_u = hx[cx0 + g.nzp];
+                if (p > ib) {  // H0(p, j+1): from the row above (LDS) or memory (top row)
+                    if (!top) {
+                        const float *xb = L.x1 + (size_t)((w + 1) * 2) * 256;
+                        hux = lds_q(xb, l4);
+                        huz = lds_q(xb + 256, l4);
+                    } else {
+                        const size_t ru = row_base(g, p, j + 1, m);
+                        hux = ld4(hx + ru + lc);
+                        huz = ld4(hz + ru + lc);
+                    }
+                }
+                float hy1 = lane_next(hc[1].v[0]), hx1 = lane_next(hc[0].v[0]);
+                if (end_lane) {
+                    hy1 = hcx[1];
+                    hx1 = hcx[0];
+                }
+                const Q4 ce[3] = {ce_raw_get<CEM>(cr, tab, 0), ce_raw_get<CEM>(cr, tab, 1), ce_raw_get<CEM>(cr, tab, 2)};
+                e_curl(e0, ce, hc, hux, huz, hN, hy1, hx1, e1);
+                e1x[0] = e0x[0] + cxx[0] * ((hz_u - hcx[2]) - (hy_k1 - hcx[1]));
+                e1x[1] = e0x[1] + cxx[1] * ((hx_k1 - hcx[0]) - (hNx[2] - hcx[2]));
+                e1x[2] = e0x[2] + cxx[2] * ((hNx[1] - hcx[1]) - (hx_u - hcx[0]));
+                if (sp1) {
+                    apply_src(*sp1, 0, p, j, k0, e1[0]);
+                    apply_src(*sp1, 1, p, j, k0, e1[1]);
+                    apply_src(*sp1, 2, p, j, k0, e1[2]);
+                    Q4 t[3] = {splat4(e1x[0]), splat4(e1x[1]), splat4(e1x[2])};
+                    for (int c = 0; c < 3; ++c) {
+                        apply_src(*sp1, c, p, j, kend, t[c]);
+                        e1x[c] = t[c].v[0];
+                    }
+                }
+                float *xs = L.x2 + (size_t)(w * 2) * 256;
+                lds_put(xs, l4, e1[0]);
+                lds_put(xs + 256, l4, e1[2]);
+                if (lane == 0) L.xe[w] = e1x[2];
+                if (end_lane) {
+                    float *cs = cbw + (size_t)q * R * 4;
+                    cs[0] = e1[0].v[3];
+                    cs[1] = e1[1].v[3];
+                }
+            }
+            if (h1row) {  // H1(p-1) (hx, hz) for the row below's E2(p-1), this plane
+                float *xs = L.x3 + (size_t)(w * 2) * 256;
+                lds_put(xs, l4, h1q[0]);
+                lds_put(xs + 256, l4, h1q[2]);
+            }
+            __syncthreads();
+            // ---------------------------------------------- phase 2: H1(p)
+            Q4 h1[3];
+            h1[0] = h1[1] = h1[2] = z4;
+            float h1x[2] = {0.f, 0.f};
+            if (do_h1) {
+                const float *xb = L.x2 + (size_t)((w - 1) * 2) * 256;
+                const Q4 exb = lds_q(xb, l4), ezb = lds_q(xb + 256, l4);
+                float xm = lane_prev(e1[0].v[3]), ym = lane_prev(e1[1].v[3]);
+                if (start_lane) {
+                    xm = c1x;
+                    ym = c1y;
+                }
+                const Q4 e1p[2] = {e1q[1], e1q[2]};
+                h_curl(hc, e1, exb, ezb, e1p[0], e1p[1], xm, ym, h1);
+                const float ey_end = lane_at(e1[1].v[3], endl), ex_end = lane_at(e1[0].v[3], endl);
+                h1x[0] = hcx[0] - 0.5f * ((e1x[2] - L.xe[w - 1]) - (e1x[1] - ey_end));
+                h1x[1] = hcx[1] - 0.5f * ((e1x[0] - ex_end) - (e1x[2] - ez1xq));
+                if (sp1) {
+                    apply_src(*sp1, 3, p, j, k0, h1[0]);
+                    apply_src(*sp1, 4, p, j, k0, h1[1]);
+                    apply_src(*sp1, 5, p, j, k0, h1[2]);
+                    Q4 t0 = splat4(h1x[0]), t1 = splat4(h1x[1]);
+                    apply_src(*sp1, 3, p, j, kend, t0);
+                    apply_src(*sp1, 4, p, j, kend, t1);
+                    h1x[0] = t0.v[0];
+                    h1x[1] = t1.v[0];
+                }
+                if (prb_hit && own && act && p >= i0 && p < i1) {
+                    for (int c = 0; c < 3; ++c) probe_quad(s_prb[0], c, p, j, k0, e1[c]);
+                    for (int c = 0; c < 3; ++c) probe_quad(s_prb[0], 3 + c, p, j, k0, h1[c]);
+                }
+            }
+            if (live) {  // H0(p+1) for the row below (read at plane p+1, after this plane's second barrier)
+                float *xs = L.x1 + (size_t)(w * 2) * 256;
+                lds_put(xs, l4, hN[0]);
+                lds_put(xs + 256, l4, hN[2]);
+            }
+            // ---------------------------------------------- phase 3: E2(p-1)
+            Q4 e2[3];
+            e2[0] = e2[1] = e2[2] = z4;
+            if (do_e2) {
+                const float *xu = L.x3 + (size_t)((w + 1) * 2) * 256;
+                const Q4 h1ux = lds_q(xu, l4), h1uz = lds_q(xu + 256, l4);
+                float hy1 = lane_next(h1q[1].v[0]), hx1 = lane_next(h1q[0].v[0]);
+                if (end_lane) {
+                    hy1 = h1xq[1];
+                    hx1 = h1xq[0];
+                }
+                const Q4 cq[3] = {ce_raw_get<CEM>(crq, tab, 0), ce_raw_get<CEM>(crq, tab, 1),
+                                  ce_raw_get<CEM>(crq, tab, 2)};
+                e_curl(e1q, cq, h1q, h1ux, h1uz, h1, hy1, hx1, e2);
+                if (sp2) {
+                    apply_src(*sp2, 0, p - 1, j, k0, e2[0]);
+                    apply_src(*sp2, 1, p - 1, j, k0, e2[1]);
+                    apply_src(*sp2, 2, p - 1, j, k0, e2[2]);
+                }
+                float *xs = L.x4 + (size_t)(w * 2) * 256;
+                lds_put(xs, l4, e2[0]);
+                lds_put(xs + 256, l4, e2[2]);
+                if (end_lane) {
+                    float *cs = cbw + (size_t)(q - 1) * R * 4;
+                    cs[2] = e2[0].v[3];
+                    cs[3] = e2[1].v[3];
+                }
+            }
+            __syncthreads();
+            // ---------------------------------------------- phase 4: H2(p-1), stores
+            if (do_h2) {
+                const float *xb = L.x4 + (size_t)((w - 1) * 2) * 256;
+                const Q4 exb = lds_q(xb, l4), ezb = lds_q(xb + 256, l4);
+                float xm = lane_prev(e2[0].v[3]), ym = lane_prev(e2[1].v[3]);
+                if (start_lane) {
+                    xm = c2x;
+                    ym = c2y;
+                }
+                Q4 h2[3];
+                h_curl(h1q, e2, exb, ezb, e2p[0], e2p[1], xm, ym, h2);
+                if (sp2) {
+                    apply_src(*sp2, 3, p - 1, j, k0, h2[0]);
+                    apply_src(*sp2, 4, p - 1, j, k0, h2[1]);
+                    apply_src(*sp2, 5, p - 1, j, k0, h2[2]);
+                }
+                if (act) {
+                    const size_t r = row_base(g, p - 1, j, m) + l4;
+                    st4(oex + r, e2[0]);
+                    st4(oey + r, e2[1]);
+                    st4(oez + r, e2[2]);
+                    st4(ohx + r, h2[0]);
+                    st4(ohy + r, h2[1]);
+                    st4(ohz + r, h2[2]);
+                    if (prb_hit) {
+                        for (int c = 0; c < 3; ++c) probe_quad(s_prb[1], c, p - 1, j, k0, e2[c]);
+                        for (int c = 0; c < 3; ++c) probe_quad(s_prb[1], 3 + c, p - 1, j, k0, h2[c]);
+                    }
+                }
+            }
+            // shift the plane registers
+            e1q[0] = e1[0];
+            e1q[1] = e1[1];
+            e1q[2] = e1[2];
+            crq = cr;
+            h1q[0] = h1[0];
+            h1q[1] = h1[1];
+            h1q[2] = h1[2];
+            h1xq[0] = h1x[0];
+            h1xq[1] = h1x[1];
+            e2p[0] = e2[1];
+            e2p[1] = e2[2];
+            ez1xq = e1x[2];
+            hc[0] = hN[0];
+            hc[1] = hN[1];
+            hc[2] = hN[2];
+            hcx[0] = hNx[0];
+            hcx[1] = hNx[1];
+            hcx[2] = hNx[2];
+        }
+    }
+}
+
 // CPML shell: every cell outside the interior box, as up to six boxes (x low/high: all
 // rows; y low/high: the box's planes; z low/high: the box's planes and rows), each
 // (planes [i0,i1), rows [j0,j1), quads [q0,q1)) with a thread per quad; start[b] is the
 // first thread of box b.  Quads run fastest, then the box's shorter one of planes and
 // rows, so that the i+1 / i-1 and j+1 / j-1 neighbour rows a quad reads are read by
 // nearby threads too (L2 hits).
+constexpr int kMaxShellBoxes = 12;
 struct ShellBoxes {
     int n;
-    int i0[6], i1[6], j0[6], j1[6], q0[6], q1[6];
-    int i_inner[6];  // planes before rows in the thread order
-    unsigned start[7];
+    int i0[kMaxShellBoxes], i1[kMaxShellBoxes], j0[kMaxShellBoxes], j1[kMaxShellBoxes], q0[kMaxShellBoxes],
+        q1[kMaxShellBoxes];
+    int i_inner[kMaxShellBoxes];  // planes before rows in the thread order
+    unsigned start[kMaxShellBoxes + 1];
 };
 
 __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int &i, int &j, int &k0)
@@ -537,6 +1017,50 @@ __global__ __launch_bounds__(256) void k_shell_h(SGeom g, ShellBoxes sb, unsigne
     int i, j, k0;
     shell_quad(sb, f, i, j, k0);
     h_quad_at<true>(g, i, j, k0, oex, oey, oez, hx, hy, hz, ohx, ohy, ohz, cp, src, prb);
+}
+
+// Appends the box (planes [i0,i1), rows [j0,j1), quads [q0,q1)) to sb unless it is empty.
+inline void add_box(ShellBoxes &sb, unsigned long long &total, int i0, int i1, int j0, int j1, int q0, int q1)
+{
+    if (i1 <= i0 || j1 <= j0 || q1 <= q0) return;
+    const int b = sb.n++;
+    sb.i0[b] = i0; sb.i1[b] = i1; sb.j0[b] = j0; sb.j1[b] = j1; sb.q0[b] = q0; sb.q1[b] = q1;
+    sb.i_inner[b] = (i1 - i0) < (j1 - j0);
+    sb.start[b] = (unsigned)total;
+    total += (unsigned long long)(i1 - i0) * (j1 - j0) * (q1 - q0);
+}
+
+// The cells outside g's box as six boxes: x low/high slabs (all rows), y low/high (the
+// box's planes), z low/high (the box's planes and rows).
+inline void add_shell(ShellBoxes &sb, unsigned long long &total, const FGeom &g, int Q)
+{
+    add_box(sb, total, 0, g.xb, 0, g.ny, 0, Q);
+    add_box(sb, total, g.xe, g.nx, 0, g.ny, 0, Q);
+    add_box(sb, total, g.xb, g.xe, 0, g.yb, 0, Q);
+    add_box(sb, total, g.xb, g.xe, g.ye, g.ny, 0, Q);
+    add_box(sb, total, g.xb, g.xe, g.yb, g.ye, 0, g.kb / 4);
+    add_box(sb, total, g.xb, g.xe, g.yb, g.ye, g.ke / 4, Q);
+}
+
+// The box's low fringe (plane xb, row yb, quad kb/4: the box cells whose step n+1 H the
+// shell's step n+2 E update reads) as three disjoint boxes.
+inline void add_low_fringe(ShellBoxes &sb, unsigned long long &total, const FGeom &g)
+{
+    const int qb = g.kb / 4, qe = g.ke / 4;
+    add_box(sb, total, g.xb, g.xb + 1, g.yb, g.ye, qb, qe);
+    add_box(sb, total, g.xb + 1, g.xe, g.yb, g.yb + 1, qb, qe);
+    add_box(sb, total, g.xb + 1, g.xe, g.yb + 1, g.ye, qb, qb + 1);
+}
+
+// The box's high fringe (plane xe-1, row ye-1, quad ke/4-1: the box cells whose step n+1
+// E the shell's step n+1 H update reads) less the low fringe, as three disjoint boxes
+// (needs at least two planes and rows and two quads).
+inline void add_high_fringe(ShellBoxes &sb, unsigned long long &total, const FGeom &g)
+{
+    const int qb = g.kb / 4, qe = g.ke / 4;
+    add_box(sb, total, g.xe - 1, g.xe, g.yb + 1, g.ye, qb + 1, qe);
+    add_box(sb, total, g.xb + 1, g.xe - 1, g.ye - 1, g.ye, qb + 1, qe);
+    add_box(sb, total, g.xb + 1, g.xe - 1, g.yb + 1, g.ye - 1, qe - 1, qe);
 }
 
 }  // namespace mifdtd
@@ -638,24 +1162,57 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     // shell boxes (only with CPML)
     ShellBoxes sb{};
     unsigned long long total = 0;
+    const int Q = nzp / 4;
     if (pml) {
-        const int Q = nzp / 4;
-        auto add = [&](int i0, int i1, int j0, int j1, int q0, int q1) {
-            if (i1 <= i0 || j1 <= j0 || q1 <= q0) return;
-            const int b = sb.n++;
-            sb.i0[b] = i0; sb.i1[b] = i1; sb.j0[b] = j0; sb.j1[b] = j1; sb.q0[b] = q0; sb.q1[b] = q1;
-            sb.i_inner[b] = (i1 - i0) < (j1 - j0);
-            sb.start[b] = (unsigned)total;
-            total += (unsigned long long)(i1 - i0) * (j1 - j0) * (q1 - q0);
-        };
-        add(0, g.xb, 0, ny, 0, Q);
-        add(g.xe, nx, 0, ny, 0, Q);
-        add(g.xb, g.xe, 0, g.yb, 0, Q);
-        add(g.xb, g.xe, g.ye, ny, 0, Q);
-        add(g.xb, g.xe, g.yb, g.ye, 0, g.kb / 4);
-        add(g.xb, g.xe, g.yb, g.ye, g.ke / 4, Q);
+        add_shell(sb, total, g, Q);
         if (total >= (1ull << 32) - 256) return fail(MIFDTD_ERANGE, "CPML shell too large");
         sb.start[sb.n] = (unsigned)total;
+    }
+
+    // step pairs (third buffer set given, CPML on): the two-step sweep's box keeps two
+    // planes / rows to the CPML slabs in x and y (the step n+1 values it forms on its
+    // halo need no CPML term) and the one-step box's z bounds less the cells next to the
+    // z slabs; per pair, shell launches on: [0] the shell, [1] the shell and the box's low
+    // fringe, [2] the shell and both fringes.
+    FGeom g2{};
+    ShellBoxes sb2[3]{};
+    unsigned long long tot2[3] = {0, 0, 0};
+    size_t lds2 = 0;
+    bool pairs = false;
+    const int R2 = T + 3;
+    if (pml && d->e_mid[0]) {
+        for (int q = 0; q < 6; q++) {
+            if (!d->e_mid[q]) return fail(MIFDTD_ENULL, "e_mid buffer is NULL");
+            if (!aligned16(d->e_mid[q])) return fail(MIFDTD_EALIGN, "e_mid buffer not 16-byte aligned");
+        }
+        if (T > 9) return fail(MIFDTD_EINVAL, "rows_per_thread must be in [1,9] with step pairs (rows+3 waves)");
+        const int np_ = d->npml;
+        g2 = g;
+        g2.xb = np_ + 2; g2.xe = nx - 2 - np_;
+        g2.yb = np_ + 2; g2.ye = ny - 2 - np_;
+        g2.kb = 32 * ((np_ + 2 + 31) / 32); g2.ke = 32 * ((nz - 2 - np_) / 32);
+        pairs = g2.xe - g2.xb >= 4 && g2.ye - g2.yb >= 4 && g2.ke > g2.kb;  // else single steps only
+        if (pairs) {
+            g2.nrb = (g2.ye - g2.yb + T - 1) / T;
+            g2.nchunks = (g2.xe - g2.xb + CX - 1) / CX;
+            g2.zb0 = 256 * (g2.kb / 256);
+            g2.nseg = (g2.ke - g2.zb0 + 255) / 256;
+            const long long nb2 = (long long)g2.nrb * g2.nchunks;
+            if (nb2 >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
+            g2.nblocks = (int)nb2;
+            lds2 = sizeof(float) * ((cem == kCeTable ? 768 : 0) + (size_t)4 * R2 * 512 + (size_t)4 * ((R2 + 3) / 4) +
+                                    (size_t)(CX + 3) * R2 * 4);
+            if (lds2 + 2 * sizeof(FSrc) + 2 * sizeof(FProbe) > kMaxLds)
+                return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS with step pairs (> %zu)",
+                            T, CX, lds2, kMaxLds);
+            for (int v = 0; v < 3; v++) {
+                add_shell(sb2[v], tot2[v], g2, Q);
+                if (v >= 1) add_low_fringe(sb2[v], tot2[v], g2);
+                if (v == 2) add_high_fringe(sb2[v], tot2[v], g2);
+                if (tot2[v] >= (1ull << 32) - 256) return fail(MIFDTD_ERANGE, "CPML shell too large");
+                sb2[v].start[sb2[v].n] = (unsigned)tot2[v];
+            }
+        }
     }
     SGeom sg{};
     sg.nx = nx; sg.ny = ny; sg.nzp = nzp; sg.nz = nz;
@@ -691,12 +1248,11 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     }
 
     const float *in[6];
-    float *out[6];
-    for (int q = 0; q < 6; q++) { in[q] = d->e[q]; out[q] = d->e_alt[q]; }
+    float *out[6], *mid[6];
+    for (int q = 0; q < 6; q++) { in[q] = d->e[q]; out[q] = d->e_alt[q]; mid[q] = d->e_mid[q]; }
     float *pin = d->psiE, *pout = d->psiE_alt;
 
     hipStream_t st = to_stream(stream);
-    const unsigned nthr = 64u * (unsigned)(T + 1);
     // optional HIP events around every sweep launch (the dominant kernel of the step),
     // destroyed on every return path
     struct Marks {
@@ -714,52 +1270,123 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             marks.push_back(e);
         }
     }
-    const unsigned shell_blocks = (unsigned)((total + 255) / 256);
-    for (int s = 0; s < nsteps; s++) {
-        for (int n = 0; n < src.n; n++) ssrc.tval[n] = src.tval[n] = tvals[(size_t)s * src.n + n];
-        sprb.out = prb.out = d->nprobe ? probe_series + (size_t)s * d->nprobe : nullptr;
-        if (pml) {
-            scp.pe = pin;
-            scp.pe_out = pout;
-#define MIFDTD_SHELL_E_LAUNCH(CEV)                                                                               \
-    k_shell_e<CEV><<<shell_blocks, 256, 0, st>>>(sg, sb, (unsigned)total, in[0], in[1], in[2], out[0], out[1],   \
-                                                 out[2], in[3], in[4], in[5], d->ce[0], d->ce[1], d->ce[2],      \
-                                                 d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, scp, ssrc, \
-                                                 sprb)
-            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable);
-            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays);
-            else MIFDTD_SHELL_E_LAUNCH(kCeConst);
+    int nsweeps = 0;  // sweep launches so far (event pairs used)
+
+    // the sources' time values and probe rows of step s
+    auto step_tables = [&](int s, FSrc &fs, SSrc &ss, FProbe &fp, SProbe &sp) {
+        fs = src;
+        ss = ssrc;
+        fp = prb;
+        sp = sprb;
+        for (int n = 0; n < src.n; n++) ss.tval[n] = fs.tval[n] = tvals[(size_t)s * src.n + n];
+        sp.out = fp.out = d->nprobe ? probe_series + (size_t)s * d->nprobe : nullptr;
+    };
+    // shell E pass over boxes b: E from ei (H from hi) to eo, psi E from pe to pe_out
+    auto shell_e = [&](const ShellBoxes &b, unsigned long long tot, const float *const *ei, const float *const *hi,
+                       float *const *eo, float *pe, float *pe_out, const SSrc &ss, const SProbe &sp) -> int {
+        SCpml c = scp;
+        c.pe = pe;
+        c.pe_out = pe_out;
+        const unsigned nbk = (unsigned)((tot + 255) / 256);
+        if (!nbk) return 0;
+#define MIFDTD_SHELL_E_LAUNCH(CEV)                                                                                    \
+    k_shell_e<CEV><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, ei[0], ei[1], ei[2], eo[0], eo[1], eo[2], hi[3], hi[4], \
+                                        hi[5], d->ce[0], d->ce[1], d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], \
+                                        d->ce_lut, c, ss, sp)
+        if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable);
+        else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays);
+        else MIFDTD_SHELL_E_LAUNCH(kCeConst);
 #undef MIFDTD_SHELL_E_LAUNCH
-            MIFDTD_CHECK_LAUNCH("CPML shell E launch");
-        }
-        if (sweep) {
-            if (sweep_ms) (void)hipEventRecord(marks[2 * (size_t)s], st);
-#define MIFDTD_FUSED_LAUNCH(CEV)                                                                                 \
-    k_fused<CEV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
-                                                     out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
-                                                     d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, src, prb)
-            if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable);
-            else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays);
-            else MIFDTD_FUSED_LAUNCH(kCeConst);
-#undef MIFDTD_FUSED_LAUNCH
-            if (sweep_ms) (void)hipEventRecord(marks[2 * (size_t)s + 1], st);
-            MIFDTD_CHECK_LAUNCH("fused sweep launch");
-        }
-        if (pml) {
-            k_shell_h<<<shell_blocks, 256, 0, st>>>(sg, sb, (unsigned)total, out[0], out[1], out[2], in[3], in[4], in[5],
-                                                    out[3], out[4], out[5], scp, ssrc, sprb);
-            MIFDTD_CHECK_LAUNCH("CPML shell H launch");
-        }
+        MIFDTD_CHECK_LAUNCH("CPML shell E launch");
+        return 0;
+    };
+    // shell H pass over boxes b: H from hi to ho with the new E of en (psi H in place)
+    auto shell_h = [&](const ShellBoxes &b, unsigned long long tot, const float *const *en, const float *const *hi,
+                       float *const *ho, const SSrc &ss, const SProbe &sp) -> int {
+        const unsigned nbk = (unsigned)((tot + 255) / 256);
+        if (!nbk) return 0;
+        k_shell_h<<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, en[0], en[1], en[2], hi[3], hi[4], hi[5], ho[3], ho[4],
+                                       ho[5], scp, ss, sp);
+        MIFDTD_CHECK_LAUNCH("CPML shell H launch");
+        return 0;
+    };
+    auto mark = [&](int q) {
+        if (sweep_ms) (void)hipEventRecord(marks[2 * (size_t)nsweeps + q], st);
+    };
+    auto swap_sets = [&]() {
         for (int q = 0; q < 6; q++) { const float *x = in[q]; in[q] = out[q]; out[q] = const_cast<float *>(x); }
         float *tp = pin;
         pin = pout;
         pout = tp;
+    };
+    const unsigned nthr = 64u * (unsigned)(T + 1);
+    const unsigned nthr2 = 64u * (unsigned)R2;
+    int swaps = 0;
+    int s = 0;
+    // step pairs: shell E of step s (shell and both fringes) and shell H of step s (shell
+    // and low fringe) into the middle set, shell E of step s+1, the two-step sweep, then
+    // shell H of step s+1; psi E goes pin -> pout (step s) and is updated in place in pout
+    // (step s+1).
+    for (; pairs && s + 2 <= nsteps; s += 2) {
+        FSrc fs1, fs2;
+        SSrc ss1, ss2;
+        FProbe fp1, fp2;
+        SProbe sp1, sp2;
+        step_tables(s, fs1, ss1, fp1, sp1);
+        step_tables(s + 1, fs2, ss2, fp2, sp2);
+        if (int rc = shell_e(sb2[2], tot2[2], in, in, mid, pin, pout, ss1, sp1)) return rc;
+        if (int rc = shell_h(sb2[1], tot2[1], mid, in, mid, ss1, sp1)) return rc;
+        if (int rc = shell_e(sb2[0], tot2[0], mid, mid, out, pout, pout, ss2, sp2)) return rc;
+        mark(0);
+#define MIFDTD_FUSED2_LAUNCH(CEV)                                                                                   \
+    k_fused2<CEV><<<g2.nblocks, nthr2, lds2, st>>>(g2, in[0], in[1], in[2], in[3], in[4], in[5], mid[0], mid[1],    \
+                                                   out[0], out[1], out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], \
+                                                   d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs1, \
+                                                   fs2, fp1, fp2)
+        if (cem == kCeTable) MIFDTD_FUSED2_LAUNCH(kCeTable);
+        else if (cem == kCeArrays) MIFDTD_FUSED2_LAUNCH(kCeArrays);
+        else MIFDTD_FUSED2_LAUNCH(kCeConst);
+#undef MIFDTD_FUSED2_LAUNCH
+        mark(1);
+        ++nsweeps;
+        MIFDTD_CHECK_LAUNCH("two-step sweep launch");
+        if (int rc = shell_h(sb2[0], tot2[0], out, mid, out, ss2, sp2)) return rc;
+        swap_sets();
+        ++swaps;
     }
-    if (parity) *parity = nsteps & 1;
+    // single steps: shell E, the one-step sweep, shell H
+    for (; s < nsteps; s++) {
+        FSrc fs;
+        SSrc ss;
+        FProbe fp;
+        SProbe sp;
+        step_tables(s, fs, ss, fp, sp);
+        if (pml)
+            if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp)) return rc;
+        if (sweep) {
+            mark(0);
+#define MIFDTD_FUSED_LAUNCH(CEV)                                                                                 \
+    k_fused<CEV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
+                                                     out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
+                                                     d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp)
+            if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable);
+            else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays);
+            else MIFDTD_FUSED_LAUNCH(kCeConst);
+#undef MIFDTD_FUSED_LAUNCH
+            mark(1);
+            ++nsweeps;
+            MIFDTD_CHECK_LAUNCH("fused sweep launch");
+        }
+        if (pml)
+            if (int rc = shell_h(sb, total, out, in, out, ss, sp)) return rc;
+        swap_sets();
+        ++swaps;
+    }
+    if (parity) *parity = swaps & 1;
     if (sweep_ms) {
         float total_ms = 0.f;
         hipError_t err = hipStreamSynchronize(st);
-        for (int q = 0; sweep && err == hipSuccess && q < nsteps; q++) {
+        for (int q = 0; err == hipSuccess && q < nsweeps; q++) {
             float t = 0.f;
             err = hipEventElapsedTime(&t, marks[2 * (size_t)q], marks[2 * (size_t)q + 1]);
             total_ms += t;

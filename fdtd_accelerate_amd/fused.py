@@ -20,7 +20,7 @@ import torch
 
 from . import _lib
 from . import fields as _fields
-from .autotune import fused_tiles, lds_fits, slab_planes
+from .autotune import fused_pairs, fused_tiles, lds_fits, lds_fits_pairs, slab_planes
 
 _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 
@@ -72,17 +72,27 @@ def _ping_pong_sets(fields):
 class FusedStepper:
     kernel_name = "k_fused"
 
-    def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None):
+    def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None):
         f = sim.fields
         _check_whole_step(sim, "fused")
         cp = sim.cpml
         self.sim = sim
+        npml = cp.npml if cp else 0
+        # step pairs: two steps per interior sweep (needs CPML; a third field set holds the
+        # first step's shell values)
+        self.pairs = bool(cp is not None and (step_pairs if step_pairs is not None
+                                              else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
         f.ehs, self.alt = _ping_pong_sets(f)
-        T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, cp.npml if cp else 0)
+        T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
         self.T = rows_per_thread or T
         self.CX = planes_per_chunk or CX
-        if not lds_fits(self.T, self.CX, f.ce_mat is not None):
-            raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} need more than 64 KiB of LDS")
+        table = f.ce_mat is not None
+        if not lds_fits(self.T, self.CX, table):
+            raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} need more than 160 KiB of LDS")
+        if self.pairs and not (self.T <= 9 and lds_fits_pairs(self.T, self.CX, table)):
+            raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} do not fit the two-step sweep "
+                             "(at most 9 rows, 160 KiB of LDS)")
+        self.mid = [torch.zeros_like(t) for t in f.ehs] if self.pairs else None
         if cp is not None:
             cp.alt_psiE()
 
@@ -130,6 +140,9 @@ class FusedStepper:
             d.probe_ijk[n][0], d.probe_ijk[n][1], d.probe_ijk[n][2] = i, j, k
         d.rows_per_thread = self.T
         d.planes_per_chunk = self.CX
+        if self.mid:
+            for c in range(6):
+                d.e_mid[c] = self.mid[c].data_ptr()
         return d
 
     def _tvals(self, nsteps):
@@ -163,15 +176,20 @@ class FusedStepper:
                 sim.cpml.swap_psiE()
         return sweep_ms.value
 
-    def sweep_box(self):
+    def sweep_box(self, pairs=None):
         """(xb, xe, yb, ye, kb, ke): the interior box the single-pass sweep updates (the
-        cells with no CPML term; the whole grid without CPML), as the C side sets it."""
+        cells with no CPML term; the whole grid without CPML), as the C side sets it; with
+        step pairs, the two-step sweep's box (two cells further from the CPML slabs)."""
         f = self.sim.fields
         nx, ny, nz = f.nx, f.ny, f.nz
         cp = self.sim.cpml
         if cp is None:
             return 0, nx, 0, ny, 0, f.nz_pitch
         p = cp.npml
+        if self.pairs if pairs is None else pairs:
+            box = (p + 2, nx - 2 - p, p + 2, ny - 2 - p, 32 * ((p + 33) // 32), 32 * ((nz - 2 - p) // 32))
+            if box[1] - box[0] >= 4 and box[3] - box[2] >= 4 and box[5] > box[4]:
+                return box
         box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32))
         if box[0] >= box[1] or box[2] >= box[3] or box[4] >= box[5]:
             return nx, nx, ny, ny, f.nz_pitch, f.nz_pitch
@@ -201,6 +219,8 @@ class SlabStepper(FusedStepper):
         _check_whole_step(sim, "slab")
         self.sim = sim
         self.alt = None
+        self.mid = None
+        self.pairs = False
         self.T = self.CX = 0
         cp = sim.cpml
         self.S = slab_planes or slab_planes_for(f, cp)

@@ -269,16 +269,24 @@ typedef struct {
      * when ce_mat[0] is set, ce[] is ignored.  Index arrays must be 4-byte aligned. */
     const unsigned char *ce_mat[3];
     const float *ce_lut;
+    /* fused only, with CPML: a third buffer set (ex..hz).  When set, steps advance in
+     * pairs: the interior sweep forms two steps per pass (the first step's interior
+     * state stays on chip) over a box two cells further from the CPML slabs, and the
+     * shell passes write the first step's shell values here; a trailing odd step is an
+     * ordinary one.  rows_per_thread must then be 1..9 (rows+3 waves per work-group).
+     * NULL: one step per sweep. */
+    float *e_mid[6];
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
- * of each source; probe_series: DEVICE array (nsteps, nprobe).  On return *parity is
- * nsteps & 1: 1 means the current state is in e_alt[] / psiE*_alt[]. */
+ * of each source; probe_series: DEVICE array (nsteps, nprobe).  On return *parity is 1
+ * when the current state is in e_alt[] / psiE_alt (nsteps & 1 with one step per sweep;
+ * with e_mid[] set, step pairs and a trailing odd step each swap the two sets once). */
 int mifdtd_fused_run(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
                      float *probe_series, int *parity, mifdtd_stream_t stream);
 /* The same, with HIP events around each interior-sweep launch: *sweep_ms receives the
- * average sweep duration (waits for the stream to reach the last sweep).  No reference
- * counterpart (measurement). */
+ * sweep time per step (the sum of the sweep launches' durations / nsteps; waits for the
+ * stream to reach the last sweep).  No reference counterpart (measurement). */
 int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *tvals,
                            float *probe_series, int *parity, mifdtd_stream_t stream,
                            float *sweep_ms);

@@ -5,7 +5,7 @@ initial fields (every cell, the walls and the CPML slabs live from the first ste
   and probe): the in-place engine equals the float32 C oracle after 2 steps; the fused
   engine (CE as uint8 material indices, field sets in one staggered allocation) equals
   the in-place engine (float CE) in every field, psi value and probe sample after 2 and
-  then 5 steps;
+  then 5 steps, with single steps and with step pairs;
 * the 512^3 4th-order symplectic workload (100-sphere dielectric, CPML 10,
   S = 0.9 S_max): equal to the float32 C oracle after 2 steps;
 * the 64-cube uneven batch (CPML 8): every instance of the batched run
@@ -114,6 +114,33 @@ def test_yee_1024cube_spheres_engines_and_oracle():
     arrays = fus.fields.ehs + fus._fused.alt
     base = arrays[0].untyped_storage().data_ptr()
     assert all(t.untyped_storage().data_ptr() == base for t in arrays)
+
+
+def test_yee_1024cube_step_pairs_equal_inplace():
+    """Step pairs on the 1024^3 workload (two-step sweep over the box 14 .. 1010 in x and
+    y, 32 .. 992 in z, 1776 blocks of 12 waves; shell and fringe passes into the third
+    buffer set): equal to the in-place engine after 2 steps (one pair) and then 5 more
+    (two pairs and a single step), fields, psi and probe series."""
+    _need(160)
+    from fdtd_accelerate_amd.fused import FusedStepper
+    ref = _yee("kemp", False)
+    fus = _yee("fused", True)
+    _randomize(ref.fields.ehs, 21)
+    for a, b in zip(ref.fields.ehs, fus.fields.ehs):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    fus._fused = FusedStepper(fus, step_pairs=True)
+    assert fus._fused.pairs and fus._fused.sweep_box() == (14, 1010, 14, 1010, 32, 992)
+    for steps in (2, 5):
+        a = ref.run(steps)
+        b = fus.run(steps)
+        assert _equal_host(a, b), steps
+        for name in NAMES:
+            assert torch.equal(ref.field(name), fus.field(name)), (steps, name)
+    for key in ref.cpml.psiE:
+        for c in range(2):
+            assert torch.equal(ref.cpml.psiE[key][c], fus.cpml.psiE[key][c]), ("psiE", key, c)
+            assert torch.equal(ref.cpml.psiH[key][c], fus.cpml.psiH[key][c]), ("psiH", key, c)
 
 
 def test_s4_512cube_spheres_vs_oracle():

@@ -123,6 +123,80 @@ def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
     assert same_values(series[0], series[1])
 
 
+def _pair_box(ns, npml):
+    """The two-step sweep's box (xb, xe, yb, ye, kb, ke) or None when the grid runs single steps."""
+    nx, ny, nz = ns
+    p = npml
+    box = (p + 2, nx - 2 - p, p + 2, ny - 2 - p, 32 * ((p + 33) // 32), 32 * ((nz - 2 - p) // 32))
+    if box[1] - box[0] >= 4 and box[3] - box[2] >= 4 and box[5] > box[4]:
+        return box
+    return None
+
+
+@pytest.mark.parametrize("ns,npml,T,CX,steps,table", [
+    ((22, 22, 200), 3, 1, 1, 3, False),
+    ((40, 36, 160), 6, 2, 11, 4, False),
+    ((26, 70, 300), 5, 7, 9, 2, False),
+    ((20, 30, 300), 4, 5, 7, 5, True),
+    ((17, 45, 530), 3, 9, 64, 6, False),
+    ((29, 40, 300), 5, 9, 16, 5, True),
+    ((33, 27, 700), 4, 4, 3, 7, False),
+    ((12, 13, 1000), 4, 3, 5, 3, False),
+])
+def test_fused_pairs_match_inplace(ns, npml, T, CX, steps, table):
+    """Step pairs (two steps per interior sweep, shell passes into a third buffer set):
+    equal to the in-place engine after odd and even step counts -- fields, psi, probe
+    series -- with sources and probes in the CPML shell, on both fringes of the two-step
+    box, inside it and across its z segments (256 cells)."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    from fdtd_accelerate_amd.sim import YeeSim
+    nx, ny, nz = ns
+    fp = {"rows_per_thread": T, "planes_per_chunk": CX, "step_pairs": True}
+    sims = [_sim(ns, npml, "kemp"), YeeSim(*ns, npml=npml, coeff_use="e", engine="fused", fused_params=fp)]
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    rng = np.random.default_rng(17)
+    vals = (0.5 / np.array([1.0, 2.25, 4.0, 9.0])).astype(np.float32)
+    mats = [rng.integers(0, len(vals), (nx, ny, nz)) for _ in range(3)]
+    box = _pair_box(ns, npml)
+    xb, xe, yb, ye, kb, ke = box or (npml + 2, nx - 2 - npml, npml + 2, ny - 2 - npml, 32, 64)
+    kz = min(ke - 1, 256) if ke > 256 else (kb + ke) // 2
+    probes = [("ez", (nx // 2 + 1, ny // 2, nz // 2)), ("hx", (1, 2, 3)), ("hz", (nx - 1, ny - 1, nz - 1)),
+              ("hy", (xb, yb, kb)), ("ey", (xe - 1, ye - 1, ke - 1)), ("hx", (xe - 1, ye - 1, ke - 1)),
+              ("ex", (xb + 1, yb + 1, kz)), ("hz", (xb + 1, yb + 1, kz - 1))]
+    for s in sims:
+        _fill(s, 23)
+        if table:
+            s.fields.set_ces(*[vals[m] for m in mats])
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_source("hy", (3, 1, 0), (3, ny - 2, nz - 1), tf, 0.25)
+        s.add_source("ey", (1, 1, kz), (nx - 2, ny - 2, kz), tf, 0.5)
+        s.add_source("hx", (nx // 2, yb + 1, 0), (nx // 2, yb + 1, nz - 1), tf, 0.125)
+        for comp, pt in probes:
+            s.add_probe(comp, pt)
+    if table:
+        assert sims[1].fields.compress_ce()
+    series = [s.run(steps) for s in sims]
+    assert sims[1]._fused.pairs
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert same_values(a, b), (name, np.abs(a - b).max(), np.argwhere(canon_arr(a) != canon_arr(b))[:4])
+    assert same_values(series[0], series[1]), np.argwhere(series[0] != series[1])[:8]
+    for key in sims[0].cpml.psiE:
+        for c in range(2):
+            assert same_values(sims[0].cpml.psiE[key][c].cpu().numpy(), sims[1].cpml.psiE[key][c].cpu().numpy()), ("psiE", key, c)
+            assert same_values(sims[0].cpml.psiH[key][c].cpu().numpy(), sims[1].cpml.psiH[key][c].cpu().numpy()), ("psiH", key, c)
+    # continuing with another odd call keeps equality (parity of the buffer sets)
+    series = [s.run(3) for s in sims]
+    assert same_values(series[0], series[1])
+    for name in ("ex", "hz"):
+        assert same_values(sims[0].field(name).cpu().numpy(), sims[1].field(name).cpu().numpy()), name
+
+
+def canon_arr(a):
+    return np.ascontiguousarray(a, dtype=np.float32) + np.float32(0.0)
+
+
 def test_fused_field_arena(monkeypatch):
     """Field sets carved from one allocation with staggered starts (the layout used for
     arrays of 1 GiB and more, forced here on a small grid, for the in-place engine's six

@@ -244,3 +244,35 @@ def test_static_batch_rejects_beyond_its_limit_without_gpu():
     descs = (_lib.InstanceDesc * 129)()
     rc = _lib.load().mifdtd_batch_run_static(descs, 129, None, None, None, None, 1, None, None, 0, None)
     assert rc == -4 and "at most 128" in _lib.last_error()
+
+
+def test_ctypes_mirrors_match_header(tmp_path):
+    """Every ctypes mirror in _lib has the size and field offsets of its C struct in
+    include/mifdtd.h (a C probe compiled with gcc against the header prints them)."""
+    import ctypes
+    import shutil
+    import subprocess
+    from fdtd_accelerate_amd import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("no gcc")
+    pairs = [(_lib.CpmlHalf, "mifdtd_cpml_half"), (_lib.TfsfPatches, "mifdtd_tfsf_patches"),
+             (_lib.FusedDesc, "mifdtd_fused_desc"), (_lib.InstanceDesc, "mifdtd_instance"),
+             (_lib.DesignBox, "mifdtd_design_box")]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mifdtd.h"', "int main(void) {"]
+    for cls, cname in pairs:
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, "-I", "/opt/rocm/include", str(src), "-o", str(exe)], check=True,
+                   capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cls, cname in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
