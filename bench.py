@@ -408,29 +408,40 @@ def run_tfsf(args):
 def run_drude(args):
     """Drude dispersion: the default sphere workload (CPML on all faces) with a Drude box
     [n/4, 3n/4)^3 (plasma frequency 0.5, damping 0.05 per unit time) on the in-place
-    engine.  The same grid is timed without and then with the Drude medium; the Drude
-    update reads E, CE, J, CJA, CJB and writes E and J (28 B per component, 84 B per box
-    cell)."""
+    engine (--engine kemp) or the fused engine (default).  The same grid is timed without
+    and then with the Drude medium; the Drude update reads J, CJA, CJB (and, in place, E and
+    CE) and writes J (and, in place, E): 28 B per component, 84 B per box cell."""
     import torch
     from fdtd_accelerate_amd import Drude
     a = argparse.Namespace(**vars(args))
-    a.engine = "kemp"
-    a.no_ce_lut = True  # the Drude update reads the float CE arrays
+    fused = args.engine == "fused"
+    a.engine = "fused" if fused else "kemp"
+    a.no_ce_lut = not fused  # the in-place Drude update reads the float CE arrays
     sim = build_sim(a)
     n = args.n
     p0, p1 = (n // 4,) * 3, (3 * n // 4 - 1,) * 3
-    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
-    Drude(sim.fields, p0, p1, 0.5, 0.05)
-    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    nd = (p1[0] - p0[0] + 1) ** 3
     cells = n ** 3
+    if fused:
+        # the sweep and shell E pass carry the Drude update: the sweep's 60 B per cell plus
+        # 84 B per Drude cell (the box lies inside the sweep box)
+        ms0, _, _, _, _ = time_fused(sim, args.steps, args.warmup)
+        Drude(sim.fields, p0, p1, 0.5, 0.05)
+        ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup)
+        kbytes += 84.0 * nd / cells
+        kname = kname.replace("k_fused (", "k_fused with a Drude box (", 1)
+    else:
+        ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+        Drude(sim.fields, p0, p1, 0.5, 0.05)
+        ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
     achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
     return {
         "metric": METRIC, "value": round(cells / (ms * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point "
-                               f"source, Drude box [{p0[0]},{p1[0]}]^3 (wp 0.5, gamma 0.05)", "engine": "kemp",
-                   "cells": cells, "drude_cells": (p1[0] - p0[0] + 1) ** 3},
+                               f"source, Drude box [{p0[0]},{p1[0]}]^3 (wp 0.5, gamma 0.05)",
+                   "engine": "fused" if fused else "kemp", "cells": cells, "drude_cells": nd},
         "ms_per_step_without_drude": round(ms0, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

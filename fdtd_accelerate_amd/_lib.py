@@ -46,6 +46,8 @@ class FusedDesc(ctypes.Structure):
         ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
         ("rows_per_thread", _I), ("planes_per_chunk", _I),
         ("ce_mat", _P * 3), ("ce_lut", _P), ("e_mid", _P * 6),
+        ("drude_box", _I * 6), ("drude_a", _P * 3), ("drude_b", _P * 3), ("drude_j", _P * 3),
+        ("drude_j_alt", _P * 3),
     ]
 
 

@@ -44,6 +44,7 @@
 namespace mifdtd {
 
 constexpr int kMaxRows = 15;  // owned rows per block (a block has rows+1 waves, at most 1024 threads)
+constexpr int kDrudeMaxRows = 7;  // with a Drude box: rows+1 waves at 2 per SIMD (up to 256 VGPRs)
 constexpr size_t kMaxLds = 160 * 1024;  // LDS of one work-group on gfx950
 constexpr int kMaxSrc = 4;
 constexpr int kMaxProbe = 8;
@@ -138,11 +139,11 @@ __device__ __forceinline__ size_t row_base(const FGeom &g, int i, int j, int m)
 // (all of step n, zero outside the grid), hx_nx/hy_nx = hx/hy of the element after the
 // segment for its last lane.  All memory loads of the row are issued first.  src: the
 // block's source table (LDS) or null.
-template <int CEM>
+template <int CEM, bool DR = false>
 __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const float *tab, const FSrc *src, int i, int j,
                                           int m, unsigned l4, unsigned lc, bool end_lane, const Q4 hc[3],
                                           const Q4 &hux, const Q4 &huz, const Q4 hN[3], float hx_nx, float hy_nx,
-                                          Q4 &exn, Q4 &eyn, Q4 &ezn)
+                                          Q4 &exn, Q4 &eyn, Q4 &ezn, const SDrude *dr = nullptr, Q4 *jn = nullptr)
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int k0 = g.zb0 + m * 256 + (int)l4;
@@ -192,6 +193,11 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
         }
     }
     ezn = e;
+    if (DR && dr->row_in(i, j)) {  // Drude box: after the core update, before the sources
+        drude_quad(*dr, 0, i, j, k0, c0, exn, jn[0]);
+        drude_quad(*dr, 1, i, j, k0, c1, eyn, jn[1]);
+        drude_quad(*dr, 2, i, j, k0, c2, ezn, jn[2]);
+    }
     if (src) {
         apply_src(*src, 0, i, j, k0, exn);
         apply_src(*src, 1, i, j, k0, eyn);
@@ -273,8 +279,8 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
 // plane's loads issue together.  The LDS exchange is ordered by the barrier and the
 // parity.  Source and probe tables are copied to LDS once per block and consulted only
 // by blocks whose rows they touch.
-template <int CEM>
-__global__ __launch_bounds__(1024, 4) void k_fused(
+template <int CEM, bool DR>
+__global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) void k_fused(
     FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
@@ -282,7 +288,7 @@ __global__ __launch_bounds__(1024, 4) void k_fused(
     float *__restrict__ ohx, float *__restrict__ ohy, float *__restrict__ ohz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my,
-    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb)
+    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb, SDrude dr)
 {
     const Ctx c{ex, ey, ez, hx, hy, hz, oex, oey, oez, ohx, ohy, ohz, cex, cey, cez, mx, my, mz};
     extern __shared__ float lds[];
@@ -402,8 +408,10 @@ __global__ __launch_bounds__(1024, 4) void k_fused(
                 }
             }
             Q4 exn = zero4(), eyn = zero4(), ezn = zero4();
+            Q4 jn[3] = {zero4(), zero4(), zero4()};  // J' of a Drude row (stored by the owner)
             if (live)
-                compute_E<CEM>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn, ezn);
+                compute_E<CEM, DR>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn,
+                                   ezn, &dr, jn);
             if (w < T) {
                 float *es = ebuf + (size_t)((par * T + w) * 2) * 256;
                 *reinterpret_cast<float4 *>(es + l4) = make_float4(exn.v[0], exn.v[1], exn.v[2], exn.v[3]);
@@ -444,6 +452,12 @@ __global__ __launch_bounds__(1024, 4) void k_fused(
                     st4(ohx + rbase + l4, hxo);
                     st4(ohy + rbase + l4, hyo);
                     st4(ohz + rbase + l4, hzo);
+                    if (DR && dr.row_in(i, j)) {
+                        const int k0 = g.zb0 + m * 256 + (int)l4;
+                        drude_store(dr, 0, i, j, k0, jn[0]);
+                        drude_store(dr, 1, i, j, k0, jn[1]);
+                        drude_store(dr, 2, i, j, k0, jn[2]);
+                    }
                 }
                 if (prb_hit) {
                     const int k0 = g.zb0 + m * 256 + (int)l4;
@@ -983,7 +997,7 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
 }
 
 // CE: CEM as in the sweep (kCeTable stages the 3 x 256 table in LDS per block).
-template <int CEM>
+template <int CEM, bool DR>
 __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigned total, const float *ex, const float *ey,
                                                  const float *ez, float *oex, float *oey, float *oez,
                                                  const float *__restrict__ hx, const float *__restrict__ hy,
@@ -992,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigne
                                                  const unsigned char *__restrict__ mx,
                                                  const unsigned char *__restrict__ my,
                                                  const unsigned char *__restrict__ mz, const float *__restrict__ lut,
-                                                 SCpml cp, SSrc src, SProbe prb)
+                                                 SCpml cp, SSrc src, SProbe prb, SDrude dr)
 {
     __shared__ float tab[CEM == kCeTable ? 768 : 1];
     if (CEM == kCeTable) {
@@ -1003,8 +1017,8 @@ __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigne
     if (f >= total) return;
     int i, j, k0;
     shell_quad(sb, f, i, j, k0);
-    e_quad_at<CEM, true>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my, mz,
-                         tab);
+    e_quad_at<CEM, true, DR>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my,
+                             mz, tab, &dr);
 }
 
 __global__ __launch_bounds__(256) void k_shell_h(SGeom g, ShellBoxes sb, unsigned total, const float *__restrict__ oex,
@@ -1214,6 +1228,38 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             }
         }
     }
+    // Drude box (single steps only)
+    const bool drude = d->drude_j[0] != nullptr;
+    SDrude sdr{};
+    if (drude) {
+        if (pairs) return fail(MIFDTD_EINVAL, "step pairs (e_mid) do not support a Drude box");
+        if (T > kDrudeMaxRows)
+            return fail(MIFDTD_EINVAL, "rows_per_thread must be in [1,%d] with a Drude box", kDrudeMaxRows);
+        const int *b = d->drude_box;
+        if (b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] >= nx || b[3] >= ny || b[5] >= nz || b[0] > b[1] || b[2] > b[3] ||
+            b[4] > b[5])
+            return fail(MIFDTD_EINVAL, "Drude box outside the grid");
+        for (int q = 0; q < 3; q++) {
+            if (!d->drude_a[q] || !d->drude_b[q] || !d->drude_j[q] || !d->drude_j_alt[q])
+                return fail(MIFDTD_ENULL, "Drude arrays must all be set");
+            if (!aligned16(d->drude_a[q]) || !aligned16(d->drude_b[q]) || !aligned16(d->drude_j[q]) ||
+                !aligned16(d->drude_j_alt[q]))
+                return fail(MIFDTD_EALIGN, "Drude arrays not 16-byte aligned");
+        }
+        sdr.i0 = b[0]; sdr.i1 = b[1]; sdr.j0 = b[2]; sdr.j1 = b[3]; sdr.k0 = b[4]; sdr.k1 = b[5];
+        sdr.lj = b[3] - b[2] + 1;
+        sdr.lk = b[5] - b[4] + 1;
+        for (int q = 0; q < 3; q++) {
+            sdr.a[q] = d->drude_a[q];
+            sdr.b[q] = d->drude_b[q];
+            sdr.jin[q] = d->drude_j[q];
+            sdr.jout[q] = d->drude_j_alt[q];
+        }
+    } else {
+        sdr.i0 = sdr.j0 = sdr.k0 = 1;  // an empty box
+        sdr.i1 = sdr.j1 = sdr.k1 = 0;
+    }
+
     SGeom sg{};
     sg.nx = nx; sg.ny = ny; sg.nzp = nzp; sg.nz = nz;
 
@@ -1289,13 +1335,19 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         c.pe_out = pe_out;
         const unsigned nbk = (unsigned)((tot + 255) / 256);
         if (!nbk) return 0;
-#define MIFDTD_SHELL_E_LAUNCH(CEV)                                                                                    \
-    k_shell_e<CEV><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, ei[0], ei[1], ei[2], eo[0], eo[1], eo[2], hi[3], hi[4], \
-                                        hi[5], d->ce[0], d->ce[1], d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], \
-                                        d->ce_lut, c, ss, sp)
-        if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable);
-        else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays);
-        else MIFDTD_SHELL_E_LAUNCH(kCeConst);
+#define MIFDTD_SHELL_E_LAUNCH(CEV, DRV)                                                                           \
+    k_shell_e<CEV, DRV><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, ei[0], ei[1], ei[2], eo[0], eo[1], eo[2], hi[3],   \
+                                             hi[4], hi[5], d->ce[0], d->ce[1], d->ce[2], d->ce_mat[0], d->ce_mat[1], \
+                                             d->ce_mat[2], d->ce_lut, c, ss, sp, sdr)
+        if (drude) {
+            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable, true);
+            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays, true);
+            else MIFDTD_SHELL_E_LAUNCH(kCeConst, true);
+        } else {
+            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable, false);
+            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays, false);
+            else MIFDTD_SHELL_E_LAUNCH(kCeConst, false);
+        }
 #undef MIFDTD_SHELL_E_LAUNCH
         MIFDTD_CHECK_LAUNCH("CPML shell E launch");
         return 0;
@@ -1318,6 +1370,11 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         float *tp = pin;
         pin = pout;
         pout = tp;
+        for (int q = 0; q < 3; q++) {  // Drude current sets
+            const float *x = sdr.jin[q];
+            sdr.jin[q] = sdr.jout[q];
+            sdr.jout[q] = const_cast<float *>(x);
+        }
     };
     const unsigned nthr = 64u * (unsigned)(T + 1);
     const unsigned nthr2 = 64u * (unsigned)R2;
@@ -1365,13 +1422,20 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp)) return rc;
         if (sweep) {
             mark(0);
-#define MIFDTD_FUSED_LAUNCH(CEV)                                                                                 \
-    k_fused<CEV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
-                                                     out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
-                                                     d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp)
-            if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable);
-            else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays);
-            else MIFDTD_FUSED_LAUNCH(kCeConst);
+#define MIFDTD_FUSED_LAUNCH(CEV, DRV)                                                                             \
+    k_fused<CEV, DRV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0],       \
+                                                          out[1], out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], \
+                                                          d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2],        \
+                                                          d->ce_lut, fs, fp, sdr)
+            if (drude) {
+                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, true);
+                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, true);
+                else MIFDTD_FUSED_LAUNCH(kCeConst, true);
+            } else {
+                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, false);
+                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, false);
+                else MIFDTD_FUSED_LAUNCH(kCeConst, false);
+            }
 #undef MIFDTD_FUSED_LAUNCH
             mark(1);
             ++nsweeps;

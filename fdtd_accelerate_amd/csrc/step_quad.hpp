@@ -40,6 +40,69 @@ struct SProbe {
     float *out;  // row of the probe series for this step; probe n writes out[slot[n]]
 };
 
+// A Drude box (auxiliary current; kuFDTD matter.py:104-221, drude_ade.c): on cells of the
+// inclusive box [i0,i1] x [j0,j1] x [k0,k1], right after the core E update and before
+// CPML-E and the sources, each component takes e' = e - CE J ; J' = CJA J + CJB e'.
+// a/b: CJA/CJB, C-order box arrays (lj x lk per plane); J is read from jin and J' written
+// to jout (the step's two current sets).
+struct SDrude {
+    int i0, i1, j0, j1, k0, k1;
+    int lj, lk;
+    const float *a[3], *b[3], *jin[3];
+    float *jout[3];
+    __device__ __forceinline__ bool row_in(int i, int j) const { return i >= i0 && i <= i1 && j >= j0 && j <= j1; }
+};
+
+// Drude update of component c on the quad (i, j, k0..k0+3) of a row inside the box (row_in):
+// e: the core-updated values, ce: the quad's CE; returns J' in jn (stored by the caller
+// for the cells it owns).
+__device__ __forceinline__ void drude_quad(const SDrude &d, int c, int i, int j, int k0, const Q4 &ce, Q4 &e, Q4 &jn)
+{
+    const size_t rowb = ((size_t)(i - d.i0) * d.lj + (j - d.j0)) * d.lk;
+    bool in[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) in[l] = k0 + l >= d.k0 && k0 + l <= d.k1;
+    Q4 f, a, b;
+    if (in[0] && in[3] && ((rowb + (size_t)(k0 - d.k0)) & 3u) == 0) {  // whole quad, 16-byte aligned
+        const size_t n0 = rowb + (size_t)(k0 - d.k0);
+        f = ld4(d.jin[c] + n0);
+        a = ld4(d.a[c] + n0);
+        b = ld4(d.b[c] + n0);
+    } else {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const size_t n = in[l] ? rowb + (size_t)(k0 + l - d.k0) : rowb;
+            f.v[l] = in[l] ? d.jin[c][n] : 0.f;
+            a.v[l] = in[l] ? d.a[c][n] : 0.f;
+            b.v[l] = in[l] ? d.b[c][n] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        if (in[l]) {
+            const float et = e.v[l] - ce.v[l] * f.v[l];
+            e.v[l] = et;
+            jn.v[l] = a.v[l] * f.v[l] + b.v[l] * et;
+        }
+    }
+}
+
+// Stores the J' of drude_quad for the quad's cells inside the box.
+__device__ __forceinline__ void drude_store(const SDrude &d, int c, int i, int j, int k0, const Q4 &jn)
+{
+    const size_t rowb = ((size_t)(i - d.i0) * d.lj + (j - d.j0)) * d.lk;
+    const size_t n0 = rowb + (size_t)(k0 - d.k0);
+    if (k0 >= d.k0 && k0 + 3 <= d.k1 && (n0 & 3u) == 0) {
+        st4(d.jout[c] + n0, jn);
+        return;
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int k = k0 + l;
+        if (k >= d.k0 && k <= d.k1) d.jout[c][rowb + (size_t)(k - d.k0)] = jn.v[l];
+    }
+}
+
 struct SGeom {
     int nx, ny, nzp, nz;
     unsigned Q;        // quads per row
@@ -128,14 +191,14 @@ __device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m,
 // is read from cp.pe and written to cp.pe_out.  CE: CM = 0 the constant 0.5, 1 the float
 // arrays cex.., 2 the uint8 material indices mx.. into tab ([3][256] floats, the same
 // values as the arrays, so the products are identical).
-template <int CM, bool PML>
+template <int CM, bool PML, bool DR = false>
 __device__ __forceinline__ void e_quad_at(
     const SGeom &g, int i, int j, int k0, const float *exi, const float *eyi, const float *ezi, float *exo,
     float *eyo, float *ezo, const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const SCpml &cp, const SSrc &src, const SProbe &prb, const unsigned char *__restrict__ mx = nullptr,
     const unsigned char *__restrict__ my = nullptr, const unsigned char *__restrict__ mz = nullptr,
-    const float *tab = nullptr)
+    const float *tab = nullptr, const SDrude *dr = nullptr)
 {
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
     const size_t id = ((size_t)i * ny + j) * nzp + k0;
@@ -198,15 +261,21 @@ __device__ __forceinline__ void e_quad_at(
 
     // -------------------------------------------------------------- compute
     // ex: += cex*((hz[j+1]-hz) - (hy[k+1]-hy)) ; CPML + y psi2 (hz), - z psi1 (hy)
-    if (jy) {
-        float y1[4];
+    // Drude box: J' of the three components (rows inside the box)
+    const bool drow = DR && dr->row_in(i, j);
+    Q4 jn0 = z4, jn1 = z4, jn2 = z4;
+    float y1[4];
 #pragma unroll
-        for (int l = 0; l < 4; ++l) y1[l] = l < 3 ? hy0.v[l + 1] : hyn;
+    for (int l = 0; l < 4; ++l) y1[l] = l < 3 ? hy0.v[l + 1] : hyn;
+    if (jy) {
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             const float v = e0.v[l] + c0.v[l] * ((hzU.v[l] - hz0.v[l]) - (y1[l] - hy0.v[l]));
             e0.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
         }
+    }
+    if (drow) drude_quad(*dr, 0, i, j, k0, c0, e0, jn0);
+    if (jy) {
         if (PML) {
             if (yt > 0) {
                 const float b = cp.bE[yt], a = cp.aE[yt];
@@ -228,15 +297,18 @@ __device__ __forceinline__ void e_quad_at(
         }
     }
     // ey: += cey*((hx[k+1]-hx) - (hz[i+1]-hz)) ; CPML - x psi1 (hz), + z psi2 (hx)
-    if (ix) {
-        float x1[4];
+    float x1[4];
 #pragma unroll
-        for (int l = 0; l < 4; ++l) x1[l] = l < 3 ? hx0.v[l + 1] : hxn;
+    for (int l = 0; l < 4; ++l) x1[l] = l < 3 ? hx0.v[l + 1] : hxn;
+    if (ix) {
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
             const float v = e1.v[l] + c1.v[l] * ((x1[l] - hx0.v[l]) - (hzN.v[l] - hz0.v[l]));
             e1.v[l] = (k0 + l < nz - 1) ? v : v * 0.f;
         }
+    }
+    if (drow) drude_quad(*dr, 1, i, j, k0, c1, e1, jn1);
+    if (ix) {
         if (PML) {
             if (xt > 0) {
                 const float b = cp.bE[xt], a = cp.aE[xt];
@@ -264,6 +336,9 @@ __device__ __forceinline__ void e_quad_at(
             const float v = e2.v[l] + c2.v[l] * ((hyN.v[l] - hy0.v[l]) - (hxU.v[l] - hx0.v[l]));
             e2.v[l] = (k0 + l < nz) ? v : v * 0.f;
         }
+    }
+    if (drow) drude_quad(*dr, 2, i, j, k0, c2, e2, jn2);
+    if (ix && jy) {
         if (PML) {
             if (xt > 0) {
                 const float b = cp.bE[xt], a = cp.aE[xt];
@@ -296,6 +371,11 @@ __device__ __forceinline__ void e_quad_at(
     st4(exo + id, e0);
     st4(eyo + id, e1);
     st4(ezo + id, e2);
+    if (drow) {
+        drude_store(*dr, 0, i, j, k0, jn0);
+        drude_store(*dr, 1, i, j, k0, jn1);
+        drude_store(*dr, 2, i, j, k0, jn2);
+    }
     if (PML) {
         if (xt > 0) {
             st4(cp.pe_out + ox1, px1);

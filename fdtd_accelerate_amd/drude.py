@@ -8,7 +8,9 @@ with CJA = (2 - gamma dt) / (2 + gamma dt) and CJB = 4 (dt wp)^2 / (2 + gamma dt
 computed in float32 as ``matter.py:192-202`` does.  The plasma frequency wp and the
 damping gamma (in the grid's units: dx = 1, dt = fields.dt) are scalars, region-sized
 arrays, or a triple of them (one per component).  CE is the grid's E coefficient (0.5
-without CE arrays).  Supported by the in-place engine; the whole-step engines reject it.
+without CE arrays).  Supported by the in-place engine and (one box) by the fused engine,
+whose sweep and shell E pass apply it in the same order; the slab and batched engines
+reject it.
 """
 from __future__ import annotations
 

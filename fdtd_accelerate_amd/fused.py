@@ -26,16 +26,18 @@ _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 
 
 def in_place_only():
-    """Instance types only the in-place engine applies (it runs the Fields instance list;
-    the whole-step and batched engines take their step from a descriptor)."""
+    """Instance types the in-place engine applies through the Fields instance list; the
+    whole-step and batched engines take their step from a descriptor and reject them,
+    except those an engine carries itself (the fused engine: one Drude box)."""
     from .drude import Drude
     from .pbc import Pbc
     from .tfsf import Tfsf
     return ((Pbc, "periodic boundaries"), (Tfsf, "TFSF source"), (Drude, "Drude medium"))
 
 
-def _check_whole_step(sim, engine):
-    """Limits shared by the whole-step engines (same descriptor)."""
+def _check_whole_step(sim, engine, allow=()):
+    """Limits shared by the whole-step engines (same descriptor); ``allow``: instance
+    types of ``in_place_only`` this engine carries itself."""
     f = sim.fields
     if f.ch_on:
         raise ValueError(f"the {engine} engine uses CH = 0.5; use engine='kemp' for CH arrays")
@@ -43,7 +45,7 @@ def _check_whole_step(sim, engine):
     if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
         raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
     for cls, what in in_place_only():
-        if any(isinstance(i, cls) for i in f.instance_list):
+        if cls not in allow and any(isinstance(i, cls) for i in f.instance_list):
             raise ValueError(f"the {engine} engine has no {what}; use engine='kemp'")
     if len(sim.sources) > _lib.MAX_SRC:
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_SRC} sources")
@@ -73,15 +75,22 @@ class FusedStepper:
     kernel_name = "k_fused"
 
     def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None):
+        from .drude import Drude
         f = sim.fields
-        _check_whole_step(sim, "fused")
+        _check_whole_step(sim, "fused", allow=(Drude,))
+        drudes = [i for i in f.instance_list if isinstance(i, Drude)]
+        if len(drudes) > 1:
+            raise ValueError("the fused engine carries one Drude box; use engine='kemp' for more")
+        self.drude = drudes[0] if drudes else None
+        if self.drude is not None and step_pairs:
+            raise ValueError("step pairs do not support a Drude box")
         cp = sim.cpml
         self.sim = sim
         npml = cp.npml if cp else 0
         # step pairs: two steps per interior sweep (needs CPML; a third field set holds the
         # first step's shell values)
-        self.pairs = bool(cp is not None and (step_pairs if step_pairs is not None
-                                              else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
+        self.pairs = bool(cp is not None and self.drude is None and
+                          (step_pairs if step_pairs is not None else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
         f.ehs, self.alt = _ping_pong_sets(f)
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
         self.T = rows_per_thread or T
@@ -92,7 +101,14 @@ class FusedStepper:
         if self.pairs and not (self.T <= 9 and lds_fits_pairs(self.T, self.CX, table)):
             raise ValueError(f"rows_per_thread {self.T} / planes_per_chunk {self.CX} do not fit the two-step sweep "
                              "(at most 9 rows, 160 KiB of LDS)")
+        if self.drude is not None:  # the Drude sweep runs rows+1 <= 8 waves (256 VGPRs each)
+            if rows_per_thread is None:
+                self.T = min(self.T, 7)
+            if self.T > 7:
+                raise ValueError(f"rows_per_thread {self.T}: at most 7 with a Drude box")
         self.mid = [torch.zeros_like(t) for t in f.ehs] if self.pairs else None
+        # the Drude current of step n+1 (the two sets swap with the field sets)
+        self.j_alt = [torch.zeros_like(t) for t in self.drude.J] if self.drude is not None else None
         if cp is not None:
             cp.alt_psiE()
 
@@ -143,6 +159,16 @@ class FusedStepper:
         if self.mid:
             for c in range(6):
                 d.e_mid[c] = self.mid[c].data_ptr()
+        dr = getattr(self, "drude", None)
+        if dr is not None:
+            box = (dr.pt0[0], dr.pt1[0], dr.pt0[1], dr.pt1[1], dr.pt0[2], dr.pt1[2])
+            for m in range(6):
+                d.drude_box[m] = box[m]
+            for c in range(3):
+                d.drude_a[c] = dr.cja[c].data_ptr()
+                d.drude_b[c] = dr.cjb[c].data_ptr()
+                d.drude_j[c] = dr.J[c].data_ptr()
+                d.drude_j_alt[c] = self.j_alt[c].data_ptr()
         return d
 
     def _tvals(self, nsteps):
@@ -174,6 +200,8 @@ class FusedStepper:
             f.ehs, self.alt = self.alt, f.ehs
             if sim.cpml is not None:
                 sim.cpml.swap_psiE()
+            if getattr(self, "drude", None) is not None:
+                self.drude.J, self.j_alt = self.j_alt, self.drude.J
         return sweep_ms.value
 
     def sweep_box(self, pairs=None):
@@ -221,6 +249,7 @@ class SlabStepper(FusedStepper):
         self.alt = None
         self.mid = None
         self.pairs = False
+        self.drude = None
         self.T = self.CX = 0
         cp = sim.cpml
         self.S = slab_planes or slab_planes_for(f, cp)

@@ -276,6 +276,16 @@ typedef struct {
      * ordinary one.  rows_per_thread must then be 1..9 (rows+3 waves per work-group).
      * NULL: one step per sweep. */
     float *e_mid[6];
+    /* fused only: one Drude box (auxiliary current, kuFDTD matter.py:104-221 with
+     * drude_ade.c: after the core E update, before CPML-E and the sources, e' = e - CE J,
+     * J' = CJA J + CJB e'), or drude_j[0] == NULL.  drude_box: i0, i1, j0, j1, k0, k1
+     * (inclusive, inside the grid); drude_a / drude_b: CJA / CJB per component as C-order
+     * box arrays; drude_j: J of step n, drude_j_alt: receives step n+1 (the two swap with
+     * the field sets: *parity 1 means the current J is in drude_j_alt).  Arrays 16-byte
+     * aligned.  Not with e_mid. */
+    int drude_box[6];
+    const float *drude_a[3], *drude_b[3];
+    float *drude_j[3], *drude_j_alt[3];
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
