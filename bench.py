@@ -371,23 +371,25 @@ def run_pbc(args):
 def run_tfsf(args):
     """TFSF plane wave: 1024^3 with CPML on all faces, the sphere dielectric inside the
     box [64, n-65]^3 (its faces in vacuum), a plane wave along x (wavelength 20 cells,
-    polarization angle 0.3) entering through all six faces; in-place engine.  The same
-    grid is timed without and then with the plane wave."""
+    polarization angle 0.3) entering through all six faces; fused engine (default) or
+    in-place (--engine kemp).  The same grid is timed without and then with the plane wave."""
     import torch
     from fdtd_accelerate_amd import Tfsf, geometry
     from fdtd_accelerate_amd.sim import YeeSim
     n, npml = args.n, args.npml
     lo_box, hi_box = 64, n - 65
-    sim = YeeSim(n, n, n, npml=npml, coeff_use="e")
+    fused = args.engine == "fused"
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="e", engine="fused" if fused else "kemp")
     lo, hi = (lo_box + 8,) * 3, (hi_box - 8,) * 3
     rmax = min(40.0, (hi_box - lo_box) / 8)
     spheres = geometry.random_spheres(2024, args.spheres, (min(8.0, rmax / 2), rmax), (1.0, 12.0), lo, hi)
     geometry.paint_ce(sim.fields.ces, spheres, (n, n, n))
     sim.fields.compress_ce()
     torch.cuda.synchronize()
-    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+    runner = time_fused if fused else time_kemp
+    ms0, _, _, _, _ = runner(sim, args.steps, args.warmup)
     Tfsf(sim.fields, (lo_box,) * 3, (hi_box,) * 3, ("fb", "fb", "fb"), 20.0, ("normal", "x"), 0.3)
-    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
     cells = n ** 3
     achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
     return {
@@ -396,7 +398,7 @@ def run_tfsf(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres inside the TFSF box [{lo_box},{hi_box}]^3, "
                                f"CPML {npml} all faces, x-directed TFSF plane wave (wavelength 20, polarization 0.3)",
-                   "engine": "kemp", "cells": cells},
+                   "engine": "fused" if fused else "kemp", "cells": cells},
         "ms_per_step_without_tfsf": round(ms0, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

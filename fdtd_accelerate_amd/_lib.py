@@ -48,6 +48,8 @@ class FusedDesc(ctypes.Structure):
         ("ce_mat", _P * 3), ("ce_lut", _P), ("e_mid", _P * 6),
         ("drude_box", _I * 6), ("drude_a", _P * 3), ("drude_b", _P * 3), ("drude_j", _P * 3),
         ("drude_j_alt", _P * 3),
+        ("tfsf_e", TfsfPatches), ("tfsf_h", TfsfPatches), ("tfsf_line_e", _P), ("tfsf_line_h", _P),
+        ("tfsf_abc", _P), ("tfsf_line_n", _I), ("tfsf_src", _P),
     ]
 
 

@@ -139,12 +139,15 @@ __device__ __forceinline__ size_t row_base(const FGeom &g, int i, int j, int m)
 // (all of step n, zero outside the grid), hx_nx/hy_nx = hx/hy of the element after the
 // segment for its last lane.  All memory loads of the row are issued first.  src: the
 // block's source table (LDS) or null.
-template <int CEM, bool DR = false>
+template <int CEM, int FEAT = 0>
 __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const float *tab, const FSrc *src, int i, int j,
                                           int m, unsigned l4, unsigned lc, bool end_lane, const Q4 hc[3],
                                           const Q4 &hux, const Q4 &huz, const Q4 hN[3], float hx_nx, float hy_nx,
-                                          Q4 &exn, Q4 &eyn, Q4 &ezn, const SDrude *dr = nullptr, Q4 *jn = nullptr)
+                                          Q4 &exn, Q4 &eyn, Q4 &ezn, const SDrude *dr = nullptr, Q4 *jn = nullptr,
+                                          const STfsf *tf = nullptr, unsigned tmask = 0)
 {
+    constexpr bool DR = (FEAT & kFeatDrude) != 0;
+    constexpr bool TF = (FEAT & kFeatTfsf) != 0;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int k0 = g.zb0 + m * 256 + (int)l4;
     const size_t rb = row_base(g, i, j, m);
@@ -203,15 +206,17 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
         apply_src(*src, 1, i, j, k0, eyn);
         apply_src(*src, 2, i, j, k0, ezn);
     }
+    if (TF && tmask) tfsf_quad(*tf, 0, i, j, k0, exn, eyn, ezn, tmask);  // TFSF E patches after the sources
 }
 
 // H_new of row j of plane i for this lane's quad: hc = H(i,j) of step n, exn/eyn/ezn =
 // E_new(i,j), exb/ezb = E_new(i,j-1) (ex, ez), eyp/ezp = E_new(i-1,j) (ey, ez),
 // exm/eym = E_new(i,j) at k0-1 (ex, ey).
+template <bool TF = false>
 __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i, int j, int m, unsigned l4,
                                           const Q4 hc[3], const Q4 &exn, const Q4 &eyn, const Q4 &ezn, const Q4 &exb,
                                           const Q4 &ezb, const Q4 &eyp, const Q4 &ezp, float exm, float eym, Q4 &hxo,
-                                          Q4 &hyo, Q4 &hzo)
+                                          Q4 &hyo, Q4 &hzo, const STfsf *tf = nullptr, unsigned tmask = 0)
 {
     const int nz = g.nz;
     const int k0 = g.zb0 + m * 256 + (int)l4;
@@ -251,6 +256,7 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
         apply_src(*src, 4, i, j, k0, hyo);
         apply_src(*src, 5, i, j, k0, hzo);
     }
+    if (TF && tmask) tfsf_quad(*tf, 3, i, j, k0, hxo, hyo, hzo, tmask);  // TFSF H patches after the sources
 }
 
 // Interior sweep (single pass, ping-pong buffers).  A block of T+1 waves owns rows
@@ -279,8 +285,8 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
 // plane's loads issue together.  The LDS exchange is ordered by the barrier and the
 // parity.  Source and probe tables are copied to LDS once per block and consulted only
 // by blocks whose rows they touch.
-template <int CEM, bool DR>
-__global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) void k_fused(
+template <int CEM, int FEAT>
+__global__ __launch_bounds__((FEAT & kFeatDrude) ? 64 * (kDrudeMaxRows + 1) : 1024, (FEAT & kFeatDrude) ? 2 : 4) void k_fused(
     FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
@@ -288,17 +294,22 @@ __global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) v
     float *__restrict__ ohx, float *__restrict__ ohy, float *__restrict__ ohz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my,
-    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb, SDrude dr)
+    const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb, SDrude dr,
+    STfsf tfe, STfsf tfh)
 {
+    constexpr bool DR = (FEAT & kFeatDrude) != 0;
+    constexpr bool TF = (FEAT & kFeatTfsf) != 0;
     const Ctx c{ex, ey, ez, hx, hy, hz, oex, oey, oez, ohx, ohy, ohz, cex, cey, cez, mx, my, mz};
     extern __shared__ float lds[];
     __shared__ FSrc s_src;
     __shared__ FProbe s_prb;
+    __shared__ STfsf s_tf[TF ? 2 : 1];  // TFSF E and H patch tables (read from LDS, not kernel arguments)
     const int T = g.T;
     float *tab = lds;                                 // [3][256] CE table (kCeTable)
     float *ebuf = lds + (CEM == kCeTable ? 768 : 0);  // [2][T][2][256]: ex, ez of rows w = 0..T-1
     float *hbuf = ebuf + 2 * T * 512;                 // [2][T][2][256]: hx, hz of H(i+1) of rows w = 1..T
     float *cbuf = hbuf + 2 * T * 512;                 // [cx+1][T+1][2]: ex, ey at the segment's last cell
+    float *tline = cbuf + (size_t)(g.cx + 1) * (T + 1) * 2;  // TFSF: [2][line_n] the incident line's H, E
 
     // XCD-aware block order: consecutive logical blocks (neighbouring row blocks of a
     // chunk, which share halo rows) land on the same XCD.
@@ -323,7 +334,16 @@ __global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) v
     if (threadIdx.x == 0) {
         if (src_hit) s_src = src;
         if (prb_hit) s_prb = prb;
+        if (TF) {
+            s_tf[0] = tfe;
+            s_tf[TF ? 1 : 0] = tfh;
+            s_tf[0].line = tline;  // the patches read the block's LDS copy of the line
+            s_tf[TF ? 1 : 0].line = tline + tfe.line_n;
+        }
     }
+    if (TF)
+        for (int t = threadIdx.x; t < 2 * tfe.line_n; t += blockDim.x)
+            tline[t] = t < tfe.line_n ? tfe.line[t] : tfh.line[t - tfe.line_n];
     if (CEM == kCeTable)
         for (int t = threadIdx.x; t < 768; t += blockDim.x) tab[t] = lut[t];
     __syncthreads();
@@ -352,6 +372,9 @@ __global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) v
         // first cell of a box starting inside the row: E_new(kb-1) from the output
         // (stored by the shell E pass before this launch)
         const bool carry_mem = m == 0 && g.kb > 0 && own && start_lane;
+        // TFSF patches that meet this row's planes and segment (the plane loop visits only these)
+        const unsigned tmE = TF ? tfsf_mask(s_tf[0], ib, i1 - 1, j, base, base + 255) : 0u;
+        const unsigned tmH = TF ? tfsf_mask(s_tf[TF ? 1 : 0], ib, i1 - 1, j, base, base + 255) : 0u;
         Q4 eyp = zero4(), ezp = zero4();  // E_new(i-1, j)
         Q4 hc[3], hux = zero4(), huz = zero4();
         float hx_nx = 0.f, hy_nx = 0.f;  // hx, hy of the element after the segment
@@ -410,8 +433,8 @@ __global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) v
             Q4 exn = zero4(), eyn = zero4(), ezn = zero4();
             Q4 jn[3] = {zero4(), zero4(), zero4()};  // J' of a Drude row (stored by the owner)
             if (live)
-                compute_E<CEM, DR>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn,
-                                   ezn, &dr, jn);
+                compute_E<CEM, FEAT>(g, c, tab, sp, i, j, m, l4, lc, end_lane, hc, hux, huz, hN, hx_nx, hy_nx, exn, eyn,
+                                     ezn, &dr, jn, &s_tf[0], tmE);
             if (w < T) {
                 float *es = ebuf + (size_t)((par * T + w) * 2) * 256;
                 *reinterpret_cast<float4 *>(es + l4) = make_float4(exn.v[0], exn.v[1], exn.v[2], exn.v[3]);
@@ -443,7 +466,8 @@ __global__ __launch_bounds__(DR ? 64 * (kDrudeMaxRows + 1) : 1024, DR ? 2 : 4) v
                 const Q4 exb = Q4{{xb[l4], xb[l4 + 1], xb[l4 + 2], xb[l4 + 3]}};
                 const Q4 ezb = Q4{{xb[256 + l4], xb[256 + l4 + 1], xb[256 + l4 + 2], xb[256 + l4 + 3]}};
                 Q4 hxo, hyo, hzo;
-                compute_H(g, sp, i, j, m, l4, hc, exn, eyn, ezn, exb, ezb, eyp, ezp, exm, eym, hxo, hyo, hzo);
+                compute_H<TF>(g, sp, i, j, m, l4, hc, exn, eyn, ezn, exb, ezb, eyp, ezp, exm, eym, hxo, hyo, hzo,
+                              &s_tf[TF ? 1 : 0], tmH);
                 const size_t rbase = row_base(g, i, j, m);
                 if (act) {
                     st4(oex + rbase + l4, exn);
@@ -997,7 +1021,7 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
 }
 
 // CE: CEM as in the sweep (kCeTable stages the 3 x 256 table in LDS per block).
-template <int CEM, bool DR>
+template <int CEM, int FEAT>
 __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigned total, const float *ex, const float *ey,
                                                  const float *ez, float *oex, float *oey, float *oez,
                                                  const float *__restrict__ hx, const float *__restrict__ hy,
@@ -1006,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigne
                                                  const unsigned char *__restrict__ mx,
                                                  const unsigned char *__restrict__ my,
                                                  const unsigned char *__restrict__ mz, const float *__restrict__ lut,
-                                                 SCpml cp, SSrc src, SProbe prb, SDrude dr)
+                                                 SCpml cp, SSrc src, SProbe prb, SDrude dr, STfsf tfe)
 {
     __shared__ float tab[CEM == kCeTable ? 768 : 1];
     if (CEM == kCeTable) {
@@ -1017,20 +1041,21 @@ __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigne
     if (f >= total) return;
     int i, j, k0;
     shell_quad(sb, f, i, j, k0);
-    e_quad_at<CEM, true, DR>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my,
-                             mz, tab, &dr);
+    e_quad_at<CEM, true, FEAT>(g, i, j, k0, ex, ey, ez, oex, oey, oez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my,
+                               mz, tab, &dr, &tfe);
 }
 
+template <bool TF>
 __global__ __launch_bounds__(256) void k_shell_h(SGeom g, ShellBoxes sb, unsigned total, const float *__restrict__ oex,
                                                  const float *__restrict__ oey, const float *__restrict__ oez,
                                                  const float *hx, const float *hy, const float *hz, float *ohx,
-                                                 float *ohy, float *ohz, SCpml cp, SSrc src, SProbe prb)
+                                                 float *ohy, float *ohz, SCpml cp, SSrc src, SProbe prb, STfsf tfh)
 {
     const unsigned f = blockIdx.x * 256u + threadIdx.x;
     if (f >= total) return;
     int i, j, k0;
     shell_quad(sb, f, i, j, k0);
-    h_quad_at<true>(g, i, j, k0, oex, oey, oez, hx, hy, hz, ohx, ohy, ohz, cp, src, prb);
+    h_quad_at<true>(g, i, j, k0, oex, oey, oez, hx, hy, hz, ohx, ohy, ohz, cp, src, prb, TF ? &tfh : nullptr);
 }
 
 // Appends the box (planes [i0,i1), rows [j0,j1), quads [q0,q1)) to sb unless it is empty.
@@ -1081,6 +1106,17 @@ inline void add_high_fringe(ShellBoxes &sb, unsigned long long &total, const FGe
 
 // =========================================================================== C ABI
 using namespace mifdtd;
+
+// Launches M(CEV, FEAT) for the run-time feature set f (kFeatDrude | kFeatTfsf bits).
+#define MIFDTD_BY_FEAT(M, CEV, f)                  \
+    do {                                           \
+        switch (f) {                               \
+        case 0: M(CEV, 0); break;                  \
+        case kFeatDrude: M(CEV, kFeatDrude); break; \
+        case kFeatTfsf: M(CEV, kFeatTfsf); break;  \
+        default: M(CEV, kFeatDrude | kFeatTfsf);   \
+        }                                          \
+    } while (0)
 
 extern "C" {
 
@@ -1168,8 +1204,9 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
     g.nblocks = (int)nbl;
     const size_t lds_bytes = sizeof(float) * ((cem == kCeTable ? 768 : 0) + (size_t)2 * 2 * T * 2 * 256 +
-                                              (size_t)(CX + 1) * (T + 1) * 2);
-    if (lds_bytes + sizeof(FSrc) + sizeof(FProbe) > kMaxLds)
+                                              (size_t)(CX + 1) * (T + 1) * 2 +
+                                              (d->tfsf_line_n > 0 ? 2 * (size_t)d->tfsf_line_n : 0));
+    if (lds_bytes + sizeof(FSrc) + sizeof(FProbe) + 2 * sizeof(STfsf) > kMaxLds)
         return fail(MIFDTD_EINVAL, "rows_per_thread %d / planes_per_chunk %d need %zu B of LDS (> %zu)", T, CX, lds_bytes,
                     kMaxLds);
 
@@ -1260,6 +1297,45 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         sdr.i1 = sdr.j1 = sdr.k1 = 0;
     }
 
+    // TFSF plane wave (single steps only)
+    const bool tfsf = d->tfsf_line_n > 0;
+    STfsf tfe{}, tfh{};
+    static_assert(kTfMaxPatch == MIFDTD_TFSF_MAX_PATCH, "TFSF patch tables differ");
+    if (tfsf) {
+        if (pairs) return fail(MIFDTD_EINVAL, "step pairs (e_mid) do not support a TFSF source");
+        if (!d->tfsf_line_e || !d->tfsf_line_h || !d->tfsf_abc) return fail(MIFDTD_ENULL, "TFSF line pointer is NULL");
+        if (d->tfsf_line_n < 6) return fail(MIFDTD_EINVAL, "TFSF line of %d points (at least 6)", d->tfsf_line_n);
+        if (nsteps > 0 && !d->tfsf_src) return fail(MIFDTD_ENULL, "tfsf_src is NULL");
+        const mifdtd_tfsf_patches *P[2] = {&d->tfsf_e, &d->tfsf_h};
+        STfsf *Q[2] = {&tfe, &tfh};
+        for (int h = 0; h < 2; h++) {
+            const mifdtd_tfsf_patches &pp = *P[h];
+            STfsf &q = *Q[h];
+            if (pp.n < 0 || pp.n > kTfMaxPatch) return fail(MIFDTD_EINVAL, "TFSF patch count %d", pp.n);
+            if (pp.axis < 0 || pp.axis > 2 || pp.axis != d->tfsf_e.axis) return fail(MIFDTD_EINVAL, "TFSF axis %d", pp.axis);
+            q.n = pp.n;
+            q.axis = pp.axis;
+            q.line_n = d->tfsf_line_n;
+            q.line = h == 0 ? d->tfsf_line_h : d->tfsf_line_e;  // E patches read the line's H and vice versa
+            for (int r = 0; r < pp.n; r++) {
+                const int *b = pp.box[r];
+                if (pp.comp[r] < 3 * h || pp.comp[r] > 3 * h + 2)
+                    return fail(MIFDTD_EINVAL, "TFSF %s patch %d of component %d", h ? "H" : "E", r, pp.comp[r]);
+                if (b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] >= nx || b[3] >= ny || b[5] >= nz || b[0] > b[1] ||
+                    b[2] > b[3] || b[4] > b[5])
+                    return fail(MIFDTD_EINVAL, "TFSF patch %d outside the grid", r);
+                const int ext = b[2 * pp.axis + 1] - b[2 * pp.axis];
+                if (pp.base[r] < 0 || pp.base[r] + ext >= d->tfsf_line_n)
+                    return fail(MIFDTD_EINVAL, "TFSF patch %d reads outside the line", r);
+                q.comp[r] = pp.comp[r];
+                for (int m = 0; m < 6; m++) q.box[r][m] = b[m];
+                q.base[r] = pp.base[r];
+                q.coef[r] = pp.coef[r];
+            }
+        }
+    }
+    const int feat = (drude ? kFeatDrude : 0) | (tfsf ? kFeatTfsf : 0);
+
     SGeom sg{};
     sg.nx = nx; sg.ny = ny; sg.nzp = nzp; sg.nz = nz;
 
@@ -1329,25 +1405,19 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     };
     // shell E pass over boxes b: E from ei (H from hi) to eo, psi E from pe to pe_out
     auto shell_e = [&](const ShellBoxes &b, unsigned long long tot, const float *const *ei, const float *const *hi,
-                       float *const *eo, float *pe, float *pe_out, const SSrc &ss, const SProbe &sp) -> int {
+                       float *const *eo, float *pe, float *pe_out, const SSrc &ss, const SProbe &sp, int fe) -> int {
         SCpml c = scp;
         c.pe = pe;
         c.pe_out = pe_out;
         const unsigned nbk = (unsigned)((tot + 255) / 256);
         if (!nbk) return 0;
-#define MIFDTD_SHELL_E_LAUNCH(CEV, DRV)                                                                           \
-    k_shell_e<CEV, DRV><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, ei[0], ei[1], ei[2], eo[0], eo[1], eo[2], hi[3],   \
-                                             hi[4], hi[5], d->ce[0], d->ce[1], d->ce[2], d->ce_mat[0], d->ce_mat[1], \
-                                             d->ce_mat[2], d->ce_lut, c, ss, sp, sdr)
-        if (drude) {
-            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable, true);
-            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays, true);
-            else MIFDTD_SHELL_E_LAUNCH(kCeConst, true);
-        } else {
-            if (cem == kCeTable) MIFDTD_SHELL_E_LAUNCH(kCeTable, false);
-            else if (cem == kCeArrays) MIFDTD_SHELL_E_LAUNCH(kCeArrays, false);
-            else MIFDTD_SHELL_E_LAUNCH(kCeConst, false);
-        }
+#define MIFDTD_SHELL_E_LAUNCH(CEV, FV)                                                                            \
+    k_shell_e<CEV, FV><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, ei[0], ei[1], ei[2], eo[0], eo[1], eo[2], hi[3],    \
+                                            hi[4], hi[5], d->ce[0], d->ce[1], d->ce[2], d->ce_mat[0], d->ce_mat[1],  \
+                                            d->ce_mat[2], d->ce_lut, c, ss, sp, sdr, tfe)
+        if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_SHELL_E_LAUNCH, kCeTable, fe);
+        else if (cem == kCeArrays) MIFDTD_BY_FEAT(MIFDTD_SHELL_E_LAUNCH, kCeArrays, fe);
+        else MIFDTD_BY_FEAT(MIFDTD_SHELL_E_LAUNCH, kCeConst, fe);
 #undef MIFDTD_SHELL_E_LAUNCH
         MIFDTD_CHECK_LAUNCH("CPML shell E launch");
         return 0;
@@ -1357,8 +1427,12 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
                        float *const *ho, const SSrc &ss, const SProbe &sp) -> int {
         const unsigned nbk = (unsigned)((tot + 255) / 256);
         if (!nbk) return 0;
-        k_shell_h<<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, en[0], en[1], en[2], hi[3], hi[4], hi[5], ho[3], ho[4],
-                                       ho[5], scp, ss, sp);
+        if (tfsf)
+            k_shell_h<true><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, en[0], en[1], en[2], hi[3], hi[4], hi[5], ho[3],
+                                                 ho[4], ho[5], scp, ss, sp, tfh);
+        else
+            k_shell_h<false><<<nbk, 256, 0, st>>>(sg, b, (unsigned)tot, en[0], en[1], en[2], hi[3], hi[4], hi[5], ho[3],
+                                                  ho[4], ho[5], scp, ss, sp, tfh);
         MIFDTD_CHECK_LAUNCH("CPML shell H launch");
         return 0;
     };
@@ -1391,9 +1465,9 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         SProbe sp1, sp2;
         step_tables(s, fs1, ss1, fp1, sp1);
         step_tables(s + 1, fs2, ss2, fp2, sp2);
-        if (int rc = shell_e(sb2[2], tot2[2], in, in, mid, pin, pout, ss1, sp1)) return rc;
+        if (int rc = shell_e(sb2[2], tot2[2], in, in, mid, pin, pout, ss1, sp1, 0)) return rc;
         if (int rc = shell_h(sb2[1], tot2[1], mid, in, mid, ss1, sp1)) return rc;
-        if (int rc = shell_e(sb2[0], tot2[0], mid, mid, out, pout, pout, ss2, sp2)) return rc;
+        if (int rc = shell_e(sb2[0], tot2[0], mid, mid, out, pout, pout, ss2, sp2, 0)) return rc;
         mark(0);
 #define MIFDTD_FUSED2_LAUNCH(CEV)                                                                                   \
     k_fused2<CEV><<<g2.nblocks, nthr2, lds2, st>>>(g2, in[0], in[1], in[2], in[3], in[4], in[5], mid[0], mid[1],    \
@@ -1418,24 +1492,22 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         FProbe fp;
         SProbe sp;
         step_tables(s, fs, ss, fp, sp);
+        if (tfsf)  // the incident line's E half: the E patches of this step read its H
+            if (int rc = mifdtd_tfsf_line(d->tfsf_line_e, d->tfsf_line_h, d->tfsf_abc, d->tfsf_line_n, 1, d->tfsf_src[s],
+                                          stream))
+                return rc;
         if (pml)
-            if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp)) return rc;
+            if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp, feat)) return rc;
         if (sweep) {
             mark(0);
-#define MIFDTD_FUSED_LAUNCH(CEV, DRV)                                                                             \
-    k_fused<CEV, DRV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0],       \
-                                                          out[1], out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], \
-                                                          d->ce[2], d->ce_mat[0], d->ce_mat[1], d->ce_mat[2],        \
-                                                          d->ce_lut, fs, fp, sdr)
-            if (drude) {
-                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, true);
-                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, true);
-                else MIFDTD_FUSED_LAUNCH(kCeConst, true);
-            } else {
-                if (cem == kCeTable) MIFDTD_FUSED_LAUNCH(kCeTable, false);
-                else if (cem == kCeArrays) MIFDTD_FUSED_LAUNCH(kCeArrays, false);
-                else MIFDTD_FUSED_LAUNCH(kCeConst, false);
-            }
+#define MIFDTD_FUSED_LAUNCH(CEV, FV)                                                                              \
+    k_fused<CEV, FV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
+                                                         out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
+                                                         d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp,  \
+                                                         sdr, tfe, tfh)
+            if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTable, feat);
+            else if (cem == kCeArrays) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeArrays, feat);
+            else MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeConst, feat);
 #undef MIFDTD_FUSED_LAUNCH
             mark(1);
             ++nsweeps;
@@ -1443,6 +1515,9 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         }
         if (pml)
             if (int rc = shell_h(sb, total, out, in, out, ss, sp)) return rc;
+        if (tfsf)  // the line's H half (the H patches of this step read its E)
+            if (int rc = mifdtd_tfsf_line(d->tfsf_line_e, d->tfsf_line_h, d->tfsf_abc, d->tfsf_line_n, 0, 0.f, stream))
+                return rc;
         swap_sets();
         ++swaps;
     }

@@ -103,6 +103,61 @@ __device__ __forceinline__ void drude_store(const SDrude &d, int c, int i, int j
     }
 }
 
+// TFSF face patches of one half step (kuFDTD tfsf.py:138-226 tables, normal incidence):
+// patch p adds 0.5 * (coef * line[base + along]) to component comp on its inclusive box,
+// along being the offset along the propagation axis from the box start; patches apply in
+// table order after the sources.  line: the 1-D incident line of the other half (H for
+// the E patches, E for the H patches).
+constexpr int kTfMaxPatch = 12;
+struct STfsf {
+    int n, axis, line_n;
+    int comp[kTfMaxPatch], box[kTfMaxPatch][6], base[kTfMaxPatch];
+    float coef[kTfMaxPatch];
+    const float *line;
+};
+
+__device__ __forceinline__ void tfsf_add(const STfsf &t, int p, int i, int j, int k0, Q4 &f)
+{
+    const int *b = t.box[p];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        const int k = k0 + l;
+        if (k >= b[4] && k <= b[5]) {
+            const int along = t.axis == 0 ? i - b[0] : t.axis == 1 ? j - b[2] : k - b[4];
+            const float v = 0.5f * (t.coef[p] * t.line[t.base[p] + along]);
+            f.v[l] = f.v[l] + v;
+        }
+    }
+}
+
+// The patches of one half step on the quad (i, j, k0..k0+3); c0: 0 for E, 3 for H; mask:
+// the patches to consider (bit p; patches known not to touch the quad may be left out),
+// applied in table order.
+__device__ __forceinline__ void tfsf_quad(const STfsf &t, int c0, int i, int j, int k0, Q4 &f0, Q4 &f1, Q4 &f2,
+                                          unsigned mask = ~0u)
+{
+    for (int p = 0; p < t.n; ++p) {
+        if (!((mask >> p) & 1u)) continue;
+        const int *b = t.box[p];
+        if (i < b[0] || i > b[1] || j < b[2] || j > b[3]) continue;
+        const int c = t.comp[p] - c0;
+        if (c == 0) tfsf_add(t, p, i, j, k0, f0);
+        else if (c == 1) tfsf_add(t, p, i, j, k0, f1);
+        else tfsf_add(t, p, i, j, k0, f2);
+    }
+}
+
+// Bit p set for each patch of t whose box meets planes [ia, ib], row j and cells [ka, kb].
+__device__ __forceinline__ unsigned tfsf_mask(const STfsf &t, int ia, int ib, int j, int ka, int kb)
+{
+    unsigned m = 0;
+    for (int p = 0; p < t.n; ++p) {
+        const int *b = t.box[p];
+        if (b[0] <= ib && ia <= b[1] && j >= b[2] && j <= b[3] && b[4] <= kb && ka <= b[5]) m |= 1u << p;
+    }
+    return m;
+}
+
 struct SGeom {
     int nx, ny, nzp, nz;
     unsigned Q;        // quads per row
@@ -191,15 +246,19 @@ __device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m,
 // is read from cp.pe and written to cp.pe_out.  CE: CM = 0 the constant 0.5, 1 the float
 // arrays cex.., 2 the uint8 material indices mx.. into tab ([3][256] floats, the same
 // values as the arrays, so the products are identical).
-template <int CM, bool PML, bool DR = false>
+constexpr int kFeatDrude = 1, kFeatTfsf = 2;  // optional E-pass features (template bits)
+
+template <int CM, bool PML, int FEAT = 0>
 __device__ __forceinline__ void e_quad_at(
     const SGeom &g, int i, int j, int k0, const float *exi, const float *eyi, const float *ezi, float *exo,
     float *eyo, float *ezo, const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const SCpml &cp, const SSrc &src, const SProbe &prb, const unsigned char *__restrict__ mx = nullptr,
     const unsigned char *__restrict__ my = nullptr, const unsigned char *__restrict__ mz = nullptr,
-    const float *tab = nullptr, const SDrude *dr = nullptr)
+    const float *tab = nullptr, const SDrude *dr = nullptr, const STfsf *tf = nullptr)
 {
+    constexpr bool DR = (FEAT & kFeatDrude) != 0;
+    constexpr bool TF = (FEAT & kFeatTfsf) != 0;
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
     const size_t id = ((size_t)i * ny + j) * nzp + k0;
     const size_t sx = (size_t)ny * nzp;
@@ -365,6 +424,7 @@ __device__ __forceinline__ void e_quad_at(
     sapply_src(src, 0, i, j, k0, e0);
     sapply_src(src, 1, i, j, k0, e1);
     sapply_src(src, 2, i, j, k0, e2);
+    if (TF) tfsf_quad(*tf, 0, i, j, k0, e0, e1, e2);
 
     // --------------------------------------------------------------- stores
     // (unchanged lanes and wall rows store back the value they loaded)
@@ -423,7 +483,7 @@ template <bool PML>
 __device__ __forceinline__ void h_quad_at(
     const SGeom &g, int i, int j, int k0, const float *__restrict__ ex, const float *__restrict__ ey,
     const float *__restrict__ ez, const float *hxi, const float *hyi, const float *hzi, float *hxo, float *hyo,
-    float *hzo, const SCpml &cp, const SSrc &src, const SProbe &prb)
+    float *hzo, const SCpml &cp, const SSrc &src, const SProbe &prb, const STfsf *tf = nullptr)
 {
     const int nx = g.nx, ny = g.ny, nzp = g.nzp, nz = g.nz;
     (void)nx;
@@ -583,6 +643,7 @@ __device__ __forceinline__ void h_quad_at(
     sapply_src(src, 3, i, j, k0, h0);
     sapply_src(src, 4, i, j, k0, h1);
     sapply_src(src, 5, i, j, k0, h2);
+    if (tf) tfsf_quad(*tf, 3, i, j, k0, h0, h1, h2);
 
     // --------------------------------------------------------------- stores
     st4(hxo + id, h0);

@@ -28,7 +28,8 @@ _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 def in_place_only():
     """Instance types the in-place engine applies through the Fields instance list; the
     whole-step and batched engines take their step from a descriptor and reject them,
-    except those an engine carries itself (the fused engine: one Drude box)."""
+    except those an engine carries itself (the fused engine: one Drude box and one TFSF
+    plane wave)."""
     from .drude import Drude
     from .pbc import Pbc
     from .tfsf import Tfsf
@@ -71,25 +72,46 @@ def _ping_pong_sets(fields):
     return views[:6], views[6:]
 
 
+def _check_tfsf_sources(sim, tfsf):
+    """Sources and TFSF patches share the 'incident' priority; the fused engine applies the
+    sources first, so no source may add to a patch cell of the same component."""
+    for src in sim.sources:
+        comp = _COMP[src.str_f]
+        for P in (tfsf.patches_e, tfsf.patches_h):
+            for q in range(P.n):
+                if P.comp[q] != comp:
+                    continue
+                b = P.box[q]
+                if all(src.pt0[a] <= b[2 * a + 1] and b[2 * a] <= src.pt1[a] for a in range(3)):
+                    raise ValueError(f"a {src.str_f} source on a TFSF face cell; use engine='kemp'")
+
+
 class FusedStepper:
     kernel_name = "k_fused"
 
     def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None):
         from .drude import Drude
+        from .tfsf import Tfsf
         f = sim.fields
-        _check_whole_step(sim, "fused", allow=(Drude,))
+        _check_whole_step(sim, "fused", allow=(Drude, Tfsf))
         drudes = [i for i in f.instance_list if isinstance(i, Drude)]
         if len(drudes) > 1:
             raise ValueError("the fused engine carries one Drude box; use engine='kemp' for more")
         self.drude = drudes[0] if drudes else None
-        if self.drude is not None and step_pairs:
-            raise ValueError("step pairs do not support a Drude box")
+        tfsfs = [i for i in f.instance_list if isinstance(i, Tfsf)]
+        if len(tfsfs) > 1:
+            raise ValueError("the fused engine carries one TFSF source; use engine='kemp' for more")
+        self.tfsf = tfsfs[0] if tfsfs else None
+        if (self.drude is not None or self.tfsf is not None) and step_pairs:
+            raise ValueError("step pairs do not support a Drude box or a TFSF source")
+        if self.tfsf is not None:
+            _check_tfsf_sources(sim, self.tfsf)
         cp = sim.cpml
         self.sim = sim
         npml = cp.npml if cp else 0
         # step pairs: two steps per interior sweep (needs CPML; a third field set holds the
         # first step's shell values)
-        self.pairs = bool(cp is not None and self.drude is None and
+        self.pairs = bool(cp is not None and self.drude is None and self.tfsf is None and
                           (step_pairs if step_pairs is not None else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
         f.ehs, self.alt = _ping_pong_sets(f)
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
@@ -159,6 +181,13 @@ class FusedStepper:
         if self.mid:
             for c in range(6):
                 d.e_mid[c] = self.mid[c].data_ptr()
+        tf = getattr(self, "tfsf", None)
+        if tf is not None:  # tfsf_src is set per call (its values depend on the step count)
+            d.tfsf_e = tf.patches_e
+            d.tfsf_h = tf.patches_h
+            d.tfsf_line_e, d.tfsf_line_h, d.tfsf_abc = (tf.line_e.data_ptr(), tf.line_h.data_ptr(),
+                                                        tf.abc.data_ptr())
+            d.tfsf_line_n = tf.n_line
         dr = getattr(self, "drude", None)
         if dr is not None:
             box = (dr.pt0[0], dr.pt1[0], dr.pt0[1], dr.pt1[1], dr.pt0[2], dr.pt1[2])
@@ -186,6 +215,10 @@ class FusedStepper:
             buf = torch.zeros((nsteps, len(sim.probe_points)), dtype=torch.float32, device=f.device)
         d = self._desc()
         tv = self._tvals(nsteps)
+        tf = getattr(self, "tfsf", None)
+        if tf is not None:  # the line's hard source per step, as Tfsf.update_e computes it
+            tsrc = np.array([np.float32(np.sin(tf.wdt * (tf.tstep + k))) for k in range(max(nsteps, 1))], np.float32)
+            d.tfsf_src = tsrc.ctypes.data
         parity = ctypes.c_int(0)
         args = (ctypes.byref(d), nsteps, tv.ctypes.data, None if buf is None else buf.data_ptr(),
                 ctypes.byref(parity), _lib.stream_handle(f.stream))
@@ -196,6 +229,8 @@ class FusedStepper:
             _lib.call("mifdtd_fused_run", *args)
         for s in sim.sources:
             s.tstep += nsteps
+        if tf is not None:
+            tf.tstep += nsteps
         if parity.value:
             f.ehs, self.alt = self.alt, f.ehs
             if sim.cpml is not None:
@@ -250,6 +285,7 @@ class SlabStepper(FusedStepper):
         self.mid = None
         self.pairs = False
         self.drude = None
+        self.tfsf = None
         self.T = self.CX = 0
         cp = sim.cpml
         self.S = slab_planes or slab_planes_for(f, cp)

@@ -11,6 +11,8 @@ per the face and coefficient tables of ``calc_face_list`` / ``calc_inc_list``
 (``tfsf.py:138-226``).  The box [pt0, pt1] is inclusive; None takes 1 / n-2 as in
 ``tfsf.py:104-108``.  Units are the grid's: dx = 1, light velocity 1, dt = fields.dt,
 wavelength in cells.  The reference's oblique branch is empty; only 'normal' exists.
+The in-place engine runs it from the instance list; the fused engine (one plane wave)
+applies the patches in its sweep and shell passes and advances the line around each step.
 """
 from __future__ import annotations
 

@@ -286,6 +286,17 @@ typedef struct {
     int drude_box[6];
     const float *drude_a[3], *drude_b[3];
     float *drude_j[3], *drude_j_alt[3];
+    /* fused only: a TFSF plane wave (kuFDTD tfsf.py:90-242, normal incidence), or
+     * tfsf_line_n == 0.  Each step: the 1-D line's E half (mifdtd_tfsf_line with the hard
+     * source tfsf_src[s], a HOST array of nsteps values), the step with the E patches
+     * (tfsf_e, reading the line's H) and the H patches (tfsf_h, reading the line's E)
+     * applied after the sources, then the line's H half.  Patch tables as for
+     * mifdtd_tfsf_faces (start[] unused; E patches of components 0..2, H of 3..5).  No
+     * source may share a cell and component with a patch.  Not with e_mid. */
+    mifdtd_tfsf_patches tfsf_e, tfsf_h;
+    float *tfsf_line_e, *tfsf_line_h, *tfsf_abc;
+    int tfsf_line_n;
+    const float *tfsf_src;
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
