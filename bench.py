@@ -331,15 +331,16 @@ def run_batch(args):
 
 def run_pbc(args):
     """Periodic boundaries: the sphere workload of the default run with periodic x and y
-    (Pbc(fields, "xy")) and CPML only on the z faces, on the in-place engine (the engine
-    that carries periodic boundaries).  The same grid is timed first without and then
-    with the periodic boundaries; the dominant kernel is update_e either way."""
+    (Pbc(fields, "xy")) and CPML only on the z faces, on the fused engine (default) or
+    the in-place engine (--engine kemp).  The same grid is timed first without and then
+    with the periodic boundaries."""
     import torch
     from fdtd_accelerate_amd import Pbc, geometry
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse
     n, npml = args.n, args.npml
-    sim = YeeSim(n, n, n, npml=npml, coeff_use="e", cpml_faces=("", "", "fb"))
+    fused = args.engine == "fused"
+    sim = YeeSim(n, n, n, npml=npml, coeff_use="e", cpml_faces=("", "", "fb"), engine="fused" if fused else "kemp")
     lo, hi = (npml,) * 3, (n - npml,) * 3
     rmax = min(40.0, (n - 2 * npml) / 8)
     spheres = geometry.random_spheres(2024, args.spheres, (min(8.0, rmax / 2), rmax), (1.0, 12.0), lo, hi)
@@ -349,9 +350,10 @@ def run_pbc(args):
     sim.add_probe("ez", (c + n // 4, c, c))
     sim.fields.compress_ce()
     torch.cuda.synchronize()
-    ms0, _, _, _, _ = time_kemp(sim, args.steps, args.warmup)
+    runner = time_fused if fused else time_kemp
+    ms0, _, _, _, _ = runner(sim, args.steps, args.warmup)
     Pbc(sim.fields, "xy")
-    ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
+    ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
     cells = n ** 3
     achieved = kbytes * cells / (kern_ms * 1e-3) / 1e9
     return {
@@ -359,7 +361,7 @@ def run_pbc(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, periodic x and y, CPML {npml} on the "
-                               f"z faces, Ez point source", "engine": "kemp", "cells": cells},
+                               f"z faces, Ez point source", "engine": "fused" if fused else "kemp", "cells": cells},
         "ms_per_step_without_periodic": round(ms0, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

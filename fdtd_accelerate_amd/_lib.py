@@ -50,6 +50,7 @@ class FusedDesc(ctypes.Structure):
         ("drude_j_alt", _P * 3),
         ("tfsf_e", TfsfPatches), ("tfsf_h", TfsfPatches), ("tfsf_line_e", _P), ("tfsf_line_h", _P),
         ("tfsf_abc", _P), ("tfsf_line_n", _I), ("tfsf_src", _P),
+        ("cpml_axes", _I), ("npbc", _I), ("pbc_axes", _I * 3),
     ]
 
 

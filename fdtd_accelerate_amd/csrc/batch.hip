@@ -124,6 +124,7 @@ static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, 
                 return fail(MIFDTD_EALIGN, "instance %d: psi buffers NULL or misaligned", n);
             pml = true;
             B.cp.npml = np_;
+            B.cp.axes = 7;
             B.cp.bE = bE;
             B.cp.aE = aE;
             B.cp.bH = bH;

@@ -1157,6 +1157,17 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     if (d->nsrc && !tvals) return fail(MIFDTD_ENULL, "tvals is NULL");
 
     const bool pml = d->npml > 0;
+    if (d->cpml_axes < 0 || d->cpml_axes > 7) return fail(MIFDTD_EINVAL, "cpml_axes %d outside [0,7]", d->cpml_axes);
+    const int pml_axes = pml ? (d->cpml_axes ? d->cpml_axes : 7) : 0;
+    // periodic axes (in the order the copies run)
+    if (d->npbc < 0 || d->npbc > 3) return fail(MIFDTD_EINVAL, "npbc %d outside [0,3]", d->npbc);
+    int periodic = 0;
+    for (int q = 0; q < d->npbc; q++) {
+        const int a = d->pbc_axes[q];
+        if (a < 0 || a > 2 || ((periodic >> a) & 1)) return fail(MIFDTD_EINVAL, "periodic axis %d", a);
+        if ((pml_axes >> a) & 1) return fail(MIFDTD_EINVAL, "axis %d is both periodic and CPML", a);
+        periodic |= 1 << a;
+    }
     SCpml scp{};
     if (pml) {
         const int np_ = d->npml;
@@ -1168,6 +1179,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (!aligned16(d->psiE) || !aligned16(d->psiE_alt) || !aligned16(d->psiH))
             return fail(MIFDTD_EALIGN, "packed psi buffers not 16-byte aligned");
         scp.npml = np_;
+        scp.axes = pml_axes;
         scp.bE = d->bE; scp.aE = d->aE; scp.bH = d->bH; scp.aH = d->aH;
         scp.ph = d->psiH;
         scp.sX = (size_t)np_ * ny * nzp;
@@ -1180,21 +1192,28 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     g.nx = nx; g.ny = ny; g.nzp = nzp; g.nz = nz;
     g.T = T;
     g.cx = CX;
-    if (pml) {
+    {
+        // per axis: inside the CPML slabs; on a periodic axis all but the last plane (whose
+        // E the periodic copy replaces before H reads it); else the whole extent
         const int np_ = d->npml;
-        g.xb = np_ + 1; g.xe = nx - 1 - np_;
-        g.yb = np_ + 1; g.ye = ny - 1 - np_;
-        g.kb = 32 * ((np_ + 1 + 31) / 32); g.ke = 32 * ((nz - 1 - np_) / 32);
+        auto bounds = [&](int a, int n, int &lo, int &hi) {
+            if ((pml_axes >> a) & 1) { lo = np_ + 1; hi = n - 1 - np_; }
+            else if ((periodic >> a) & 1) { lo = 0; hi = n - 1; }
+            else { lo = 0; hi = n; }
+        };
+        bounds(0, nx, g.xb, g.xe);
+        bounds(1, ny, g.yb, g.ye);
+        if ((pml_axes >> 2) & 1) { g.kb = 32 * ((np_ + 1 + 31) / 32); g.ke = 32 * ((nz - 1 - np_) / 32); }
+        else if ((periodic >> 2) & 1) { g.kb = 0; g.ke = 32 * ((nz - 1) / 32); }
+        else { g.kb = 0; g.ke = nzp; }
         if (g.xb >= g.xe || g.yb >= g.ye || g.kb >= g.ke) {  // no interior: the shell is the grid
             g.xb = g.xe = nx;
             g.yb = g.ye = ny;
             g.kb = g.ke = nzp;
         }
-    } else {
-        g.xb = 0; g.xe = nx;
-        g.yb = 0; g.ye = ny;
-        g.kb = 0; g.ke = nzp;
     }
+    // shell passes run whenever the box is not the whole grid
+    const bool shell_on = !(g.xb == 0 && g.xe == nx && g.yb == 0 && g.ye == ny && g.kb == 0 && g.ke == nzp);
     const bool sweep = g.xe > g.xb;
     g.nrb = sweep ? (g.ye - g.yb + T - 1) / T : 0;
     g.nchunks = sweep ? (g.xe - g.xb + CX - 1) / CX : 0;
@@ -1214,7 +1233,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     ShellBoxes sb{};
     unsigned long long total = 0;
     const int Q = nzp / 4;
-    if (pml) {
+    if (shell_on) {
         add_shell(sb, total, g, Q);
         if (total >= (1ull << 32) - 256) return fail(MIFDTD_ERANGE, "CPML shell too large");
         sb.start[sb.n] = (unsigned)total;
@@ -1232,6 +1251,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     bool pairs = false;
     const int R2 = T + 3;
     if (pml && d->e_mid[0]) {
+        if (pml_axes != 7 || periodic) return fail(MIFDTD_EINVAL, "step pairs need CPML on every face and no periodic axis");
         for (int q = 0; q < 6; q++) {
             if (!d->e_mid[q]) return fail(MIFDTD_ENULL, "e_mid buffer is NULL");
             if (!aligned16(d->e_mid[q])) return fail(MIFDTD_EALIGN, "e_mid buffer not 16-byte aligned");
@@ -1496,7 +1516,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             if (int rc = mifdtd_tfsf_line(d->tfsf_line_e, d->tfsf_line_h, d->tfsf_abc, d->tfsf_line_n, 1, d->tfsf_src[s],
                                           stream))
                 return rc;
-        if (pml)
+        if (shell_on)
             if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp, feat)) return rc;
         if (sweep) {
             mark(0);
@@ -1513,8 +1533,18 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             ++nsweeps;
             MIFDTD_CHECK_LAUNCH("fused sweep launch");
         }
-        if (pml)
+        for (int q = 0; q < d->npbc; q++) {  // periodic E copies (priority after the incident corrections)
+            const int a = d->pbc_axes[q];
+            const int c1 = a == 0 ? 1 : 0, c2 = a == 2 ? 1 : 2;  // the E components tangential to axis a
+            if (int rc = mifdtd_pbc(out[c1], out[c2], nx, ny, nzp, pad, a, 1, stream)) return rc;
+        }
+        if (shell_on)
             if (int rc = shell_h(sb, total, out, in, out, ss, sp)) return rc;
+        for (int q = 0; q < d->npbc; q++) {  // periodic H copies
+            const int a = d->pbc_axes[q];
+            const int c1 = a == 0 ? 4 : 3, c2 = a == 2 ? 4 : 5;
+            if (int rc = mifdtd_pbc(out[c1], out[c2], nx, ny, nzp, pad, a, 0, stream)) return rc;
+        }
         if (tfsf)  // the line's H half (the H patches of this step read its E)
             if (int rc = mifdtd_tfsf_line(d->tfsf_line_e, d->tfsf_line_h, d->tfsf_abc, d->tfsf_line_n, 0, 0.f, stream))
                 return rc;

@@ -71,6 +71,7 @@ int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps, con
         if (!aligned16(d->psiE) || !aligned16(d->psiH))
             return fail(MIFDTD_EALIGN, "packed psi buffers not 16-byte aligned");
         cp.npml = np_;
+        cp.axes = 7;
         cp.bE = d->bE;
         cp.aE = d->aE;
         cp.bH = d->bH;

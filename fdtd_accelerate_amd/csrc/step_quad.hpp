@@ -14,6 +14,7 @@ constexpr int kSMaxProbe = 8;
 
 struct SCpml {
     int npml;
+    int axes;                // bit a: CPML on both faces of axis a (x = 1, y = 2, z = 4)
     const float *bE, *aE, *bH, *aH;
     float *pe, *ph;          // packed psi E / psi H
     float *pe_out;           // where the E pass stores psi E (pe for in-place updates)
@@ -286,7 +287,7 @@ __device__ __forceinline__ void e_quad_at(
     int zt[4] = {-1, -1, -1, -1};
     bool anyz = false;
     if (PML) {
-        xt = sslab_e(i, nx, npml, xs, xp);  // x slab planes are all < nx-1
+        xt = (cp.axes & 1) ? sslab_e(i, nx, npml, xs, xp) : -1;  // x slab planes are all < nx-1
         if (xt > 0) {
             const size_t o = ((size_t)xp * ny + j) * nzp + k0;
             ox1 = cp.off(xs, 0) + o;
@@ -294,7 +295,7 @@ __device__ __forceinline__ void e_quad_at(
             px1 = ld4(cp.pe + ox1);
             px2 = ld4(cp.pe + ox2);
         }
-        yt = sslab_e(j, ny, npml, ys, yp);  // y slab rows are all < ny-1
+        yt = (cp.axes & 2) ? sslab_e(j, ny, npml, ys, yp) : -1;  // y slab rows are all < ny-1
         if (yt > 0) {
             const size_t o = ((size_t)i * npml + yp) * nzp + k0;
             oy1 = cp.off(2 + ys, 0) + o;
@@ -302,7 +303,7 @@ __device__ __forceinline__ void e_quad_at(
             py1 = ld4(cp.pe + oy1);
             py2 = ld4(cp.pe + oy2);
         }
-        if (k0 < npml || k0 + 3 >= nz - 1 - npml) {
+        if ((cp.axes & 4) && (k0 < npml || k0 + 3 >= nz - 1 - npml)) {
             const size_t o = ((size_t)i * ny + j) * npml;
 #pragma unroll
             for (int l = 0; l < 4; ++l) {
@@ -510,7 +511,7 @@ __device__ __forceinline__ void h_quad_at(
     int zt[4] = {-1, -1, -1, -1};
     bool anyz = false;
     if (PML) {
-        xt = ip ? sslab_h(i, nx, npml, xs, xp) : -1;  // plane 0 is the wall
+        xt = ip && (cp.axes & 1) ? sslab_h(i, nx, npml, xs, xp) : -1;  // plane 0 is the wall
         if (xt > 0) {
             const size_t o = ((size_t)xp * ny + j) * nzp + k0;
             ppx1 = cp.ph + cp.off(xs, 0) + o;
@@ -518,7 +519,7 @@ __device__ __forceinline__ void h_quad_at(
             px1 = ld4(ppx1);
             px2 = ld4(ppx2);
         }
-        yt = jp ? sslab_h(j, ny, npml, ys, yp) : -1;  // row 0 is the wall
+        yt = jp && (cp.axes & 2) ? sslab_h(j, ny, npml, ys, yp) : -1;  // row 0 is the wall
         if (yt > 0) {
             const size_t o = ((size_t)i * npml + yp) * nzp + k0;
             ppy1 = cp.ph + cp.off(2 + ys, 0) + o;
@@ -526,7 +527,7 @@ __device__ __forceinline__ void h_quad_at(
             py1 = ld4(ppy1);
             py2 = ld4(ppy2);
         }
-        if (k0 < npml || k0 + 3 >= nz - npml) {
+        if ((cp.axes & 4) && (k0 < npml || k0 + 3 >= nz - npml)) {
             const size_t o = ((size_t)i * ny + j) * npml;
 #pragma unroll
             for (int l = 0; l < 4; ++l) {

@@ -28,23 +28,28 @@ _COMP = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
 def in_place_only():
     """Instance types the in-place engine applies through the Fields instance list; the
     whole-step and batched engines take their step from a descriptor and reject them,
-    except those an engine carries itself (the fused engine: one Drude box and one TFSF
-    plane wave)."""
+    except those an engine carries itself (the fused engine: one Drude box, one TFSF plane
+    wave and periodic boundaries)."""
     from .drude import Drude
     from .pbc import Pbc
     from .tfsf import Tfsf
     return ((Pbc, "periodic boundaries"), (Tfsf, "TFSF source"), (Drude, "Drude medium"))
 
 
-def _check_whole_step(sim, engine, allow=()):
+def _check_whole_step(sim, engine, allow=(), cpml_axes=False):
     """Limits shared by the whole-step engines (same descriptor); ``allow``: instance
-    types of ``in_place_only`` this engine carries itself."""
+    types of ``in_place_only`` this engine carries itself; ``cpml_axes``: CPML may cover
+    a subset of the axes (both faces of each or none) instead of all six faces."""
     f = sim.fields
     if f.ch_on:
         raise ValueError(f"the {engine} engine uses CH = 0.5; use engine='kemp' for CH arrays")
     cp = sim.cpml
-    if cp is not None and any(d != "fb" and set(d) != {"f", "b"} for d in cp.apply_direction):
-        raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
+    if cp is not None:
+        both = [set(d) == {"f", "b"} for d in cp.apply_direction]
+        if cpml_axes and any(not b and d for b, d in zip(both, cp.apply_direction)):
+            raise ValueError(f"the {engine} engine needs CPML on both faces of an axis or none; use engine='kemp'")
+        if not cpml_axes and not all(both):
+            raise ValueError(f"the {engine} engine needs CPML on all six faces; use engine='kemp'")
     for cls, what in in_place_only():
         if cls not in allow and any(isinstance(i, cls) for i in f.instance_list):
             raise ValueError(f"the {engine} engine has no {what}; use engine='kemp'")
@@ -72,6 +77,19 @@ def _ping_pong_sets(fields):
     return views[:6], views[6:]
 
 
+def _check_periodic_probes(sim, axes):
+    """The fused engine records probes inside its passes, before the periodic copies: no
+    probe may sit on a copied cell (E tangential to a periodic axis on its last plane, H
+    tangential on its first)."""
+    f = sim.fields
+    tangential = {0: ((1, 2), (4, 5)), 1: ((0, 2), (3, 5)), 2: ((0, 1), (3, 4))}
+    for comp, pt in sim.probe_points:
+        for a in axes:
+            e_c, h_c = tangential[a]
+            if (comp in e_c and pt[a] == f.ns[a] - 1) or (comp in h_c and pt[a] == 0):
+                raise ValueError(f"a probe on a periodic copy ({pt}); use engine='kemp'")
+
+
 def _check_tfsf_sources(sim, tfsf):
     """Sources and TFSF patches share the 'incident' priority; the fused engine applies the
     sources first, so no source may add to a patch cell of the same component."""
@@ -91,9 +109,20 @@ class FusedStepper:
 
     def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None):
         from .drude import Drude
+        from .pbc import Pbc
         from .tfsf import Tfsf
         f = sim.fields
-        _check_whole_step(sim, "fused", allow=(Drude, Tfsf))
+        _check_whole_step(sim, "fused", allow=(Drude, Tfsf, Pbc), cpml_axes=True)
+        # periodic axes in the order the in-place engine copies them (instance order, then
+        # each instance's axes)
+        self.pbc_axes = [ "xyz".index(a) for i in f.instance_list if isinstance(i, Pbc) for a in i.axes]
+        if len(set(self.pbc_axes)) != len(self.pbc_axes):
+            raise ValueError("the fused engine copies each periodic axis once; use engine='kemp'")
+        cp_ = sim.cpml
+        self.cpml_axes = 0 if cp_ is None else sum(1 << a for a, d in enumerate(cp_.apply_direction) if d)
+        if any((self.cpml_axes >> a) & 1 for a in self.pbc_axes):
+            raise ValueError("an axis with CPML cannot be periodic")
+        _check_periodic_probes(sim, self.pbc_axes)
         drudes = [i for i in f.instance_list if isinstance(i, Drude)]
         if len(drudes) > 1:
             raise ValueError("the fused engine carries one Drude box; use engine='kemp' for more")
@@ -111,7 +140,10 @@ class FusedStepper:
         npml = cp.npml if cp else 0
         # step pairs: two steps per interior sweep (needs CPML; a third field set holds the
         # first step's shell values)
-        self.pairs = bool(cp is not None and self.drude is None and self.tfsf is None and
+        full_cpml = cp is not None and self.cpml_axes == 7 and not self.pbc_axes
+        if step_pairs and not full_cpml:
+            raise ValueError("step pairs need CPML on all six faces and no periodic axis")
+        self.pairs = bool(full_cpml and self.drude is None and self.tfsf is None and
                           (step_pairs if step_pairs is not None else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
         f.ehs, self.alt = _ping_pong_sets(f)
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
@@ -181,6 +213,11 @@ class FusedStepper:
         if self.mid:
             for c in range(6):
                 d.e_mid[c] = self.mid[c].data_ptr()
+        d.cpml_axes = getattr(self, "cpml_axes", 0)
+        pa = getattr(self, "pbc_axes", [])
+        d.npbc = len(pa)
+        for q, a in enumerate(pa):
+            d.pbc_axes[q] = a
         tf = getattr(self, "tfsf", None)
         if tf is not None:  # tfsf_src is set per call (its values depend on the step count)
             d.tfsf_e = tf.patches_e
@@ -246,17 +283,27 @@ class FusedStepper:
         f = self.sim.fields
         nx, ny, nz = f.nx, f.ny, f.nz
         cp = self.sim.cpml
-        if cp is None:
-            return 0, nx, 0, ny, 0, f.nz_pitch
-        p = cp.npml
-        if self.pairs if pairs is None else pairs:
+        p = cp.npml if cp is not None else 0
+        if cp is not None and (self.pairs if pairs is None else pairs):
             box = (p + 2, nx - 2 - p, p + 2, ny - 2 - p, 32 * ((p + 33) // 32), 32 * ((nz - 2 - p) // 32))
             if box[1] - box[0] >= 4 and box[3] - box[2] >= 4 and box[5] > box[4]:
                 return box
-        box = (p + 1, nx - 1 - p, p + 1, ny - 1 - p, 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32))
-        if box[0] >= box[1] or box[2] >= box[3] or box[4] >= box[5]:
+        axes = getattr(self, "cpml_axes", 7) if cp is not None else 0
+        per = set(getattr(self, "pbc_axes", []))
+
+        def bounds(a, n):  # inside the CPML slabs; a periodic axis less its last plane
+            if (axes >> a) & 1:
+                return p + 1, n - 1 - p
+            return (0, n - 1) if a in per else (0, n)
+
+        (xb, xe), (yb, ye) = bounds(0, nx), bounds(1, ny)
+        if axes & 4:
+            kb, ke = 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32)
+        else:
+            kb, ke = (0, 32 * ((nz - 1) // 32)) if 2 in per else (0, f.nz_pitch)
+        if xb >= xe or yb >= ye or kb >= ke:
             return nx, nx, ny, ny, f.nz_pitch, f.nz_pitch
-        return box
+        return xb, xe, yb, ye, kb, ke
 
     def sweep_cells(self):
         xb, xe, yb, ye, kb, ke = self.sweep_box()
@@ -286,6 +333,8 @@ class SlabStepper(FusedStepper):
         self.pairs = False
         self.drude = None
         self.tfsf = None
+        self.pbc_axes = []
+        self.cpml_axes = 0 if sim.cpml is None else 7
         self.T = self.CX = 0
         cp = sim.cpml
         self.S = slab_planes or slab_planes_for(f, cp)

@@ -6,7 +6,8 @@ components take, on the first plane, the values of the last plane (the slice tab
 ``K7/util/common_exchange.py:1-29``).  The grid then has period n - 1 along the axis.
 Priority 'pbc': after the core, CPML and source updates of the same half step.  Axes
 are applied in the order given.  Supported by the in-place engine (``Fields`` instance
-list); the whole-step engines reject a grid that carries one.
+list) and by the fused engine (the copies after its E and H passes; its box leaves the
+last plane of a periodic axis to the shell pass); the slab and batched engines reject it.
 """
 from __future__ import annotations
 

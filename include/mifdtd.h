@@ -297,6 +297,15 @@ typedef struct {
     float *tfsf_line_e, *tfsf_line_h, *tfsf_abc;
     int tfsf_line_n;
     const float *tfsf_src;
+    /* fused only: CPML on a subset of axes -- bit a set: both faces of axis a (x = 1,
+     * y = 2, z = 4); 0 with npml > 0 means all three -- and periodic boundaries
+     * (K7/cpu/pbc.py:5-39): pbc_axes[0..npbc) in the order the copies run, after each
+     * step's E (plane n-1 takes plane 0) and H (plane 0 takes plane n-1) as
+     * mifdtd_pbc does.  An axis is CPML or periodic, not both; probes are recorded
+     * before the copies, so none may sit on a copied cell. */
+    int cpml_axes;
+    int npbc;
+    int pbc_axes[3];
 } mifdtd_fused_desc;
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values

@@ -1,8 +1,9 @@
 """Periodic boundaries on the GPU (``Pbc``, ``mifdtd_pbc``) against the float32 oracle:
 random fields (walls and boundary planes included), CE, CPML on the non-periodic faces,
 a point source and probes; every field and probe sample byte-equal after the steps.
-Covers each axis, several axes in both orders, z pads, and the engines that reject a
-periodic grid."""
+Covers each axis, several axes in both orders, z pads, the in-place and the fused engine
+(its box bordering the periodic planes, and shell passes only), and the engines that
+reject a periodic grid."""
 import numpy as np
 import pytest
 import torch
@@ -27,7 +28,7 @@ def _same(a, b):
         (np.ascontiguousarray(b) + np.float32(0)).tobytes()
 
 
-@pytest.mark.parametrize("ns,axes,npml,seg,steps", [
+_CASES = [
     ((20, 18, 24), "x", 4, 64, 9),
     ((18, 21, 20), "y", 3, 64, 8),
     ((16, 18, 21), "z", 4, 16, 10),   # nz 21: pad 3, the z copy uses the real last cell
@@ -35,14 +36,29 @@ def _same(a, b):
     ((20, 22, 26), "yx", 0, 32, 7),
     ((17, 19, 23), "xyz", 0, 16, 12),
     ((18, 16, 30), "zx", 5, 64, 6),
-])
-def test_pbc_matches_oracle(ns, axes, npml, seg, steps):
+]
+# fused engine: the above (no sweep box along a short CPML z: shell passes only) and grids
+# whose sweep box reaches the periodic planes (box 0 .. n-1 on a periodic axis)
+_FUSED = [
+    ((20, 22, 100), "xy", 4, 64, 7, {"rows_per_thread": 3, "planes_per_chunk": 5}),
+    ((24, 20, 130), "x", 3, 64, 8, {"rows_per_thread": 2, "planes_per_chunk": 7}),
+    ((18, 16, 141), "z", 4, 64, 9, None),          # z box 0 .. 128 of nz 141 (pitch 144)
+    ((17, 19, 70), "xyz", 0, 64, 12, {"rows_per_thread": 4, "planes_per_chunk": 64}),
+    ((22, 18, 300), "yx", 5, 64, 6, {"rows_per_thread": 9, "planes_per_chunk": 16}),
+]
+
+
+@pytest.mark.parametrize("ns,axes,npml,seg,steps,engine,fp",
+                         [c + ("kemp", None) for c in _CASES] + [c + ("fused", None) for c in _CASES] +
+                         [c[:5] + ("fused", c[5]) for c in _FUSED])
+def test_pbc_matches_oracle(ns, axes, npml, seg, steps, engine, fp):
     from fdtd_accelerate_amd import Pbc
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse
     nx, ny, nz = ns
     faces = tuple("" if "xyz"[a] in axes else "fb" for a in range(3))
-    sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e", cpml_faces=faces, segment_nbytes=seg)
+    sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e", cpml_faces=faces, segment_nbytes=seg, engine=engine,
+                 fused_params=fp)
     f = sim.fields
     rng = np.random.default_rng(len(axes) * 100 + nz)
     init = [rng.standard_normal(ns).astype(np.float32) for _ in range(6)]
@@ -64,7 +80,9 @@ def test_pbc_matches_oracle(ns, axes, npml, seg, steps):
     o.add_source("ez", src, src, tf)
     o.add_probe("ey", (0, ny // 2, nz // 2))
     o.add_probe("hz", (nx - 1, ny - 1, nz - 1))
-    series = sim.run(steps)
+    series = np.concatenate([sim.run(steps // 2 + 1), sim.run(steps - steps // 2 - 1)])
+    if engine == "fused":
+        assert sim._fused.pbc_axes == ["xyz".index(a) for a in axes]
     o.run(steps)
     assert _same(series, np.array(o.series, np.float32))
     for k, name in enumerate(NAMES):
@@ -81,11 +99,26 @@ def test_whole_step_engines_reject_periodic_grids():
     from fdtd_accelerate_amd import Pbc
     from fdtd_accelerate_amd.batch import BatchedYee
     from fdtd_accelerate_amd.sim import YeeSim
-    for engine in ("fused", "slab"):
-        s = YeeSim(16, 16, 16, npml=0, coeff_use="", engine=engine)
-        Pbc(s.fields, "x")
-        with pytest.raises(ValueError, match="periodic"):
-            s.run(1)
+    s = YeeSim(16, 16, 16, npml=0, coeff_use="", engine="slab")
+    Pbc(s.fields, "x")
+    with pytest.raises(ValueError, match="periodic"):
+        s.run(1)
+    # the fused engine carries periodic axes, but no probe on a copied cell, no one-sided
+    # CPML face and no step pairs with them
+    s = YeeSim(16, 16, 16, npml=0, coeff_use="", engine="fused")
+    Pbc(s.fields, "x")
+    s.add_probe("ey", (15, 3, 3))
+    with pytest.raises(ValueError, match="periodic"):
+        s.run(1)
+    s = YeeSim(16, 16, 16, npml=3, coeff_use="", cpml_faces=("", "f", "fb"), engine="fused")
+    Pbc(s.fields, "x")
+    with pytest.raises(ValueError, match="both faces"):
+        s.run(1)
+    s = YeeSim(16, 16, 64, npml=3, coeff_use="", cpml_faces=("", "fb", "fb"), engine="fused",
+               fused_params={"step_pairs": True})
+    Pbc(s.fields, "x")
+    with pytest.raises(ValueError, match="periodic"):
+        s.run(1)
     s = YeeSim(16, 16, 16, npml=0, coeff_use="")
     Pbc(s.fields, "z")
     with pytest.raises(ValueError, match="periodic"):
