@@ -3,10 +3,9 @@
 // a CPML for it, on the reference's layout: (nx, ny, nz) Fortran order, x contiguous,
 // no padding.
 //
-// One thread per x-quad (4 consecutive x cells of one (j, k) row); nx must be a
-// multiple of 4.  The x-neighbours i-2 .. i+5 of a quad come from the quad itself plus
-// scalar loads of the neighbouring quads; the y/z neighbours +-1, +-2 rows/planes are
-// quad loads.  Per-cell arithmetic keeps the reference's evaluation order:
+// A lane per x-quad (4 consecutive x cells of one (j, k) row; nx must be a multiple of
+// 4), work-groups marching z (below).  Per-cell arithmetic keeps the reference's
+// evaluation order:
 //   H: h -= (cl*ch) * (27*(((a0 - a1) - b0) + b1) - (((a2 - a3) - b2) + b3))
 //   E: e += (dl*ce) * (27*(((a0 - a1) - b0) + b1) - (((a2 - a3) - b2) + b3))
 // so that with -ffp-contract=off the result equals the reference C bit for bit.
@@ -21,150 +20,8 @@ constexpr int S4B = 256;
 constexpr int kS4Rows = 8;     // owned rows per work-group of the z-marching stages (T+3 waves)
 constexpr int kS4Planes = 32;  // planes per z chunk
 
-struct S4Geom {
-    int nx, ny, nz;
-    unsigned Qx;      // quads per x row
-    unsigned nquads;  // Qx * ny * nz
-};
-
-__device__ __forceinline__ Q4 ld4z(const float *p, bool on) { return on ? ld4(p) : splat4(0.f); }
-
-// d/dx(27,-1) of f about the quad: out[l] = 27*((f[i+1]-f[i]) - ...) is assembled by the
-// caller; here the 7 x-neighbours i0-2 .. i0+5 of the quad are gathered.
-struct XWin {
-    float v[8];  // v[m] = f[i0 - 2 + m], m = 0..7  (i0-2 .. i0+5)
-};
-
-__device__ __forceinline__ XWin xwin(const float *f, size_t id, const Q4 &q, bool has_prev, bool has_next)
-{
-    XWin w;
-    w.v[0] = has_prev ? f[id - 2] : 0.f;
-    w.v[1] = has_prev ? f[id - 1] : 0.f;
-#pragma unroll
-    for (int l = 0; l < 4; ++l) w.v[2 + l] = q.v[l];
-    w.v[6] = has_next ? f[id + 4] : 0.f;
-    w.v[7] = has_next ? f[id + 5] : 0.f;
-    return w;
-}
-
-template <bool HC>
-__global__ __launch_bounds__(S4B) void k_s4_h(S4Geom g, float cl,
-                                              const float *__restrict__ ex, const float *__restrict__ ey,
-                                              const float *__restrict__ ez, float *__restrict__ hx,
-                                              float *__restrict__ hy, float *__restrict__ hz,
-                                              const float *__restrict__ cx, const float *__restrict__ cy,
-                                              const float *__restrict__ cz, float ch0)
-{
-    const unsigned f = blockIdx.x * S4B + threadIdx.x;
-    if (f >= g.nquads) return;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const unsigned row = f / g.Qx;
-    const int i0 = (int)(f - row * g.Qx) * 4;
-    const int k = (int)(row / (unsigned)ny);
-    const int j = (int)row - k * ny;
-    if (j > ny - 2 || k > nz - 2) return;  // loop bounds j < ny-1, k < nz-1
-    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-    const size_t id = ((size_t)k * ny + j) * nx + i0;
-    const bool rx = j > 1 && k > 1;  // hx rows
-    const bool ry = k > 1;           // hy rows (i > 1 per lane)
-    const bool rz = j > 1;           // hz rows (i > 1 per lane)
-    const bool hp = i0 > 0, hn = i0 + 4 < nx;
-    const float c27 = 27.f;
-
-    const Q4 ex0 = ld4(ex + id), ey0 = ld4(ey + id), ez0 = ld4(ez + id);
-    // hx: ez(j), ez(j-1), ey(k), ey(k-1), ez(j+1), ez(j-2), ey(k+1), ey(k-2)
-    const Q4 ezjm = ld4z(ez + id - sy, rx), ezjp = ld4z(ez + id + sy, rx), ezj2 = ld4z(ez + id - 2 * sy, rx);
-    const Q4 eykm = ld4z(ey + id - sz, rx), eykp = ld4z(ey + id + sz, rx), eyk2 = ld4z(ey + id - 2 * sz, rx);
-    // hy: ex(k), ex(k-1), ez(i), ez(i-1), ex(k+1), ex(k-2), ez(i+1), ez(i-2)
-    const Q4 exkm = ld4z(ex + id - sz, ry), exkp = ld4z(ex + id + sz, ry), exk2 = ld4z(ex + id - 2 * sz, ry);
-    // hz: ey(i), ey(i-1), ex(j), ex(j-1), ey(i+1), ey(i-2), ex(j+1), ex(j-2)
-    const Q4 exjm = ld4z(ex + id - sy, rz), exjp = ld4z(ex + id + sy, rz), exj2 = ld4z(ex + id - 2 * sy, rz);
-    const XWin wz = xwin(ez, id, ez0, hp && ry, hn && ry);
-    const XWin wy = xwin(ey, id, ey0, hp && rz, hn && rz);
-    Q4 h0 = ld4(hx + id), h1 = ld4(hy + id), h2 = ld4(hz + id);
-    const Q4 c0 = HC ? ld4(cx + id) : splat4(ch0);
-    const Q4 c1 = HC ? ld4(cy + id) : splat4(ch0);
-    const Q4 c2 = HC ? ld4(cz + id) : splat4(ch0);
-
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-        const int i = i0 + l;
-        if (i > nx - 2) continue;  // loop bound i < nx-1
-        if (rx && j < ny - 1 && k < nz - 1)
-            h0.v[l] -= cl * c0.v[l] * (c27 * (ez0.v[l] - ezjm.v[l] - ey0.v[l] + eykm.v[l])
-                                       - (ezjp.v[l] - ezj2.v[l] - eykp.v[l] + eyk2.v[l]));
-        if (i > 1 && ry)
-            h1.v[l] -= cl * c1.v[l] * (c27 * (ex0.v[l] - exkm.v[l] - ez0.v[l] + wz.v[l + 1])
-                                       - (exkp.v[l] - exk2.v[l] - wz.v[l + 3] + wz.v[l]));
-        if (i > 1 && rz)
-            h2.v[l] -= cl * c2.v[l] * (c27 * (ey0.v[l] - wy.v[l + 1] - ex0.v[l] + exjm.v[l])
-                                       - (wy.v[l + 3] - wy.v[l] - exjp.v[l] + exj2.v[l]));
-    }
-    if (rx) st4(hx + id, h0);
-    if (ry) st4(hy + id, h1);
-    if (rz) st4(hz + id, h2);
-}
-
-template <bool HC>
-__global__ __launch_bounds__(S4B) void k_s4_e(S4Geom g, float dl,
-                                              float *__restrict__ ex, float *__restrict__ ey,
-                                              float *__restrict__ ez, const float *__restrict__ hx,
-                                              const float *__restrict__ hy, const float *__restrict__ hz,
-                                              const float *__restrict__ cx, const float *__restrict__ cy,
-                                              const float *__restrict__ cz, float ce0)
-{
-    const unsigned f = blockIdx.x * S4B + threadIdx.x;
-    if (f >= g.nquads) return;
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const unsigned row = f / g.Qx;
-    const int i0 = (int)(f - row * g.Qx) * 4;
-    const int k = (int)(row / (unsigned)ny);
-    const int j = (int)row - k * ny;
-    if (j > ny - 2 || k > nz - 2) return;
-    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-    const size_t id = ((size_t)k * ny + j) * nx + i0;
-    const bool rx = j > 0 && j < ny - 2 && k > 0 && k < nz - 2;  // ex rows
-    const bool ry = k > 0 && k < nz - 2;                         // ey rows (0 < i < nx-2 per lane)
-    const bool rz = j > 0 && j < ny - 2;                         // ez rows (0 < i < nx-2 per lane)
-    const bool hp = i0 > 0, hn = i0 + 4 < nx;
-    const float c27 = 27.f;
-
-    const Q4 hx0 = ld4(hx + id), hy0 = ld4(hy + id), hz0 = ld4(hz + id);
-    // ex: hz(j+1), hz, hy(k+1), hy, hz(j+2), hz(j-1), hy(k+2), hy(k-1)
-    const Q4 hzjp = ld4z(hz + id + sy, rx), hzj2 = ld4z(hz + id + 2 * sy, rx), hzjm = ld4z(hz + id - sy, rx);
-    const Q4 hykp = ld4z(hy + id + sz, rx), hyk2 = ld4z(hy + id + 2 * sz, rx), hykm = ld4z(hy + id - sz, rx);
-    // ey: hx(k+1), hx, hz(i+1), hz, hx(k+2), hx(k-1), hz(i+2), hz(i-1)
-    const Q4 hxkp = ld4z(hx + id + sz, ry), hxk2 = ld4z(hx + id + 2 * sz, ry), hxkm = ld4z(hx + id - sz, ry);
-    // ez: hy(i+1), hy, hx(j+1), hx, hy(i+2), hy(i-1), hx(j+2), hx(j-1)
-    const Q4 hxjp = ld4z(hx + id + sy, rz), hxj2 = ld4z(hx + id + 2 * sy, rz), hxjm = ld4z(hx + id - sy, rz);
-    const XWin wz = xwin(hz, id, hz0, hp && ry, hn && ry);
-    const XWin wy = xwin(hy, id, hy0, hp && rz, hn && rz);
-    Q4 e0 = ld4(ex + id), e1 = ld4(ey + id), e2 = ld4(ez + id);
-    const Q4 c0 = HC ? ld4(cx + id) : splat4(ce0);
-    const Q4 c1 = HC ? ld4(cy + id) : splat4(ce0);
-    const Q4 c2 = HC ? ld4(cz + id) : splat4(ce0);
-
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-        const int i = i0 + l;
-        if (i > nx - 2) continue;
-        if (rx)
-            e0.v[l] += dl * c0.v[l] * (c27 * (hzjp.v[l] - hz0.v[l] - hykp.v[l] + hy0.v[l])
-                                       - (hzj2.v[l] - hzjm.v[l] - hyk2.v[l] + hykm.v[l]));
-        if (i > 0 && i < nx - 2 && ry)
-            e1.v[l] += dl * c1.v[l] * (c27 * (hxkp.v[l] - hx0.v[l] - wz.v[l + 3] + hz0.v[l])
-                                       - (hxk2.v[l] - hxkm.v[l] - wz.v[l + 4] + wz.v[l + 1]));
-        if (i > 0 && i < nx - 2 && rz)
-            e2.v[l] += dl * c2.v[l] * (c27 * (wy.v[l + 3] - hy0.v[l] - hxjp.v[l] + hx0.v[l])
-                                       - (wy.v[l + 4] - wy.v[l + 1] - hxj2.v[l] + hxjm.v[l]));
-    }
-    if (rx) st4(ex + id, e0);
-    if (ry) st4(ey + id, e1);
-    if (rz) st4(ez + id, e2);
-}
-
 // ------------------------------------------------------------------ z-marching stages
-// The same stage updates with the neighbour rows and planes read once per work-group:
+// The stage updates with the neighbour rows and planes read once per work-group:
 // a work-group owns one x segment of 64 quads (a lane per quad) of T rows [j0, j0+T) and
 // marches the planes of a z chunk [k0, k1).  Wave w holds row j0-1+w (E stage) or
 // j0-2+w (H stage), so that the T owned rows have the y halo rows (E: j-1, j+1, j+2;
@@ -172,8 +29,8 @@ __global__ __launch_bounds__(S4B) void k_s4_e(S4Geom g, float dl,
 // two components its neighbours need (E: hz, hx; H: ez, ex) in LDS (double-buffered, one
 // barrier per plane).  The z neighbours (E: hy, hx at k-1 .. k+2; H: ey, ex at k-2 ..
 // k+1) stay in a four-plane register window per owned lane, and the x neighbours come
-// from the adjacent lanes (the segment's edge lanes load theirs).  Arithmetic, guards
-// and stores are those of k_s4_e / k_s4_h, so the values are identical.
+// from the adjacent lanes (the segment's edge lanes load theirs).  Arithmetic and
+// guards are those of the header above, so the values equal the reference's.
 struct S4ZGeom {
     int nx, ny, nz;
     int Qx;   // quads per x row

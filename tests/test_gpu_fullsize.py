@@ -4,12 +4,12 @@ initial fields (every cell, the walls and the CPML slabs live from the first ste
 * the 1024^3 Yee workload (200-sphere dielectric, CPML 12, Ez point source
   and probe): the in-place engine equals the float32 C oracle after 2 steps; the fused
   engine (CE as uint8 material indices, field sets in one staggered allocation) equals
-  the in-place engine (float CE) in every field, psi value and probe sample after 2 and
-  then 5 steps, with single steps and with step pairs;
+  the in-place engine (float CE) in every field, psi value and probe sample after 2, 7
+  and 57 steps (58 with step pairs), with single steps and with step pairs;
 * the 512^3 4th-order symplectic workload (100-sphere dielectric, CPML 10,
   S = 0.9 S_max): equal to the float32 C oracle after 2 steps;
 * the 64-cube uneven batch (CPML 8): every instance of the batched run
-  equals the same instance run alone after 3 steps.
+  equals the same instance run alone after 3 and then 33 steps.
 
 Values are compared with -0 == +0 (the fused kernel's masked lanes may store +0 where
 the in-place arithmetic gives value*0 = -0).  This is the only place the shapes that
@@ -107,6 +107,12 @@ def test_yee_1024cube_spheres_engines_and_oracle():
     assert _equal_host(a, b)
     for name in NAMES:
         assert torch.equal(ref.field(name), fus.field(name)), ("fused, 7 steps", name)
+    # a longer run on both engines (57 steps in all)
+    a = ref.run(50)
+    b = fus.run(50)
+    assert _equal_host(a, b)
+    for name in NAMES:
+        assert torch.equal(ref.field(name), fus.field(name)), ("fused, 57 steps", name)
     for key in ref.cpml.psiE:
         for c in range(2):
             assert torch.equal(ref.cpml.psiE[key][c], fus.cpml.psiE[key][c]), ("psiE", key, c)
@@ -119,8 +125,8 @@ def test_yee_1024cube_spheres_engines_and_oracle():
 def test_yee_1024cube_step_pairs_equal_inplace():
     """Step pairs on the 1024^3 workload (two-step sweep over the box 14 .. 1010 in x and
     y, 32 .. 992 in z, 1776 blocks of 12 waves; shell and fringe passes into the third
-    buffer set): equal to the in-place engine after 2 steps (one pair) and then 5 more
-    (two pairs and a single step), fields, psi and probe series."""
+    buffer set): equal to the in-place engine after 2 steps (one pair), 5 more (two pairs
+    and a single step) and 51 more, fields, psi and probe series."""
     _need(160)
     from fdtd_accelerate_amd.fused import FusedStepper
     ref = _yee("kemp", False)
@@ -131,7 +137,7 @@ def test_yee_1024cube_step_pairs_equal_inplace():
     torch.cuda.synchronize()
     fus._fused = FusedStepper(fus, step_pairs=True)
     assert fus._fused.pairs and fus._fused.sweep_box() == (14, 1010, 14, 1010, 32, 992)
-    for steps in (2, 5):
+    for steps in (2, 5, 51):
         a = ref.run(steps)
         b = fus.run(steps)
         assert _equal_host(a, b), steps
@@ -183,9 +189,9 @@ def test_uneven_64cube_batch_equals_single_runs():
         for x, y in zip(sa.fields.ehs, sb.fields.ehs):
             y.copy_(x)
     torch.cuda.synchronize()
-    series_a = [s.run(3) for s in a]
+    series_a = [np.concatenate([s.run(3), s.run(30)]) for s in a]
     bt = B.BatchedYee(b)
-    series_b = bt.run(3)
+    series_b = [np.concatenate(x) for x in zip(bt.run(3), bt.run(30))]
     torch.cuda.synchronize()
     for n, (sa, sb) in enumerate(zip(a, b)):
         assert _equal_host(series_a[n], series_b[n]), n
