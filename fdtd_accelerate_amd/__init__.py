@@ -8,7 +8,7 @@ C ABI (``include/mifdtd.h``).
 """
 from .util import check_type, check_value
 from .fields import Fields
-from .core import Core
+from .core import Core, CoreSplit
 from .cpml import Cpml, cpml_coefficients, sigma_max
 from .incident import IncidentDirect, GetFields, SetFields
 from .pbc import Pbc
@@ -18,7 +18,7 @@ from .sources import gaussian_pulse
 from . import geometry
 
 __all__ = [
-    "Fields", "Core", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc", "Tfsf", "Drude",
+    "Fields", "Core", "CoreSplit", "Cpml", "IncidentDirect", "GetFields", "SetFields", "Pbc", "Tfsf", "Drude",
     "cpml_coefficients", "sigma_max", "gaussian_pulse", "geometry",
     "check_type", "check_value",
 ]

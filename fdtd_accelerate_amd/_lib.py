@@ -78,6 +78,10 @@ _SIGS = {
     "mifdtd_update_h": (_I, [_I] * 4 + [_P] * 9 + [_P]),
     "mifdtd_update_e_f64": (_I, [_I] * 4 + [_P] * 9 + [_P]),
     "mifdtd_update_h_f64": (_I, [_I] * 4 + [_P] * 9 + [_P]),
+    "mifdtd_update_e_range": (_I, [_I] * 4 + [_LL, _LL] + [_P] * 9 + [_P]),
+    "mifdtd_update_h_range": (_I, [_I] * 4 + [_LL, _LL] + [_P] * 9 + [_P]),
+    "mifdtd_update_e_f64_range": (_I, [_I] * 4 + [_LL, _LL] + [_P] * 9 + [_P]),
+    "mifdtd_update_h_f64_range": (_I, [_I] * 4 + [_LL, _LL] + [_P] * 9 + [_P]),
     "mifdtd_update_e_lut": (_I, [_I] * 4 + [_P] * 10 + [_P]),
     "mifdtd_cpml_face_x_e": (_I, [_I] * 6 + [_P] * 10 + [_P]),
     "mifdtd_cpml_face_x_h": (_I, [_I] * 6 + [_P] * 8 + [_P]),

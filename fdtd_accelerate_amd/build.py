@@ -7,7 +7,8 @@ KEMP core program libraries over it.
   -Xclang -target-feature -Xclang -packed-fp32-ops   no packed-FP32 VALU forms
 It then compiles ``csrc/kemp_program.c`` once per KEMP program (precision, coeff_use,
 pad), as K7/cpu/core.py:52-85 generates one per combination, into
-``kemp/libkemp_<precision>_<coeff>_pad<p>.so``, each linked to libmifdtd.so.
+``kemp/libkemp_<precision>_<coeff>_pad<p>.so``, and once more per combination in the
+split form of K7/cpu/core_split.py (``..._pad<p>_split.so``), each linked to libmifdtd.so.
 """
 from __future__ import annotations
 
@@ -31,9 +32,10 @@ KEMP_PRECISIONS = {"single": (0, 1, 2, 3), "double": (0, 1)}
 KEMP_COEFFS = ("", "e", "h", "eh")
 
 
-def kemp_program_path(precision_float: str, coeff_use: str, pad: int) -> str:
-    """Path of the program library of (precision_float, coeff_use, pad)."""
-    return os.path.join(KEMP_DIR, f"libkemp_{precision_float}_{coeff_use or 'none'}_pad{pad}.so")
+def kemp_program_path(precision_float: str, coeff_use: str, pad: int, split: bool = False) -> str:
+    """Path of the program library of (precision_float, coeff_use, pad), split form or not."""
+    return os.path.join(KEMP_DIR, f"libkemp_{precision_float}_{coeff_use or 'none'}_pad{pad}"
+                                  f"{'_split' if split else ''}.so")
 INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 
 ARCH = os.environ.get("MIFDTD_ARCH", "gfx950")
@@ -77,16 +79,16 @@ def build(verbose: bool = False) -> str:
 
 
 def _compile_program(args):
-    prec, coeff, pad = args
-    out = kemp_program_path(prec, coeff, pad)
+    prec, coeff, pad, split = args
+    out = kemp_program_path(prec, coeff, pad, split)
     cmd = [CC, "-O2", "-std=c99", "-fPIC", "-shared", "-fvisibility=hidden", f"-I{INCLUDE}",
            f"-DKEMP_DOUBLE={int(prec == 'double')}", f"-DKEMP_CE={int('e' in coeff)}",
-           f"-DKEMP_CH={int('h' in coeff)}", f"-DKEMP_PAD={pad}",
+           f"-DKEMP_CH={int('h' in coeff)}", f"-DKEMP_PAD={pad}", f"-DKEMP_SPLIT={int(split)}",
            os.path.join(CSRC, "kemp_program.c"), "-o", out + ".tmp",
            f"-L{PKG}", "-l:libmifdtd.so", "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"KEMP program {prec}/{coeff or 'none'}/pad{pad} failed:\n{r.stderr}")
+        raise RuntimeError(f"KEMP program {prec}/{coeff or 'none'}/pad{pad}/split{int(split)} failed:\n{r.stderr}")
     os.replace(out + ".tmp", out)
     return out
 
@@ -94,7 +96,8 @@ def _compile_program(args):
 def build_kemp_programs() -> list:
     """Compile every KEMP program library (needs libmifdtd.so built)."""
     os.makedirs(KEMP_DIR, exist_ok=True)
-    jobs = [(p, c, pad) for p, pads in KEMP_PRECISIONS.items() for c in KEMP_COEFFS for pad in pads]
+    jobs = [(p, c, pad, sp) for p, pads in KEMP_PRECISIONS.items() for c in KEMP_COEFFS for pad in pads
+            for sp in (False, True)]
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         return list(ex.map(_compile_program, jobs))
 

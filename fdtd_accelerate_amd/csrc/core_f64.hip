@@ -39,13 +39,13 @@ __device__ __forceinline__ D2 ldc(const double *__restrict__ c, size_t id)
 }
 
 __global__ __launch_bounds__(kBlockD) void k_update_e_f64(
-    int nx, int ny, int nzp, int nz, unsigned npairs, unsigned P,
+    int nx, int ny, int nzp, int nz, unsigned p0, unsigned p1, unsigned P,
     double *__restrict__ ex, double *__restrict__ ey, double *__restrict__ ez,
     const double *__restrict__ hx, const double *__restrict__ hy, const double *__restrict__ hz,
     const double *__restrict__ cex, const double *__restrict__ cey, const double *__restrict__ cez)
 {
-    const unsigned f = blockIdx.x * kBlockD + threadIdx.x;
-    if (f >= npairs) return;
+    const unsigned f = p0 + blockIdx.x * kBlockD + threadIdx.x;  // pairs [p0, p1) of the grid
+    if (f >= p1) return;
     const unsigned row = f / P;
     const int k0 = (int)(f - row * P) * 2;
     const int i = (int)(row / (unsigned)ny);
@@ -91,13 +91,13 @@ __global__ __launch_bounds__(kBlockD) void k_update_e_f64(
 }
 
 __global__ __launch_bounds__(kBlockD) void k_update_h_f64(
-    int ny, int nzp, int nz, unsigned npairs, unsigned P,
+    int ny, int nzp, int nz, unsigned p0, unsigned p1, unsigned P,
     const double *__restrict__ ex, const double *__restrict__ ey, const double *__restrict__ ez,
     double *__restrict__ hx, double *__restrict__ hy, double *__restrict__ hz,
     const double *__restrict__ chx, const double *__restrict__ chy, const double *__restrict__ chz)
 {
-    const unsigned f = blockIdx.x * kBlockD + threadIdx.x;
-    if (f >= npairs) return;
+    const unsigned f = p0 + blockIdx.x * kBlockD + threadIdx.x;  // pairs [p0, p1) of the grid
+    if (f >= p1) return;
     const unsigned row = f / P;
     const int k0 = (int)(f - row * P) * 2;
     const int i = (int)(row / (unsigned)ny);
@@ -155,7 +155,8 @@ int check_grid_f64(int nx, int ny, int nzp, int pad)
 }
 
 int launch_core_f64(bool e_half, int nx, int ny, int nzp, int pad, double *ex, double *ey, double *ez, double *hx,
-                    double *hy, double *hz, const double *c0, const double *c1, const double *c2, hipStream_t st)
+                    double *hy, double *hz, const double *c0, const double *c1, const double *c2, hipStream_t st,
+                    long long idx0 = 0, long long nmax = -1)
 {
     if (int rc = check_grid_f64(nx, ny, nzp, pad)) return rc;
     const double *ptrs[6] = {ex, ey, ez, hx, hy, hz};
@@ -170,13 +171,23 @@ int launch_core_f64(bool e_half, int nx, int ny, int nzp, int pad, double *ex, d
     }
     const unsigned P = (unsigned)(nzp / 2);
     const unsigned npairs = (unsigned)nx * ny * P;
-    const unsigned grid = (npairs + kBlockD - 1) / kBlockD;
+    unsigned p0 = 0, p1 = npairs;
+    if (nmax >= 0) {  // the split form's element range, a pair per 2-element vector
+        const long long n = 2ll * npairs;
+        if (idx0 < 0 || idx0 > nmax || nmax > n)
+            return fail(MIFDTD_EINVAL, "element range [%lld, %lld) outside the grid's %lld", idx0, nmax, n);
+        if (idx0 % 2) return fail(MIFDTD_EALIGN, "idx0 %lld does not start a 2-element vector", idx0);
+        p0 = (unsigned)(idx0 / 2);
+        p1 = (unsigned)((nmax + 1) / 2);
+    }
+    if (p1 == p0) return 0;
+    const unsigned grid = (p1 - p0 + kBlockD - 1) / kBlockD;
     const int nz = nzp - pad;
     if (e_half) {
-        k_update_e_f64<<<grid, kBlockD, 0, st>>>(nx, ny, nzp, nz, npairs, P, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        k_update_e_f64<<<grid, kBlockD, 0, st>>>(nx, ny, nzp, nz, p0, p1, P, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         MIFDTD_CHECK_LAUNCH("update_e (double) launch");
     } else {
-        k_update_h_f64<<<grid, kBlockD, 0, st>>>(ny, nzp, nz, npairs, P, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+        k_update_h_f64<<<grid, kBlockD, 0, st>>>(ny, nzp, nz, p0, p1, P, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         MIFDTD_CHECK_LAUNCH("update_h (double) launch");
     }
     return 0;
@@ -204,6 +215,25 @@ int mifdtd_update_h_f64(int nx, int ny, int nz_pitch, int pad, const double *ex,
 {
     return launch_core_f64(false, nx, ny, nz_pitch, pad, const_cast<double *>(ex), const_cast<double *>(ey),
                            const_cast<double *>(ez), hx, hy, hz, chx, chy, chz, to_stream(stream));
+}
+
+int mifdtd_update_e_f64_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax, double *ex,
+                              double *ey, double *ez, const double *hx, const double *hy, const double *hz,
+                              const double *cex, const double *cey, const double *cez, mifdtd_stream_t stream)
+{
+    if (nmax < 0) return fail(MIFDTD_EINVAL, "nmax %lld < 0", nmax);
+    return launch_core_f64(true, nx, ny, nz_pitch, pad, ex, ey, ez, const_cast<double *>(hx), const_cast<double *>(hy),
+                           const_cast<double *>(hz), cex, cey, cez, to_stream(stream), idx0, nmax);
+}
+
+int mifdtd_update_h_f64_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax,
+                              const double *ex, const double *ey, const double *ez, double *hx, double *hy,
+                              double *hz, const double *chx, const double *chy, const double *chz,
+                              mifdtd_stream_t stream)
+{
+    if (nmax < 0) return fail(MIFDTD_EINVAL, "nmax %lld < 0", nmax);
+    return launch_core_f64(false, nx, ny, nz_pitch, pad, const_cast<double *>(ex), const_cast<double *>(ey),
+                           const_cast<double *>(ez), hx, hy, hz, chx, chy, chz, to_stream(stream), idx0, nmax);
 }
 
 }  // extern "C"

@@ -57,13 +57,13 @@ constexpr int kBlock = 256;
 
 template <bool HAS_C>
 __global__ __launch_bounds__(kBlock) void k_update_e(
-    int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
+    int nx, int ny, int nzp, int nz, unsigned q0, unsigned q1, unsigned Q,
     float *__restrict__ ex, float *__restrict__ ey, float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez)
 {
-    const unsigned f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= nquads) return;
+    const unsigned f = q0 + blockIdx.x * kBlock + threadIdx.x;  // quads [q0, q1) of the grid
+    if (f >= q1) return;
     const unsigned row = f / Q;
     const int k0 = (int)(f - row * Q) * 4;
     const int i = (int)(row / (unsigned)ny);
@@ -196,14 +196,14 @@ __global__ __launch_bounds__(kBlock) void k_update_e_lut(
 
 template <bool HAS_C>
 __global__ __launch_bounds__(kBlock) void k_update_h(
-    int nx, int ny, int nzp, int nz, unsigned nquads, unsigned Q,
+    int nx, int ny, int nzp, int nz, unsigned q0, unsigned q1, unsigned Q,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
     const float *__restrict__ chx, const float *__restrict__ chy, const float *__restrict__ chz)
 {
     (void)nx;
-    const unsigned f = blockIdx.x * kBlock + threadIdx.x;
-    if (f >= nquads) return;
+    const unsigned f = q0 + blockIdx.x * kBlock + threadIdx.x;  // quads [q0, q1) of the grid
+    if (f >= q1) return;
     const unsigned row = f / Q;
     const int k0 = (int)(f - row * Q) * 4;
     const int i = (int)(row / (unsigned)ny);
@@ -258,9 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_update_h(
     }
 }
 
+// idx0 / nmax: the element range [idx0, nmax) of the split form (nmax < 0: the whole
+// grid); a quad is updated when its first element lies in the range, as the reference's
+// SSE loop does, so idx0 must start a quad.
 static int launch_core(bool e_half, int nx, int ny, int nzp, int pad,
                        float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
-                       const float *c0, const float *c1, const float *c2, hipStream_t st)
+                       const float *c0, const float *c1, const float *c2, hipStream_t st,
+                       long long idx0 = 0, long long nmax = -1)
 {
     if (int rc = check_grid(nx, ny, nzp, pad)) return rc;
     const float *ptrs[6] = {ex, ey, ez, hx, hy, hz};
@@ -276,19 +280,29 @@ static int launch_core(bool e_half, int nx, int ny, int nzp, int pad,
     }
     const unsigned Q = (unsigned)(nzp / 4);
     const unsigned nquads = (unsigned)nx * ny * Q;
-    const unsigned grid = (nquads + kBlock - 1) / kBlock;
+    unsigned q0 = 0, q1 = nquads;
+    if (nmax >= 0) {
+        const long long n = 4ll * nquads;
+        if (idx0 < 0 || idx0 > nmax || nmax > n)
+            return fail(MIFDTD_EINVAL, "element range [%lld, %lld) outside the grid's %lld", idx0, nmax, n);
+        if (idx0 % 4) return fail(MIFDTD_EALIGN, "idx0 %lld does not start a 4-element vector", idx0);
+        q0 = (unsigned)(idx0 / 4);
+        q1 = (unsigned)((nmax + 3) / 4);
+    }
+    if (q1 == q0) return 0;
+    const unsigned grid = (q1 - q0 + kBlock - 1) / kBlock;
     const int nz = nzp - pad;
     if (e_half) {
         if (has_c)
-            k_update_e<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+            k_update_e<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, q0, q1, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         else
-            k_update_e<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+            k_update_e<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, q0, q1, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         MIFDTD_CHECK_LAUNCH("update_e launch");
     } else {
         if (has_c)
-            k_update_h<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+            k_update_h<true><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, q0, q1, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         else
-            k_update_h<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, nquads, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
+            k_update_h<false><<<grid, kBlock, 0, st>>>(nx, ny, nzp, nz, q0, q1, Q, ex, ey, ez, hx, hy, hz, c0, c1, c2);
         MIFDTD_CHECK_LAUNCH("update_h launch");
     }
     return 0;
@@ -770,6 +784,24 @@ int mifdtd_update_e(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey,
 {
     return launch_core(true, nx, ny, nz_pitch, pad, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
                        const_cast<float *>(hz), cex, cey, cez, to_stream(stream));
+}
+
+int mifdtd_update_e_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax, float *ex, float *ey,
+                          float *ez, const float *hx, const float *hy, const float *hz, const float *cex,
+                          const float *cey, const float *cez, mifdtd_stream_t stream)
+{
+    if (nmax < 0) return fail(MIFDTD_EINVAL, "nmax %lld < 0", nmax);
+    return launch_core(true, nx, ny, nz_pitch, pad, ex, ey, ez, const_cast<float *>(hx), const_cast<float *>(hy),
+                       const_cast<float *>(hz), cex, cey, cez, to_stream(stream), idx0, nmax);
+}
+
+int mifdtd_update_h_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax, const float *ex,
+                          const float *ey, const float *ez, float *hx, float *hy, float *hz, const float *chx,
+                          const float *chy, const float *chz, mifdtd_stream_t stream)
+{
+    if (nmax < 0) return fail(MIFDTD_EINVAL, "nmax %lld < 0", nmax);
+    return launch_core(false, nx, ny, nz_pitch, pad, const_cast<float *>(ex), const_cast<float *>(ey),
+                       const_cast<float *>(ez), hx, hy, hz, chx, chy, chz, to_stream(stream), idx0, nmax);
 }
 
 int mifdtd_update_e_lut(int nx, int ny, int nz_pitch, int pad, float *ex, float *ey, float *ez, const float *hx,

@@ -83,6 +83,33 @@ int mifdtd_update_h_f64(int nx, int ny, int nz_pitch, int pad,
                         const double *chx, const double *chy, const double *chz,
                         mifdtd_stream_t stream);
 
+/* The split form (K7/cpu/src/core_split.c:11,59, bound by K7/cpu/core_split.py:89-123 for
+ * computation/communication overlap): the same update restricted to the flat element
+ * range [idx0, nmax) of the (nx, ny, nz_pitch) arrays, vector by vector as the
+ * reference's SSE loop steps (a 4-element vector -- 2 for double -- is updated when its
+ * first element lies in the range; idx0 must start a vector).  Cells outside the range
+ * are not written. */
+int mifdtd_update_e_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax,
+                          float *ex, float *ey, float *ez,
+                          const float *hx, const float *hy, const float *hz,
+                          const float *cex, const float *cey, const float *cez,
+                          mifdtd_stream_t stream);
+int mifdtd_update_h_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax,
+                          const float *ex, const float *ey, const float *ez,
+                          float *hx, float *hy, float *hz,
+                          const float *chx, const float *chy, const float *chz,
+                          mifdtd_stream_t stream);
+int mifdtd_update_e_f64_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax,
+                              double *ex, double *ey, double *ez,
+                              const double *hx, const double *hy, const double *hz,
+                              const double *cex, const double *cey, const double *cez,
+                              mifdtd_stream_t stream);
+int mifdtd_update_h_f64_range(int nx, int ny, int nz_pitch, int pad, long long idx0, long long nmax,
+                              const double *ex, const double *ey, const double *ez,
+                              double *hx, double *hy, double *hz,
+                              const double *chx, const double *chy, const double *chz,
+                              mifdtd_stream_t stream);
+
 /* update_e with the CE arrays given as per-component uint8 material indices
  * (mat_c, same shape as the fields, 4-byte aligned) into lut (DEVICE, 3 x 256 float32:
  * row c holds the distinct CE values of component c).  Same results as mifdtd_update_e

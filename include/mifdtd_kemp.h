@@ -17,6 +17,16 @@
  * and leaves its reason in mifdtd_last_error() of libmifdtd.so.
  * The declarations below are those of the 'single', coeff 'eh' program; the others
  * drop the coefficient arrays or use double as stated.
+ *
+ * Split form, libkemp_<precision>_<coeff>_pad<p>_split.so: the program that
+ * K7/cpu/core_split.py:8-98 builds from K7/cpu/src/core_split.c:11,59, whose two
+ * symbols take idx0, nmax (c_int) after nz and update only the flat element range
+ * [idx0, nmax) of the (nx, ny, nz_pitch) grid: a 4-float (2-double) vector is updated
+ * when its first element lies in the range, so idx0 must start a vector and a range
+ * outside the grid is rejected.  core_split.py uses the whole grid and its 'pre' /
+ * 'post' plane ranges (E: [ny*nz, 3*ny*nz) / [0, ny*nz); H: [0, 2*ny*nz) /
+ * [2*ny*nz, 3*ny*nz)).  Declared below under MIFDTD_KEMP_SPLIT, since the two forms
+ * share their symbol names.
  */
 #ifndef MIFDTD_KEMP_H
 #define MIFDTD_KEMP_H
@@ -25,12 +35,19 @@
 extern "C" {
 #endif
 
+#ifdef MIFDTD_KEMP_SPLIT
+void update_e(int nx, int ny, int nz, int idx0, int nmax, float *ex, float *ey, float *ez,
+              float *hx, float *hy, float *hz, float *cex, float *cey, float *cez);
+void update_h(int nx, int ny, int nz, int idx0, int nmax, float *ex, float *ey, float *ez,
+              float *hx, float *hy, float *hz, float *chx, float *chy, float *chz);
+#else
 void update_e(int nx, int ny, int nz, float *ex, float *ey, float *ez,
               float *hx, float *hy, float *hz, float *cex, float *cey, float *cez);
 void update_h(int nx, int ny, int nz, float *ex, float *ey, float *ez,
               float *hx, float *hy, float *hz, float *chx, float *chy, float *chz);
+#endif
 
-/* (double << 12) | (ce << 8) | (ch << 4) | pad of the loaded program.  No reference
+/* (split << 16) | (double << 12) | (ce << 8) | (ch << 4) | pad of the loaded program.  No reference
  * counterpart (lets a loader check which program it bound). */
 int mifdtd_kemp_program_id(void);
 

@@ -161,17 +161,18 @@ def ref_available() -> bool:
     return os.path.isdir("/root/reference") or os.path.isdir(_REF_DIR)
 
 
-def ref_core(dtype, pad: int):
-    """The reference's ``update_e/update_h`` (``K7/cpu/src/core.c``) built in place with the
+def ref_core(dtype, pad: int, split: bool = False):
+    """The reference's ``update_e/update_h`` (``K7/cpu/src/core.c``; with ``split`` the
+    ``K7/cpu/src/core_split.c`` form taking idx0, nmax after nz) built in place with the
     masks of the given pad; ``coeff_use='eh'`` signatures (6 fields + 3 coefficient arrays)."""
     sfx = "f32" if np.dtype(dtype) == np.float32 else "f64"
-    path = os.path.join(_REF_DIR, f"libkemp_core_{sfx}_pad{pad}.so")
+    path = os.path.join(_REF_DIR, f"libkemp_{'split' if split else 'core'}_{sfx}_pad{pad}.so")
     if not os.path.exists(path):
         build(ref=True)
     cl = ctypes.CDLL(path)
     for n in ("update_e", "update_h"):
         fn = getattr(cl, n)
-        fn.argtypes = [_I] * 3 + [_P] * 9
+        fn.argtypes = [_I] * (5 if split else 3) + [_P] * 9
         fn.restype = None
     return cl
 

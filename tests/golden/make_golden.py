@@ -12,7 +12,9 @@ digest of the output with its input seed.
 The double file follows the same grid with the reference's 'double' programs (2-lane
 SSE2, nz padded to 2: nz 13 and 14 give pads 1 and 0).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # float
+    python tests/golden/make_golden.py --f64      # double
+    python tests/golden/make_golden.py --split    # split form (core_split.c), float and double
 """
 import hashlib
 import os
@@ -149,8 +151,53 @@ def big_inputs(seed):
     return ehs, ces
 
 
+def split_ranges(nx, ny, nzp, vec):
+    """Element ranges of the split-form cases: the whole grid, K7/cpu/core_split.py's
+    'pre' / 'post' ranges of E and H, a range starting and ending mid-row with nmax off
+    a vector boundary, the last vector, and an empty range."""
+    nyz, n = ny * nzp, nx * ny * nzp
+    assert nx >= 3 and n // 2 + 1 > vec * 7  # every range inside the grid (the reference loop has no bounds check)
+    return [(0, n), (nyz, 3 * nyz), (0, nyz), (0, 2 * nyz), (2 * nyz, 3 * nyz),
+            (vec * 7, n // 2 + 1), (n - vec, n), (vec * 3, vec * 3)]
+
+
+def main_split():
+    """Golden vectors of the reference's split programs (core_split.c): one E and one H
+    call over each range of ``split_ranges``, float pads 3 and 0, double pad 1; the output
+    holds the three updated components (the other three are unchanged inputs)."""
+    O.build(ref=True)
+    rec = {}
+    nx, ny = 3, 4
+    case = 0
+    for dtype, vec, nzs in ((np.float32, 4, (13, 16)), (np.float64, 2, (13,))):
+        for nz in nzs:
+            nzp = (nz + vec - 1) // vec * vec
+            pad = nzp - nz
+            ref = O.ref_core(dtype, pad, split=True)
+            for idx0, nmax in split_ranges(nx, ny, nzp, vec):
+                for ufunc in ("e", "h"):
+                    rng = np.random.default_rng(9000 + case)
+                    ehs = rand_fields(rng, nx, ny, nz, nzp, 6, dtype)
+                    ces, chs = coeffs(rng, "eh", nx, ny, nz, nzp, dtype)
+                    a = [x.copy() for x in ehs]
+                    fn, cs = (ref.update_e, ces) if ufunc == "e" else (ref.update_h, chs)
+                    fn(nx, ny, nzp, idx0, nmax, *[x.ctypes.data for x in a], *[c.ctypes.data for c in cs])
+                    p = f"c{case}_"
+                    rec[p + "meta"] = np.array([nx, ny, nz, nzp, pad, idx0, nmax, int(ufunc == "h"),
+                                                int(dtype == np.float64)], np.int64)
+                    rec[p + "in"] = np.stack(ehs)
+                    rec[p + "coef"] = np.stack(cs)
+                    rec[p + "out"] = np.stack(a[:3] if ufunc == "e" else a[3:])
+                    case += 1
+    rec["ncases"] = np.array(case)
+    np.savez_compressed(os.path.join(HERE, "kemp_core_split.npz"), **rec)
+    print("wrote", case, "split cases")
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["--f64"]:
         main_f64()
+    elif sys.argv[1:] == ["--split"]:
+        main_split()
     else:
         main()

@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from oracle import oracle as O
-from tests.conftest import golden_cases
+from tests.conftest import golden_cases, split_cases, split_expected
 
 pytestmark = pytest.mark.gpu
 
@@ -124,6 +124,109 @@ def test_core_golden_kemp_programs_double():
     import os
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "kemp_core_f64.npz"))
     assert _run_programs(g, "double") == 52
+
+
+def test_core_split_programs_golden():
+    """The split KEMP programs (K7/cpu/src/core_split.c:11,59), bound as
+    K7/cpu/core_split.py:89-98 binds them (5 c_int, 6 arrays, 3 coefficient arrays), over
+    every element range of the reference-built vectors in tests/golden/kemp_core_split.npz
+    (whole grid, 'pre' / 'post' of E and H, a range ending off a vector boundary, the
+    last vector, an empty range; float pads 3 and 0, double pad 1).  The coefficient-free
+    programs run the same ranges against the oracle with CE = CH = 0.5."""
+    from fdtd_accelerate_amd import core
+    n = 0
+    for meta, ehs, coef, out in split_cases():
+        prec = "double" if meta["f64"] else "single"
+        ints = (meta["nx"], meta["ny"], meta["nzp"], meta["idx0"], meta["nmax"])
+        upd, rest = (slice(3, 6), slice(0, 3)) if meta["h"] else (slice(0, 3), slice(3, 6))
+        for coeff_use, want in (("eh", out), ("", split_expected(meta, ehs, None))):
+            lib = core.program(prec, coeff_use, meta["pad"], split=True)
+            d = [dev(x) for x in ehs]
+            dc = [dev(x) for x in coef] if coeff_use else []
+            (lib.update_h if meta["h"] else lib.update_e)(*ints, *[t.data_ptr() for t in d + dc])
+            for c, (g, r) in enumerate(zip(d[upd], want)):
+                assert same_bits(host(g), r), (meta, coeff_use, c)
+            for g, r in zip(d[rest], ehs[rest]):
+                assert same_bits(host(g), r), (meta, coeff_use, "untouched")
+        n += 1
+    assert n == 48
+
+
+def test_core_range_rejects_bad_ranges():
+    """The range entry points refuse, before any launch, a range outside the grid, a
+    reversed range and an idx0 that does not start a vector; the arrays stay untouched."""
+    L = _lib()
+    nx, ny, nzp = 3, 4, 16
+    rng = np.random.default_rng(1)
+    for sfx, dt, vec in (("", np.float32, 4), ("_f64", np.float64, 2)):
+        ehs = [rng.standard_normal((nx, ny, nzp)).astype(dt) for _ in range(6)]
+        d = [dev(x) for x in ehs]
+        n = nx * ny * nzp
+        for idx0, nmax in ((0, n + 1), (vec * 2, vec), (1, n), (-vec, n)):
+            for fn in ("mifdtd_update_e", "mifdtd_update_h"):
+                rc = getattr(L.load(), fn + sfx + "_range")(nx, ny, nzp, 0, idx0, nmax,
+                                                            *[t.data_ptr() for t in d], None, None, None, None)
+                assert rc != 0 and L.last_error(), (sfx, idx0, nmax, fn)
+        for g, r in zip(d, ehs):
+            assert same_bits(host(g), r), sfx
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_core_split_parts_vs_oracle(precision):
+    """CoreSplit (K7/cpu/core_split.py) on a Fields grid with the 64-byte GPU pitch:
+    update_e('pre') touches x planes 1-2 only, then 'post' plane 0; update_h('pre')
+    planes 0-1, then 'post' plane 2; part '' is the whole grid, equal to Core's update."""
+    from fdtd_accelerate_amd import Core, CoreSplit, Fields
+    nx, ny, nz = 6, 7, 19
+    dt = np.float32 if precision == "single" else np.float64
+    rng = np.random.default_rng(5 if precision == "single" else 6)
+    eh = [rng.standard_normal((nx, ny, nz)).astype(dt) for _ in range(6)]
+    ce = [rng.random((nx, ny, nz)).astype(dt) for _ in range(3)]
+
+    def fresh(cls):
+        f = Fields(nx, ny, nz, coeff_use="e", precision_float=precision)
+        f.set_ehs(*eh)
+        f.set_ces(*ce)
+        return f, cls(f)
+
+    f, cs = fresh(CoreSplit)
+    pad = f.pad
+    pz = lambda a, v=0: np.pad(a, ((0, 0), (0, 0), (0, pad)), constant_values=v)
+    start = [pz(x) for x in eh]
+    rce = [pz(x, 0.5) for x in ce]  # Fields keeps CE pads at 0.5
+    full = [x.copy() for x in start]
+    O.update_e(full, rce, pad)
+    O.update_h(full, None, pad)  # H from the fully updated E
+    names = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+    def check(planes, comps, ref):
+        for c in comps:
+            want = start[c].copy()
+            want[list(planes)] = ref[c][list(planes)]
+            assert same_bits(host(f.get(names[c])), want), (precision, names[c], planes)
+
+    cs.update_e("pre")
+    check((1, 2), range(3), full)
+    cs.update_e("post")
+    check((0, 1, 2), range(3), full)
+    # the H update reads the E that planes 0-2 now hold: the oracle's H over that E
+    e3 = [np.where(np.arange(nx)[:, None, None] < 3, full[c], start[c]) for c in range(3)] + \
+        [x.copy() for x in start[3:]]
+    O.update_h(e3, None, pad)
+    cs.update_h("pre")
+    check((0, 1), range(3, 6), e3)
+    cs.update_h("post")
+    check((0, 1, 2), range(3, 6), e3)
+
+    f1, cs1 = fresh(CoreSplit)
+    f2, c2 = fresh(Core)
+    cs1.update_e("")
+    cs1.update_h("")
+    c2.update_e()
+    c2.update_h()
+    for nm in names:
+        assert same_bits(host(f1.get(nm)), host(f2.get(nm))), (precision, nm)
+    assert same_bits(host(f1.get("hz")), full[5]), precision
 
 
 def test_core_double_fields_vs_oracle():

@@ -111,19 +111,22 @@ def test_argument_errors_without_gpu():
 
 
 def test_kemp_programs_export_reference_symbols():
-    """Every KEMP core program library (precision x coeff_use x pad) loads without a GPU,
-    exports update_e/update_h with the reference's arity and names no other program."""
+    """Every KEMP core program library (precision x coeff_use x pad, plain and split form)
+    loads without a GPU, exports update_e/update_h with the reference's arity (3 ints, or 5
+    for the split form of K7/cpu/core_split.py) and names no other program."""
     from fdtd_accelerate_amd import core
     from fdtd_accelerate_amd.build import KEMP_COEFFS, KEMP_PRECISIONS
     n = 0
     for prec, pads in KEMP_PRECISIONS.items():
         for coeff in KEMP_COEFFS:
             for pad in pads:
-                lib = core.program(prec, coeff, pad)
-                assert len(lib.update_e.argtypes) == 3 + (9 if "e" in coeff else 6)
-                assert len(lib.update_h.argtypes) == 3 + (9 if "h" in coeff else 6)
-                n += 1
-    assert n == 24
+                for split in (False, True):
+                    lib = core.program(prec, coeff, pad, split=split)
+                    ni = 5 if split else 3
+                    assert len(lib.update_e.argtypes) == ni + (9 if "e" in coeff else 6)
+                    assert len(lib.update_h.argtypes) == ni + (9 if "h" in coeff else 6)
+                    n += 1
+    assert n == 48
     with pytest.raises(ValueError):
         core.program("double", "e", 2)
     with pytest.raises(ValueError):
@@ -146,6 +149,16 @@ def test_kemp_program_rejects_on_cpu_through_last_error():
     prog = core.program("double", "eh", 1)
     prog.update_h(4, 4, 7, *([None] * 9))
     assert "multiple of 2" in _lib.last_error()
+    # split form: idx0 must start a vector and the range must lie in the grid (checked
+    # before any launch; 64 stands for an aligned device pointer)
+    prog = core.program("single", "", 0, split=True)
+    prog.update_e(4, 4, 8, 2, 100, *([64] * 6))
+    assert "does not start a 4-element vector" in _lib.last_error()
+    prog.update_h(4, 4, 8, 0, 129, *([64] * 6))
+    assert "outside the grid" in _lib.last_error()
+    prog = core.program("double", "h", 1, split=True)
+    prog.update_h(4, 4, 8, 1, 100, *([64] * 9))
+    assert "does not start a 2-element vector" in _lib.last_error()
 
 
 def test_fields_requires_gpu():

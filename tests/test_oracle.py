@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.conftest import golden_cases
+from tests.conftest import golden_cases, split_cases, split_expected
 
 
 # ------------------------------------------------------------------ core vs golden
@@ -51,6 +51,20 @@ def test_oracle_matches_reference_golden_double():
             assert np.array_equal(x, r) and x.tobytes() == r.tobytes(), (meta, c)
         n += 1
     assert n == int(g["ncases"]) == 20
+
+
+def test_oracle_matches_reference_split_golden():
+    """The reference's split programs (K7/cpu/src/core_split.c over element ranges: the
+    whole grid, core_split.py's 'pre' / 'post' ranges, a range ending off a vector
+    boundary, the last vector, an empty range; float pads 3 and 0, double pad 1) equal
+    the oracle's whole-grid update restricted to the vectors that start in the range."""
+    n = 0
+    for meta, ehs, coef, out in split_cases():
+        want = split_expected(meta, ehs, coef)
+        for c, (w, r) in enumerate(zip(want, out)):
+            assert w.tobytes() == r.tobytes(), (meta, c)
+        n += 1
+    assert n == 48
 
 
 def test_oracle_matches_reference_digest(golden):
