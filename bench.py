@@ -18,14 +18,19 @@ source + H update + CPML-H (+ probe gather).
                        checkpoints every 50, adjoint gradient on the 64^3 box; reports
                        backward / forward time)
     python bench.py --workload pbc   (the 1024^3 sphere workload with periodic x and y and
-                       CPML on the z faces, in-place engine; steps timed without and
-                       with the periodic boundaries)
+                       CPML on the z faces; steps timed without and with the periodic
+                       boundaries)
     python bench.py --workload tfsf  (1024^3, CPML 12, a TFSF plane wave along x on the box
-                       [64, n-65]^3 around the sphere dielectric, in-place engine;
-                       steps timed without and with the plane wave)
+                       [64, n-65]^3 around the sphere dielectric; steps timed without
+                       and with the plane wave)
     python bench.py --workload drude (1024^3, CPML 12, a Drude box [256, 767]^3 (wp 0.5,
-                       gamma 0.05) inside the sphere dielectric, in-place engine; steps
-                       timed without and with the Drude medium)
+                       gamma 0.05) inside the sphere dielectric; steps timed without and
+                       with the Drude medium)
+        (pbc, tfsf, drude: fused engine by default, --engine kemp for the in-place one)
+    python bench.py --workload snapshot   (the 1024^3 step with an Ez plane saved every
+                       10 steps to gzip HDF5 files, against the same steps without; and
+                       one whole-grid Ez snapshot: its cost to the step loop and the
+                       time until its file is complete)
     python bench.py --copy-calibration MiB   (plain device copy of known size, for
                                               scaling the HBM byte counters)
 
@@ -79,7 +84,7 @@ def parse():
                    help="fused engine: advance steps in pairs (two-step interior sweep; slower on MI355X)")
     p.add_argument("--slab-planes", type=int, default=None)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
-    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf", "drude"))
+    p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf", "drude", "snapshot"))
     p.add_argument("--instances", type=int, default=64)
     p.add_argument("--sizes", default="64,96,128,160,192", help="batch: cube edges to draw from")
     p.add_argument("--tune", action="store_true", help="batch: autotune launch options first")
@@ -454,6 +459,81 @@ def run_drude(args):
     }
 
 
+def run_snapshot(args):
+    """Snapshots during the step loop: the default sphere workload on the fused engine,
+    K steps without snapshots, then K steps with an Ez z-plane through the source saved
+    every 10 steps (gzip HDF5 files, SaveFields), then one whole-grid Ez snapshot: the
+    time it takes from the step loop (staging copy on the fields' stream) and until its
+    file is complete (device-to-host copy and 16-thread deflate off the step loop)."""
+    import shutil
+    import tempfile
+    import torch
+    from fdtd_accelerate_amd import snapshot
+    a = argparse.Namespace(**vars(args))
+    a.engine = "fused"
+    sim = build_sim(a)
+    n, c = args.n, args.n // 2
+    sim.max_tstep = 10 ** 6
+    st = sim.fields.stream
+    d = tempfile.mkdtemp(prefix="mifdtd_snap_") + "/"
+    try:
+        def timed(k):
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            wall0 = time.perf_counter()
+            t0.record(st)
+            sim.run(k)
+            t1.record(st)
+            torch.cuda.synchronize()
+            sim.wait_savefields()
+            return t0.elapsed_time(t1) / k, time.perf_counter() - wall0
+
+        sim.run(args.warmup)
+        ms0, _ = timed(args.steps)
+        print(f"snapshot bench: {ms0:.3f} ms/step without snapshots", file=sys.stderr, flush=True)
+        plane = sim.set_savefields("ez", (0, 0, c), (-1, -1, c), (0, -1, 10), d, nthreads=16)
+        ms1, wall1 = timed(args.steps)
+        nplane = len(plane.saved)
+        print(f"snapshot bench: {ms1:.3f} ms/step with {nplane} plane snapshots", file=sys.stderr, flush=True)
+        sim.savef_list.clear()
+        whole = snapshot.SaveFields(sim.fields, "ez", (0, 0, 0), (-1, -1, -1), d, 99, 10 ** 6, nslots=1, nthreads=16)
+        sim.run(10)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        wall0 = time.perf_counter()
+        e0.record(st)
+        whole.save_fields(sim.tstep)
+        e1.record(st)
+        sim.run(10)
+        torch.cuda.synchronize()
+        steps_wall = time.perf_counter() - wall0
+        whole.wait()
+        file_s = time.perf_counter() - wall0
+        stage_ms = e0.elapsed_time(e1)
+        path = whole.saved[-1]
+        size = os.path.getsize(path)
+        whole.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    cells = n ** 3
+    return {
+        "metric": METRIC, "value": round(cells / (ms1 * 1e-3) / 1e9, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms1, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain Yee FP32 {n}^3, {args.spheres} spheres, CPML {args.npml} all faces, Ez point "
+                               f"source, fused engine; Ez plane z={c} saved every 10 steps (gzip level 4 HDF5)",
+                   "engine": "fused", "cells": cells},
+        "ms_per_step_without_snapshots": round(ms0, 4),
+        "plane_snapshots": nplane, "plane_loop_wall_s": round(wall1, 3),
+        "whole_grid_snapshot": {"bytes": 4 * cells, "stage_copy_ms": round(stage_ms, 3),
+                                "ten_steps_after_wall_s": round(steps_wall, 3), "file_complete_s": round(file_s, 2),
+                                "file_bytes": size},
+        "device": torch.cuda.get_device_name(0),
+    }
+
+
 ADJ_GOLD = os.path.join(ROOT, "tests", "golden", "adjoint_design160.npz")
 
 
@@ -569,6 +649,9 @@ def main():
         if args.steps == 200:
             args.steps = 300
         print(json.dumps(run_batch(args)), flush=True)
+        return
+    if args.workload == "snapshot":
+        print(json.dumps(run_snapshot(args)), flush=True)
         return
     if args.workload == "drude":
         print(json.dumps(run_drude(args)), flush=True)

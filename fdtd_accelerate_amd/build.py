@@ -9,6 +9,9 @@ It then compiles ``csrc/kemp_program.c`` once per KEMP program (precision, coeff
 pad), as K7/cpu/core.py:52-85 generates one per combination, into
 ``kemp/libkemp_<precision>_<coeff>_pad<p>.so``, and once more per combination in the
 split form of K7/cpu/core_split.py (``..._pad<p>_split.so``), each linked to libmifdtd.so.
+Last it compiles ``csrc/h5io.cpp`` (host code: HDF5 snapshot and geometry files) with the
+host C++ compiler against the HDF5 C library under ``$MIFDTD_HDF5`` (default
+``/opt/conda``) into ``libmifdtd_h5.so``.
 """
 from __future__ import annotations
 
@@ -25,6 +28,9 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmifdtd.so")
 KEMP_DIR = os.path.join(PKG, "kemp")
 CC = os.environ.get("CC", shutil.which("gcc") or "gcc")
+CXX = os.environ.get("CXX", shutil.which("g++") or "g++")
+H5_LIB = os.path.join(PKG, "libmifdtd_h5.so")
+HDF5_ROOT = os.environ.get("MIFDTD_HDF5", "/opt/conda")
 
 # (precision_float, pads) of the reference's programs: single pads nz to 4 floats, double
 # to 2 (K7/cpu/fields.py:27-32 with the 16-byte SSE width).
@@ -73,9 +79,26 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
     progs = build_kemp_programs()
+    h5 = build_h5()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources and {len(progs)} KEMP programs", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources and {len(progs)} KEMP programs"
+              + (f"; {h5}" if h5 else f"; no HDF5 under {HDF5_ROOT}: {H5_LIB} not built"), file=sys.stderr)
     return LIB
+
+
+def build_h5():
+    """Compile ``libmifdtd_h5.so`` (HDF5 snapshot and geometry files); returns its path,
+    or None when the HDF5 C library is not installed (the snapshot module then raises)."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        return None
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-pthread", f"-I{INCLUDE}", f"-I{inc}",
+           os.path.join(CSRC, "h5io.cpp"), "-o", H5_LIB + ".tmp", f"-L{lib}", "-lhdf5", "-lz", f"-Wl,-rpath,{lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 or r.stderr.strip():
+        raise RuntimeError(f"h5io.cpp failed:\n{r.stderr}")
+    os.replace(H5_LIB + ".tmp", H5_LIB)
+    return H5_LIB
 
 
 def _compile_program(args):

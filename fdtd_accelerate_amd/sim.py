@@ -13,7 +13,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, util
 from .core import Core
 from .cpml import Cpml
 from .fields import Fields
@@ -24,7 +24,7 @@ _NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
 
 class YeeSim:
     def __init__(self, nx, ny, nz, npml=0, coeff_use="e", cpml_faces=("fb", "fb", "fb"),
-                 cpml_R=1e-6, cpml_m=4, segment_nbytes=64, engine="kemp", fused_params=None):
+                 cpml_R=1e-6, cpml_m=4, segment_nbytes=64, engine="kemp", fused_params=None, max_tstep=None):
         if engine not in ("kemp", "fused", "slab"):
             raise ValueError(f"engine must be 'kemp', 'fused' or 'slab' : {engine!r}")
         self.fields = Fields(nx, ny, nz, coeff_use=coeff_use, segment_nbytes=segment_nbytes)
@@ -39,6 +39,16 @@ class YeeSim:
         self._fused = None
         self._slab = None
         self._fused_params = fused_params
+        self.max_tstep = max_tstep
+        self.savef_list = []
+
+    @classmethod
+    def from_geometry_h5(cls, geometry_h5_path, max_tstep=None, **kwargs):
+        """A simulation sized by a geometry file: ``coeff_use``, ``nx``, ``ny``, ``nz``
+        read from its root attributes as ``K7/main.py:70-75`` (``Fdtd3d``) reads them."""
+        from .snapshot import read_geometry_h5
+        coeff_use, nx, ny, nz = read_geometry_h5(geometry_h5_path)
+        return cls(nx, ny, nz, coeff_use=coeff_use, max_tstep=max_tstep, **kwargs)
 
     # -------------------------------------------------------------- set-up
     def set_ce(self, cex, cey, cez):
@@ -73,12 +83,55 @@ class YeeSim:
             _lib.call("mifdtd_gather", f.ehs[c].data_ptr(), offs[n:n + 1].data_ptr(), 1,
                       buf[t, n:n + 1].data_ptr(), _lib.stream_handle(f.stream))
 
-    def run(self, nsteps):
-        """Advance ``nsteps`` steps; returns the probe series of these steps (host array
-        (nsteps, nprobes), columns in the order the probes were added) or None."""
+    def set_savefields(self, str_f, pt0, pt1, tstep_range, dir_path, tag=0, max_tstep=None, **options):
+        """Snapshot files of components of the box [pt0, pt1] at the steps ``tstep`` with
+        ``tstep % tgap == 0`` and ``t0 <= tstep <= t1`` (``K7/main.py:239-293`` and its
+        loop ``:315-317``): ``tstep_range = (t0, t1, tgap)`` with t0 and t1 converted by
+        ``convert_index`` over max_tstep and t1 incremented by one, as that file does;
+        a tag already in use becomes one more than the largest; ``divide_info.pkl`` is
+        written to ``dir_path``. Unlike the reference, where the last call's range holds
+        for every SaveFields, each keeps its own range. ``options`` go to ``SaveFields``
+        (``gzip_level``, ``nslots``, ``nthreads``). Returns the ``SaveFields``."""
+        from .snapshot import SaveFields, write_divide_info
+        util.check_type("tstep_range", tstep_range, (list, tuple), int)
+        util.check_type("dir_path", dir_path, str)
+        util.check_type("tag", tag, int)
+        tmax = max_tstep if max_tstep is not None else self.max_tstep
+        if tmax is None:
+            raise ValueError("set_savefields needs max_tstep (YeeSim(max_tstep=...) or the argument)")
+        if len(tstep_range) != 3 or tstep_range[2] < 1:
+            raise ValueError(f"tstep_range must be (t0, t1, tgap) with tgap >= 1 : {tstep_range}")
         f = self.fields
-        nprobe = len(self.probe_points)
-        buf = torch.zeros((nsteps, nprobe), dtype=torch.float32, device=f.device) if nprobe else None
+        p0 = util.convert_indices(f.ns, pt0)
+        p1 = util.convert_indices(f.ns, pt1)
+        t0 = util.convert_index(tmax, tstep_range[0])
+        t1 = util.convert_index(tmax, tstep_range[1]) + 1
+        tgap = tstep_range[2]
+        tags = [sf.tag for sf in self.savef_list]
+        tag = tag if tag not in tags else max(tags) + 1
+        savef = SaveFields(f, str_f, p0, p1, dir_path, tag, tmax, **options)
+        savef.tag, savef.trange = tag, (t0, t1, tgap)
+        self.savef_list.append(savef)
+        write_divide_info(dir_path, util.shape_two_points(p0, p1), p0, p1, f.nx, savef.str_fs, tmax, tgap)
+        return savef
+
+    def _next_save(self, limit):
+        """Steps (1..limit) until the next snapshot step, or limit when none comes sooner."""
+        k = limit
+        for sf in self.savef_list:
+            t0, t1, tgap = sf.trange
+            m = max(self.tstep // tgap + 1, -(-t0 // tgap)) * tgap
+            if m <= t1 and m - self.tstep < k:
+                k = m - self.tstep
+        return k
+
+    def wait_savefields(self):
+        """Block until every snapshot file is written."""
+        for sf in self.savef_list:
+            sf.wait()
+
+    def _advance(self, nsteps, buf):
+        f = self.fields
         if self.engine == "fused":
             from .fused import FusedStepper
             if self._fused is None:
@@ -95,7 +148,24 @@ class YeeSim:
                 f.update_h()
                 if buf is not None:
                     self._record(buf, t)
-        self.tstep += nsteps
+
+    def run(self, nsteps):
+        """Advance ``nsteps`` steps; returns the probe series of these steps (host array
+        (nsteps, nprobes), columns in the order the probes were added) or None. Snapshot
+        steps of ``set_savefields`` end a chunk of steps and enqueue their snapshot."""
+        f = self.fields
+        nprobe = len(self.probe_points)
+        buf = torch.zeros((nsteps, nprobe), dtype=torch.float32, device=f.device) if nprobe else None
+        done = 0
+        while done < nsteps:
+            k = self._next_save(nsteps - done) if self.savef_list else nsteps - done
+            self._advance(k, None if buf is None else buf[done:done + k])
+            done += k
+            self.tstep += k
+            for sf in self.savef_list:
+                t0, t1, tgap = sf.trange
+                if self.tstep % tgap == 0 and t0 <= self.tstep <= t1:
+                    sf.save_fields(self.tstep)
         if buf is None:
             return None
         s = buf.cpu().numpy()
