@@ -92,6 +92,24 @@ __device__ __forceinline__ int tab_h(int q, int n, int npml, int &sd, int &p)
     return -1;
 }
 
+// Reach of the CPML adjoint terms along one axis (coordinate q of n): pass A adds
+// Dt^T(a_H t_H) at an E cell when q or q+1 is an H slab plane (q < npml or
+// q >= n-npml-1), and pass D (inside A) acts on E slab planes, a subset; pass C adds
+// D^T(a_E t_E) at an H cell when q-1 or q is an E slab plane (q <= npml or
+// q >= n-1-npml), and pass B (inside C) acts on H slab planes, a subset.  Elsewhere
+// those terms are zero and are not formed.
+__device__ __forceinline__ bool near_a(int q, int n, int npml) { return q < npml || q >= n - npml - 1; }
+__device__ __forceinline__ bool near_c(int q, int n, int npml) { return q <= npml || q >= n - 1 - npml; }
+
+// Cell (i, j, k) of flat index n (< 2^32: the launch checks it).
+__device__ __forceinline__ void decode(unsigned n, const AGeom &g, int &i, int &j, int &k)
+{
+    const unsigned r = n / (unsigned)g.nzp;
+    k = (int)(n - r * (unsigned)g.nzp);
+    i = (int)(r / (unsigned)g.ny);
+    j = (int)(r - (unsigned)i * (unsigned)g.ny);
+}
+
 // packed psi index of cell (i, j, k) on face (axis, side) at depth p
 __device__ __forceinline__ size_t psi_at(const AGeom &g, int axis, int sd, int p, int c, int i, int j, int k)
 {
@@ -150,14 +168,14 @@ __global__ __launch_bounds__(AB) void k_adj_a(AGeom g, float *__restrict__ lex, 
                                               const float *__restrict__ cez, float *__restrict__ lpE,
                                               const float *__restrict__ bE)
 {
-    const unsigned long long n = (unsigned long long)blockIdx.x * AB + threadIdx.x;
+    const unsigned n = blockIdx.x * AB + threadIdx.x;
     if (n >= g.ncell) return;
-    const int k = (int)(n % (unsigned long long)g.nzp);
-    const unsigned long long r = n / (unsigned long long)g.nzp;
-    const int j = (int)(r % (unsigned long long)g.ny);
-    const int i = (int)(r / (unsigned long long)g.ny);
+    int i, j, k;
+    decode(n, g, i, j, k);
     const size_t id = (size_t)n;
-    if (g.npml > 0) {  // pass D of the later step, on this cell (reads lE before A changes it)
+    const bool nx_ = g.npml > 0 && near_a(i, g.nx, g.npml), ny_ = g.npml > 0 && near_a(j, g.ny, g.npml),
+               nz_ = g.npml > 0 && near_a(k, g.nz, g.npml);
+    if (nx_ || ny_ || nz_) {  // pass D of the later step, on this cell (reads lE before A changes it)
         const float *LEd[3] = {lex, ley, lez};
         const float *CEd[3] = {cex, cey, cez};
         pass_d_cell(g, i, j, k, id, LEd, CEd, lpE, bE);
@@ -170,14 +188,15 @@ __global__ __launch_bounds__(AB) void k_adj_a(AGeom g, float *__restrict__ lex, 
     ay = (lhv(g, LH[0], 0, i, j, k + 1) - lhv(g, LH[0], 0, i, j, k)) - (lhv(g, LH[2], 2, i + 1, j, k) - lhv(g, LH[2], 2, i, j, k));
     az = (lhv(g, LH[1], 1, i + 1, j, k) - lhv(g, LH[1], 1, i, j, k)) - (lhv(g, LH[0], 0, i, j + 1, k) - lhv(g, LH[0], 0, i, j, k));
     float vx = lex[id] - 0.5f * ax, vy = ley[id] - 0.5f * ay, vz = lez[id] - 0.5f * az;
-    if (g.npml > 0) {
-        // x: psi1 -> hy from ez (+), psi2 -> hz from ey (-)
+    if (nx_) {  // x: psi1 -> hy from ez (+), psi2 -> hz from ey (-)
         vz += dtT(g, lhy, lpH, aH, 0, 0, 1, +1.f, i, j, k);
         vy += dtT(g, lhz, lpH, aH, 0, 1, 2, -1.f, i, j, k);
-        // y: psi1 -> hz from ex (+), psi2 -> hx from ez (-)
+    }
+    if (ny_) {  // y: psi1 -> hz from ex (+), psi2 -> hx from ez (-)
         vx += dtT(g, lhz, lpH, aH, 1, 0, 2, +1.f, i, j, k);
         vz += dtT(g, lhx, lpH, aH, 1, 1, 0, -1.f, i, j, k);
-        // z: psi1 -> hx from ey (+), psi2 -> hy from ex (-)
+    }
+    if (nz_) {  // z: psi1 -> hx from ey (+), psi2 -> hy from ex (-)
         vy += dtT(g, lhx, lpH, aH, 2, 0, 0, +1.f, i, j, k);
         vx += dtT(g, lhy, lpH, aH, 2, 1, 1, -1.f, i, j, k);
     }
@@ -270,14 +289,14 @@ __global__ __launch_bounds__(AB) void k_adj_c(AGeom g, float *__restrict__ lex, 
                                               const float *__restrict__ aE, ADesign dz,
                                               float *__restrict__ lpH, const float *__restrict__ bH)
 {
-    const unsigned long long n = (unsigned long long)blockIdx.x * AB + threadIdx.x;
+    const unsigned n = blockIdx.x * AB + threadIdx.x;
     if (n >= g.ncell) return;
-    const int k = (int)(n % (unsigned long long)g.nzp);
-    const unsigned long long r = n / (unsigned long long)g.nzp;
-    const int j = (int)(r % (unsigned long long)g.ny);
-    const int i = (int)(r / (unsigned long long)g.ny);
+    int i, j, k;
+    decode(n, g, i, j, k);
     const size_t id = (size_t)n;
-    if (g.npml > 0) {  // pass B on this cell (reads lH before C changes it)
+    const bool nx_ = g.npml > 0 && near_c(i, g.nx, g.npml), ny_ = g.npml > 0 && near_c(j, g.ny, g.npml),
+               nz_ = g.npml > 0 && near_c(k, g.nz, g.npml);
+    if (nx_ || ny_ || nz_) {  // pass B on this cell (reads lH before C changes it)
         const float *LHb[3] = {lhx, lhy, lhz};
         pass_b_cell(g, i, j, k, id, LHb, lpH, bH);
     }
@@ -292,14 +311,15 @@ __global__ __launch_bounds__(AB) void k_adj_c(AGeom g, float *__restrict__ lex, 
     const float bz = (gev(g, LE[1], CE[1], 1, i, j, k) - gev(g, LE[1], CE[1], 1, i - 1, j, k)) -
                      (gev(g, LE[0], CE[0], 0, i, j, k) - gev(g, LE[0], CE[0], 0, i, j - 1, k));
     float vx = lhx[id] + bx, vy = lhy[id] + by, vz = lhz[id] + bz;
-    if (g.npml > 0) {
-        // x: psi1 -> ey from hz (-CE), psi2 -> ez from hy (+CE)
+    if (nx_) {  // x: psi1 -> ey from hz (-CE), psi2 -> ez from hy (+CE)
         vz += dT(g, ley, cey, lpE, aE, 0, 0, 1, -1.f, i, j, k);
         vy += dT(g, lez, cez, lpE, aE, 0, 1, 2, +1.f, i, j, k);
-        // y: psi1 -> ez from hx (-), psi2 -> ex from hz (+)
+    }
+    if (ny_) {  // y: psi1 -> ez from hx (-), psi2 -> ex from hz (+)
         vx += dT(g, lez, cez, lpE, aE, 1, 0, 2, -1.f, i, j, k);
         vz += dT(g, lex, cex, lpE, aE, 1, 1, 0, +1.f, i, j, k);
-        // z: psi1 -> ex from hy (-), psi2 -> ey from hx (+)
+    }
+    if (nz_) {  // z: psi1 -> ex from hy (-), psi2 -> ey from hx (+)
         vy += dT(g, lex, cex, lpE, aE, 2, 0, 0, -1.f, i, j, k);
         vx += dT(g, ley, cey, lpE, aE, 2, 1, 1, +1.f, i, j, k);
     }
@@ -432,7 +452,7 @@ int mifdtd_adjoint_step(int nx, int ny, int nz_pitch, int pad, float *const lam[
         dz.curl = curl;
         dz.grad = grad;
     }
-    if (g.ncell / AB >= (1ull << 31)) return fail(MIFDTD_ERANGE, "grid too large");
+    if (g.ncell > 0xffffffffull - AB) return fail(MIFDTD_ERANGE, "grid of %llu cells: at most 2^32 - %d", g.ncell, AB);
     const unsigned grid = (unsigned)((g.ncell + AB - 1) / AB);
     hipStream_t st = to_stream(stream);
     k_adj_a<<<grid, AB, 0, st>>>(g, lam[0], lam[1], lam[2], lam[3], lam[4], lam[5], lpsiH, aH, eplane, iplane, c0, c1,

@@ -4,7 +4,10 @@ No reference test pins this path numerically, so the oracle is pinned by what th
 mathematics fixes: the integrator's order conditions and stability limit, exactness
 of the (27, -1)/24 difference on cubic polynomials (which fails on any index, sign or
 stagger mistake), an independent NumPy slice restatement of dielectric.c, and for the
-new CPML: the sigma = 0 no-op and absorption against a larger domain."""
+new CPML: the sigma = 0 no-op, absorption against a larger domain, x->y->z relabelling
+symmetry of the three faces, and each stage call against a restatement from plane
+positions and the core's own curl terms (which fails on a wrong table entry, sign,
+stencil, stage weight or psi slot; checked by planting such mistakes in the oracle)."""
 import math
 
 import numpy as np
@@ -209,3 +212,118 @@ def test_s4_cpml_absorbs():
     peak = np.abs(b).max()
     assert peak > 0
     assert np.abs(a - b).max() < 0.03 * peak, (np.abs(a - b).max(), peak)
+
+
+def test_s4_cpml_cyclic_axis_symmetry():
+    """x->y->z->x relabelling of a cubic 4th-order CPML problem gives the same fields: the
+    three face implementations (components, signs, psi layouts, table offsets) agree."""
+    n, npml, S = 18, 4, 0.6
+    rng = np.random.default_rng(11)
+    init = [rng.standard_normal((n, n, n)) for _ in range(6)]
+    ces = [0.025 * (1.0 + rng.random((n, n, n))) for _ in range(3)]
+    P = lambda x: np.asfortranarray(np.transpose(x, (2, 0, 1)))  # G[k,i,j] = F[i,j,k]
+    a = O.OracleS4Sim(n, n, n, S, npml=npml, dtype=np.float64, ces=ces)
+    b = O.OracleS4Sim(n, n, n, S, npml=npml, dtype=np.float64, ces=[P(ces[2]), P(ces[0]), P(ces[1])])
+    for dst, src in zip(a.ehs, init):
+        dst[:] = src
+    ex, ey, ez, hx, hy, hz = init
+    for dst, src in zip(b.ehs, (P(ez), P(ex), P(ey), P(hz), P(hx), P(hy))):
+        dst[:] = src
+    a.run(3)
+    b.run(3)
+    ex, ey, ez, hx, hy, hz = a.ehs
+    scale = max(np.abs(x).max() for x in a.ehs)
+    for got, want in zip(b.ehs, (P(ez), P(ex), P(ey), P(hz), P(hx), P(hy))):
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale)
+    for p1, p2 in list(a.psiE.values()) + list(a.psiH.values()):
+        assert np.abs(p1).max() > 0 and np.abs(p2).max() > 0
+
+
+# The core's curl terms (Maxwell, as the stages form them): for each updated component and
+# derivative axis, the driving component and the sign of its term.
+_E_TERMS = {(0, 1): (5, +1), (0, 2): (4, -1), (1, 2): (3, +1), (1, 0): (5, -1), (2, 0): (4, +1), (2, 1): (3, -1)}
+_H_TERMS = {(3, 1): (2, -1), (3, 2): (1, +1), (4, 2): (0, -1), (4, 0): (2, +1), (5, 0): (1, -1), (5, 1): (0, +1)}
+
+
+def _sigma_at(x, n, npml, m=4, R=1e-6):
+    """sigma of a plane at position x of an axis of n cells: graded as ((depth)/npml)^m over
+    the front slab x < npml and the back slab x > n-1-npml, zero elsewhere (and beyond a
+    slab's outer edge)."""
+    smax = -(m + 1) * math.log(R) / (2 * npml)
+    back = n - 1 - npml
+    d = npml - x if x < npml else (x - back if x > back else 0.0)
+    return smax * (d / npml) ** m if 0.0 < d <= npml else 0.0
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("stage", [0, 1])
+def test_s4_cpml_stage_against_position_restatement(axis, stage):
+    """One CPML-E and one CPML-H stage call per face side, against a restatement built from
+    positions: E planes sit at q + 1/2 and H planes at q, b = exp(-sigma(x) w S), a = b - 1
+    with w the stage weight (stage 1 has negative weights, b > 1); on the cells the core
+    stage writes (read off the core itself), psi <- b psi + a D with D the core's
+    difference of the driving component along the face axis (forward for E, backward for
+    H), and the component takes w * C * psi with the sign of that curl term."""
+    nx, ny, nz, npml, S = 15, 14, 16, 4, 0.6
+    ns = (nx, ny, nz)
+    rng = np.random.default_rng(100 + 3 * axis + stage)
+    F = lambda: np.asfortranarray(rng.standard_normal(ns))
+    ehs = [F() for _ in range(6)]
+    cs = [np.asfortranarray(0.03 * (1.0 + rng.random(ns))) for _ in range(3)]
+    n = ns[axis]
+    shp = O.s4_psi_shape(axis, nx, ny, nz, npml)
+    for half, w, terms, comps in (("e", O.S4_D[stage], _E_TERMS, (0, 1, 2)), ("h", O.S4_C[stage], _H_TERMS, (3, 4, 5))):
+        # cells the core stage writes
+        probe = [x.copy(order="F") for x in ehs]
+        if half == "e":
+            O.s4_update_e(probe, cs, 0.5)
+        else:
+            O.s4_update_h(probe, cs, 0.5)
+        written = {c: probe[c] != ehs[c] for c in comps}
+        # the oracle's stage, both sides of the face
+        got = [x.copy(order="F") for x in ehs]
+        psis = {sd: (np.asfortranarray(rng.standard_normal(shp)), np.asfortranarray(rng.standard_normal(shp)))
+                for sd in (0, 1)}
+        gpsi = {sd: (p[0].copy(order="F"), p[1].copy(order="F")) for sd, p in psis.items()}
+        tabs = O.cpml_tables(npml, dt=w * S, dtype=np.float64)
+        b_t, a_t = (tabs[0], tabs[1]) if half == "e" else (tabs[2], tabs[3])
+        call = O.s4_cpml_e if half == "e" else O.s4_cpml_h
+        for sd in (0, 1):
+            call(axis, sd, got, cs, gpsi[sd][0], gpsi[sd][1], b_t, a_t, w, npml)
+        # the restatement
+        want = [x.copy(order="F") for x in ehs]
+        wpsi = {sd: (p[0].copy(order="F"), p[1].copy(order="F")) for sd, p in psis.items()}
+        for slot, comp in ((0, comps[(axis + 1) % 3]), (1, comps[(axis + 2) % 3])):
+            drv, sgn = terms[(comp, axis)]
+            f = ehs[drv]
+            f1, fm1 = np.roll(f, -1, axis), np.roll(f, 1, axis)
+            if half == "e":
+                D = 27.0 * (f1 - f) - (np.roll(f, -2, axis) - fm1)
+            else:
+                D = 27.0 * (f - fm1) - (f1 - np.roll(f, 2, axis))
+            ce = cs[comp % 3]
+            for sd in (0, 1):
+                for p in range(npml):
+                    q = (p if sd == 0 else (n - 1 - npml + p if half == "e" else n - npml + p))
+                    x = q + 0.5 if half == "e" else float(q)
+                    sig = _sigma_at(x, n, npml)
+                    assert sig > 0
+                    bb = math.exp(-sig * w * S)
+                    aa = bb - 1.0
+                    cell = [slice(None)] * 3
+                    cell[axis] = q
+                    pl = [slice(None)] * 3
+                    pl[axis] = p
+                    cell, pl = tuple(cell), tuple(pl)
+                    m = written[comp][cell]
+                    psi = wpsi[sd][slot]
+                    new = bb * psi[pl] + aa * D[cell]
+                    psi[pl] = np.where(m, new, psi[pl])
+                    want[comp][cell] = np.where(m, want[comp][cell] + sgn * ((w * ce[cell]) * psi[pl]), want[comp][cell])
+        for c in range(6):
+            np.testing.assert_allclose(got[c], want[c], rtol=1e-14, atol=1e-15, err_msg=f"{half} comp {c}")
+        for sd in (0, 1):
+            for k in (0, 1):
+                np.testing.assert_allclose(gpsi[sd][k], wpsi[sd][k], rtol=1e-14, atol=1e-15,
+                                           err_msg=f"{half} side {sd} psi{k + 1}")
+        assert any(np.any(got[c] != ehs[c]) for c in comps)
