@@ -93,3 +93,39 @@ def test_geometry_h5_sizes_a_simulation(tmp_path):
     sim = YeeSim.from_geometry_h5(p, max_tstep=100, npml=4)
     assert sim.fields.ns == [30, 28, 26] and sim.fields.ce_on and not sim.fields.ch_on
     assert sim.max_tstep == 100
+
+
+def test_savefields_double_fields(tmp_path):
+    """SaveFields on a float64 Fields (KEMP's precision_float='double', core updates only):
+    float64 datasets equal to the float64 oracle at their steps."""
+    from fdtd_accelerate_amd import Core, Fields
+    from fdtd_accelerate_amd.snapshot import SaveFields, dataset_info, read_dataset
+    nx, ny, nz = 10, 12, 30
+    f = Fields(nx, ny, nz, coeff_use="e", precision_float="double")
+    rng = np.random.default_rng(21)
+    eh = [rng.standard_normal((nx, ny, nz)) for _ in range(6)]
+    ce = [rng.random((nx, ny, nz)) for _ in range(3)]
+    f.set_ehs(*eh)
+    f.set_ces(*ce)
+    Core(f)
+    d = str(tmp_path) + "/"
+    sf = SaveFields(f, ["ex", "hz"], (0, 0, 0), (nx - 1, ny - 1, nz - 1), d, 2, 100)
+    pad = f.pad
+    pz = lambda a, v=0: np.pad(a, ((0, 0), (0, 0), (0, pad)), constant_values=v)
+    ref = [pz(x) for x in eh]
+    rce = [pz(x, 0.5) for x in ce]
+    want = {}
+    for t in range(1, 7):
+        f.update_e()
+        f.update_h()
+        O.update_e(ref, rce, pad)
+        O.update_h(ref, None, pad)
+        if t % 3 == 0:
+            sf.save_fields(t)
+            want[t] = (ref[0][:, :, :nz].copy(), ref[5][:, :, :nz].copy())
+    sf.close()
+    for t, (wx, wz) in want.items():
+        p = d + "%.3d_tag2.h5" % t
+        assert dataset_info(p, "ex")[1] == np.float64
+        assert read_dataset(p, "ex").tobytes() == wx.tobytes()
+        assert read_dataset(p, "hz").tobytes() == wz.tobytes()
