@@ -28,6 +28,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -147,7 +148,7 @@ int write_dataset(hid_t file, const char *name, const void *data, int f64, int r
     bool abort = false;
     int zerr = Z_OK;
 
-    auto worker = [&]() {
+    auto work = [&]() {
         std::vector<unsigned char> raw(raw_bytes);
         for (;;) {
             size_t idx;
@@ -177,8 +178,31 @@ int write_dataset(hid_t file, const char *name, const void *data, int f64, int r
             if (zr != Z_OK) cv_space.notify_all();
         }
     };
+    auto worker = [&]() {  // an allocation failure stops the write instead of the process
+        try {
+            work();
+        } catch (...) {
+            {
+                std::lock_guard<std::mutex> lk(m);
+                abort = true;
+                zerr = Z_MEM_ERROR;
+            }
+            cv_ready.notify_all();
+            cv_space.notify_all();
+        }
+    };
     std::vector<std::thread> pool;
-    for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+    try {
+        for (int t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+    } catch (...) {  // thread creation failed: stop the ones started
+        {
+            std::lock_guard<std::mutex> lk(m);
+            abort = true;
+        }
+        cv_space.notify_all();
+        for (auto &t : pool) t.join();
+        return fail(MIFDTD_H5_EIO, "%s: cannot start %d compression threads", name, nthreads);
+    }
 
     int rc = 0;
     for (size_t i = 0; i < total; ++i) {
@@ -265,10 +289,14 @@ int mifdtd_h5_write_fields(const char *path, int ndset, const char *const *names
     if (int rc = check_dims(rank, dims, hd)) return rc;
     for (int n = 0; n < ndset; ++n)
         if (!names[n] || !data[n]) return fail(MIFDTD_H5_EINVAL, "dataset %d: null name or data", n);
-    Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
-    if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot create %s", path);
-    for (int n = 0; n < ndset; ++n)
-        if (int rc = write_dataset(file.id, names[n], data[n], f64, rank, hd, gzip_level, nthreads)) return rc;
+    try {
+        Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+        if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot create %s", path);
+        for (int n = 0; n < ndset; ++n)
+            if (int rc = write_dataset(file.id, names[n], data[n], f64, rank, hd, gzip_level, nthreads)) return rc;
+    } catch (const std::exception &e) {  // no C++ exception crosses the C interface
+        return fail(MIFDTD_H5_EIO, "writing %s: %s", path, e.what());
+    }
     return 0;
 }
 

@@ -212,7 +212,7 @@ class SaveFields:
         path = self.fpath_form % tstep
         arrays = {name: host[c].numpy() for c, name in enumerate(self.str_fs)}
 
-        def write():
+        def write(stage=stage, host=host):  # both stay referenced until their copy has completed
             copied.synchronize()
             write_fields(path, arrays, self.gzip_level, self.nthreads)
             return path
