@@ -141,6 +141,7 @@ __device__ __forceinline__ void tfsf_quad(const STfsf &t, int c0, int i, int j, 
         if (!((mask >> p) & 1u)) continue;
         const int *b = t.box[p];
         if (i < b[0] || i > b[1] || j < b[2] || j > b[3]) continue;
+        if (k0 + 3 < b[4] || k0 > b[5]) continue;  // the quad misses the patch (z-face patches: all lanes but one)
         const int c = t.comp[p] - c0;
         if (c == 0) tfsf_add(t, p, i, j, k0, f0);
         else if (c == 1) tfsf_add(t, p, i, j, k0, f1);
