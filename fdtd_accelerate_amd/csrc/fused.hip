@@ -44,6 +44,10 @@
 namespace mifdtd {
 
 constexpr int kMaxRows = 15;  // owned rows per block (a block has rows+1 waves, at most 1024 threads)
+// Up to this many rows the sweep is compiled for its real block of rows+1 <= 10 waves:
+// one such work-group per CU either way, and the register budget of 3 waves per SIMD
+// (168 VGPRs) holds the kernel without the scratch spill of a 1024-thread bound.
+constexpr int kNarrowRows = 9;
 constexpr int kDrudeMaxRows = 7;  // with a Drude box: rows+1 waves at 2 per SIMD (up to 256 VGPRs)
 constexpr size_t kMaxLds = 160 * 1024;  // LDS of one work-group on gfx950
 constexpr int kMaxSrc = 4;
@@ -285,8 +289,9 @@ __device__ __forceinline__ void compute_H(const FGeom &g, const FSrc *src, int i
 // plane's loads issue together.  The LDS exchange is ordered by the barrier and the
 // parity.  Source and probe tables are copied to LDS once per block and consulted only
 // by blocks whose rows they touch.
-template <int CEM, int FEAT>
-__global__ __launch_bounds__((FEAT & kFeatDrude) ? 64 * (kDrudeMaxRows + 1) : 1024, (FEAT & kFeatDrude) ? 2 : 4) void k_fused(
+template <int CEM, int FEAT, bool WIDE>
+__global__ __launch_bounds__((FEAT & kFeatDrude) ? 64 * (kDrudeMaxRows + 1) : WIDE ? 1024 : 64 * (kNarrowRows + 1),
+                             (FEAT & kFeatDrude) ? 2 : WIDE ? 4 : 3) void k_fused(
     FGeom g,
     const float *__restrict__ ex, const float *__restrict__ ey, const float *__restrict__ ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
@@ -1471,6 +1476,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         }
     };
     const unsigned nthr = 64u * (unsigned)(T + 1);
+    const bool wide = T > kNarrowRows;  // blocks of more than 10 waves: the 1024-thread build
     const unsigned nthr2 = 64u * (unsigned)R2;
     int swaps = 0;
     int s = 0;
@@ -1520,15 +1526,21 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
             if (int rc = shell_e(sb, total, in, in, out, pin, pout, ss, sp, feat)) return rc;
         if (sweep) {
             mark(0);
+#define MIFDTD_FUSED_LAUNCH_W(CEV, FV, W)                                                                            \
+    k_fused<CEV, FV, W><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
+                                                            out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
+                                                            d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp,  \
+                                                            sdr, tfe, tfh)
 #define MIFDTD_FUSED_LAUNCH(CEV, FV)                                                                              \
-    k_fused<CEV, FV><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
-                                                         out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
-                                                         d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp,  \
-                                                         sdr, tfe, tfh)
+    do {                                                                                                          \
+        if (((FV) & kFeatDrude) || !wide) MIFDTD_FUSED_LAUNCH_W(CEV, FV, false);                                  \
+        else MIFDTD_FUSED_LAUNCH_W(CEV, FV, !((FV) & kFeatDrude));                                                \
+    } while (0)
             if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTable, feat);
             else if (cem == kCeArrays) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeArrays, feat);
             else MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeConst, feat);
 #undef MIFDTD_FUSED_LAUNCH
+#undef MIFDTD_FUSED_LAUNCH_W
             mark(1);
             ++nsweeps;
             MIFDTD_CHECK_LAUNCH("fused sweep launch");
