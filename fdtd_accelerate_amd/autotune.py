@@ -107,6 +107,8 @@ def tune_fused(sim, candidates=((7, 64), (9, 64), (9, 96), (11, 64), (11, 128), 
         results[(T, CX)] = ms
         if best is None or ms < results[best]:
             best = (T, CX)
+    if best is None:
+        raise ValueError(f"no candidate tile fits the 160 KiB of LDS : {tuple(candidates)}")
     key = (f.nx, f.ny, f.nz_pitch, sim.cpml.npml if sim.cpml else 0)
     _load()[key] = best
     if save:
@@ -150,6 +152,8 @@ def tune_slab(sim, candidates=(1, 2, 3, 4, 6, 8, 12, 16, 32), steps=5, save=True
         st = SlabStepper(sim, slab_planes=S)
         st.run(1, None)
         results[S] = st.run(steps, None, time_kernel=True)
+    if not results:
+        raise ValueError(f"no candidate slab depth is at most nx = {f.nx} : {tuple(candidates)}")
     best = min(results, key=results.get)
     key = (f.nx, f.ny, f.nz_pitch, sim.cpml.npml if sim.cpml else 0, int(f.ce_on))
     _load_slab()[key] = best

@@ -204,6 +204,8 @@ def _load_table():
 def tune(batch, candidates=((False, 0), (True, 0), (False, 4), (True, 4), (False, 16), (True, 16)), steps=10,
          save=True):
     """Time each (use_graph, remap_rows) on the batch (advancing it) and keep the best."""
+    if not candidates:
+        raise ValueError("no (use_graph, remap_rows) candidates to time")
     res = {}
     for g, r in candidates:
         batch.use_graph, batch.remap_rows = g, r
