@@ -68,7 +68,7 @@ MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
 MEASURED_S4_TRAFFIC_GB = {(512, 10): 5.81}
 # The batched step (both launches of one step, all instances), same method, keyed by
 # (instances, cube edges drawn from); profiles/k_batch_uneven_64cubes_kernel_trace.txt.
-MEASURED_BATCH_TRAFFIC_GB = {(64, "64,96,128,160,192"): 18.38}
+MEASURED_BATCH_TRAFFIC_GB = {(64, "64,96,128,160,192"): 16.86}
 
 
 def parse():

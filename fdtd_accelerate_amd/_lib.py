@@ -62,6 +62,7 @@ class InstanceDesc(ctypes.Structure):
         ("nsrc", _I), ("src_comp", _I * MAX_SRC), ("src_box", (_I * 6) * MAX_SRC),
         ("src_sv", _P * MAX_SRC), ("src_svs", _F * MAX_SRC), ("src_overwrite", _I * MAX_SRC),
         ("nprobe", _I), ("probe_comp", _I * MAX_PROBE), ("probe_ijk", (_I * 3) * MAX_PROBE),
+        ("ce_mat", _P * 3), ("ce_lut", _P),
     ]
 
 

@@ -80,8 +80,10 @@ class DesignProblem:
         if tuple(eps.shape) != (di, dj, dk):
             raise ValueError(f"eps shape {tuple(eps.shape)} != design box {(di, dj, dk)}")
         ce = 0.5 / eps
-        for c in self.sim.fields.ces:
+        f = self.sim.fields
+        for c in f.ces:
             c[i0:i0 + di, j0:j0 + dj, k0:k0 + dk] = ce
+        f.ce_mat = f.ce_lut = None  # material indices of the previous CE no longer hold
         self._eps = eps.clone()
 
     def _step(self):

@@ -33,6 +33,8 @@ def instance_descs(sims):
     """``mifdtd_instance`` descriptors of the simulations (a ctypes array) and the shared
     CPML tables (bE, aE, bH, aH device pointers, or Nones without CPML)."""
     ref = next((s.cpml for s in sims if s.cpml is not None), None)
+    # CE as material indices when every instance has them (compress_ce), else the arrays
+    tables = all(s.fields.ce_on and s.fields.ce_mat is not None for s in sims)
     descs = (_lib.InstanceDesc * len(sims))()
     for n, s in enumerate(sims):
         f = s.fields
@@ -43,6 +45,10 @@ def instance_descs(sims):
         if f.ce_on:
             for c in range(3):
                 d.ce[c] = f.ces[c].data_ptr()
+        if tables:
+            for c in range(3):
+                d.ce_mat[c] = f.ce_mat[c].data_ptr()
+            d.ce_lut = f.ce_lut.data_ptr()
         if s.cpml is not None:
             d.npml = s.cpml.npml
             d.psiE = s.cpml.psiE_buf.data_ptr()
@@ -221,11 +227,13 @@ def tune(batch, candidates=((False, 0), (True, 0), (False, 4), (True, 4), (False
     return best, res
 
 
-def make_uneven_batch(seed=3, count=64, sizes=(64, 96, 128, 160, 192), npml=8, engine="kemp"):
+def make_uneven_batch(seed=3, count=64, sizes=(64, 96, 128, 160, 192), npml=8, engine="kemp", ce_table=True):
     """The uneven batch: ``count`` cubes with edges drawn uniformly from ``sizes``, each
     with its own 10-40 non-overlapping spheres (eps_r U[1,12], radius U[3, N/8]), CE on,
     CPML ``npml`` (m = 4, R = 1e-6), an Ez source at a seeded interior point driven by the
-    Gaussian-modulated pulse, and an Ez probe 8 cells away along x."""
+    Gaussian-modulated pulse, and an Ez probe 8 cells away along x.  With ``ce_table``
+    each instance also holds its CE as uint8 material indices + table (compress_ce; the
+    same values, read as 3 B instead of 12 B per cell)."""
     rng = np.random.default_rng(seed)
     sims = []
     for n in range(count):
@@ -235,6 +243,8 @@ def make_uneven_batch(seed=3, count=64, sizes=(64, 96, 128, 160, 192), npml=8, e
         lo, hi = (npml + 1,) * 3, (N - npml - 1,) * 3
         sph = geometry.random_spheres(int(rng.integers(1 << 30)), nsph, (3.0, max(4.0, N / 8)), (1.0, 12.0), lo, hi)
         geometry.paint_ce(sim.fields.ces, sph, (N, N, N))
+        if ce_table and not sim.fields.compress_ce():
+            raise RuntimeError(f"instance {n}: more than 256 CE values")
         lo_s = npml + 2
         src = tuple(int(v) for v in rng.integers(lo_s, max(lo_s + 1, N - npml - 10), 3))
         sim.add_source("ez", src, src, gaussian_pulse())

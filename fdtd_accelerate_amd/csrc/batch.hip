@@ -38,11 +38,13 @@ __device__ __forceinline__ unsigned xcd_rows(unsigned b, unsigned nb, unsigned R
     return (sl / R) * span + x * R + sl % R;
 }
 
-template <bool HC, bool PML, bool EHALF>
+// CM: CeMode of the E pass (kCeTable stages the block's instance table in LDS).
+template <int CM, bool PML, bool EHALF>
 __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, const unsigned *__restrict__ first_block,
                                              int ninst, const float *__restrict__ tv_row, float *probe_row,
                                              unsigned remap_rows)
 {
+    __shared__ float tab[CM == kCeTable ? 768 : 1];
     const unsigned b = blockIdx.x;
     int lo = 0, hi = ninst;  // first_block[lo] <= b < first_block[hi]
     while (hi - lo > 1) {
@@ -51,6 +53,7 @@ __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, co
         else hi = mid;
     }
     const BInst &I = inst[lo];
+    if (CM == kCeTable) stage_ce_table<SB>(tab, I.F.lut);  // the instance is the whole block's
     const unsigned nb = first_block[lo + 1] - first_block[lo];
     const unsigned f = xcd_rows(b - first_block[lo], nb, remap_rows) * SB + threadIdx.x;
     if (f >= I.g.nquads) return;
@@ -61,8 +64,8 @@ __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, co
     SProbe prb = EHALF ? I.pe : I.ph;
     prb.out = probe_row;
     if (EHALF)
-        e_quad<HC, PML>(I.g, f, I.F.e[0], I.F.e[1], I.F.e[2], I.F.e[3], I.F.e[4], I.F.e[5], I.F.ce[0], I.F.ce[1],
-                        I.F.ce[2], I.cp, src, prb);
+        e_quad<CM, PML>(I.g, f, I.F.e[0], I.F.e[1], I.F.e[2], I.F.e[3], I.F.e[4], I.F.e[5], I.F.ce[0], I.F.ce[1],
+                        I.F.ce[2], I.cp, src, prb, I.F.cm[0], I.F.cm[1], I.F.cm[2], tab);
     else
         h_quad<PML>(I.g, f, I.F.e[0], I.F.e[1], I.F.e[2], I.F.e[3], I.F.e[4], I.F.e[5], I.cp, src, prb);
 }
@@ -74,7 +77,8 @@ using namespace mifdtd;
 // What the launches of a batch need besides the device descriptors.
 struct BMeta {
     int ninst = 0;
-    bool hc = false, pml = false;
+    int cem = kCeConst;
+    bool pml = false;
     unsigned total_blocks = 0;
     int total_src = 0, total_probe = 0;
 };
@@ -85,6 +89,7 @@ static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, 
 {
     if (!inst || ninst < 1) return fail(MIFDTD_EINVAL, "need at least one instance");
     const bool hc = inst[0].ce[0] != nullptr;
+    const bool tbl = inst[0].ce_mat[0] != nullptr;
     bool pml = false;
     int batch_npml = 0;  // the one npml of the shared 2*(npml+1)-entry tables
     int tsrc = 0, tprobe = 0;
@@ -97,6 +102,15 @@ static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, 
             if (!d.e[c] || !aligned16(d.e[c])) return fail(MIFDTD_EALIGN, "instance %d: field %d NULL or misaligned", n, c);
         if ((d.ce[0] != nullptr) != hc || (hc && (!d.ce[1] || !d.ce[2])))
             return fail(MIFDTD_ENULL, "instance %d: CE arrays must be set on every instance or on none", n);
+        if ((d.ce_mat[0] != nullptr) != tbl)
+            return fail(MIFDTD_ENULL, "instance %d: CE material indices must be set on every instance or on none", n);
+        if (tbl) {
+            if (!d.ce_mat[1] || !d.ce_mat[2] || !d.ce_lut)
+                return fail(MIFDTD_ENULL, "instance %d: CE material indices need all three index arrays and the table", n);
+            for (int q = 0; q < 3; q++)
+                if (reinterpret_cast<uintptr_t>(d.ce_mat[q]) & 3u)
+                    return fail(MIFDTD_EALIGN, "instance %d: CE material index arrays not 4-byte aligned", n);
+        }
         const int nz = nzp - pad;
         BInst &B = h[n];
         B = BInst{};
@@ -110,6 +124,10 @@ static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, 
         B.g.remap = 0;
         for (int c = 0; c < 6; c++) B.F.e[c] = d.e[c];
         for (int c = 0; c < 3; c++) B.F.ce[c] = d.ce[c];
+        if (tbl) {
+            for (int c = 0; c < 3; c++) B.F.cm[c] = d.ce_mat[c];
+            B.F.lut = d.ce_lut;
+        }
         if (d.npml > 0) {
             const int np_ = d.npml;
             if (np_ > MIFDTD_MAX_NPML) return fail(MIFDTD_EINVAL, "instance %d: npml %d too large", n, np_);
@@ -180,7 +198,7 @@ static int build_batch(const mifdtd_instance *inst, int ninst, const float *bE, 
     }
     first[ninst] = (unsigned)blocks;
     m.ninst = ninst;
-    m.hc = hc;
+    m.cem = tbl ? kCeTable : hc ? kCeArrays : kCeConst;
     m.pml = pml;
     m.total_blocks = (unsigned)blocks;
     m.total_src = tsrc;
@@ -195,15 +213,20 @@ static int batch_launches(const BMeta &b, const BInst *d_inst, const unsigned *d
     for (int s = 0; s < nsteps; s++) {
         const float *tv = b.total_src ? tvals + (size_t)s * b.total_src : nullptr;
         float *pr = b.total_probe ? probe_series + (size_t)s * b.total_probe : nullptr;
-#define MIFDTD_BATCH_LAUNCH(HC, PML, EH) \
-    k_batch<HC, PML, EH><<<grid, SB, 0, st>>>(d_inst, d_first, b.ninst, tv, pr, R)
-        if (b.hc && b.pml) MIFDTD_BATCH_LAUNCH(true, true, true);
-        else if (b.hc) MIFDTD_BATCH_LAUNCH(true, false, true);
-        else if (b.pml) MIFDTD_BATCH_LAUNCH(false, true, true);
-        else MIFDTD_BATCH_LAUNCH(false, false, true);
+#define MIFDTD_BATCH_LAUNCH(CM, PML, EH) \
+    k_batch<CM, PML, EH><<<grid, SB, 0, st>>>(d_inst, d_first, b.ninst, tv, pr, R)
+#define MIFDTD_BATCH_E(CM)                                   \
+    do {                                                     \
+        if (b.pml) MIFDTD_BATCH_LAUNCH(CM, true, true);      \
+        else MIFDTD_BATCH_LAUNCH(CM, false, true);           \
+    } while (0)
+        if (b.cem == kCeTable) MIFDTD_BATCH_E(kCeTable);
+        else if (b.cem == kCeArrays) MIFDTD_BATCH_E(kCeArrays);
+        else MIFDTD_BATCH_E(kCeConst);
         MIFDTD_CHECK_LAUNCH("batch E launch");
-        if (b.pml) MIFDTD_BATCH_LAUNCH(false, true, false);
-        else MIFDTD_BATCH_LAUNCH(false, false, false);
+        if (b.pml) MIFDTD_BATCH_LAUNCH(kCeConst, true, false);
+        else MIFDTD_BATCH_LAUNCH(kCeConst, false, false);
+#undef MIFDTD_BATCH_E
         MIFDTD_CHECK_LAUNCH("batch H launch");
 #undef MIFDTD_BATCH_LAUNCH
     }

@@ -87,11 +87,6 @@ struct Ctx {  // buffers of one launch (restrict-qualified at the kernel boundar
     const unsigned char *mx, *my, *mz;          // CE material indices (kCeTable)
 };
 
-// How CE reaches the E update: the constant 0.5, three float arrays, or three uint8
-// material-index arrays into a 3 x 256 float table held in LDS (same float values, so
-// the products are identical; 3 B per cell instead of 12).
-enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2 };
-
 // CE of the quad at element rb + lc (rb uniform, lc the lane's element offset)
 template <int CEM>
 __device__ __forceinline__ Q4 load_ce(const float *cf, const unsigned char *mt, const float *tab, size_t rb, unsigned lc)

@@ -14,12 +14,16 @@
 
 namespace mifdtd {
 
-template <bool HC, bool PML>
+// CM: CeMode (kCeTable stages the 3 x 256 CE table in LDS per block).
+template <int CM, bool PML>
 __global__ __launch_bounds__(SB) void k_slab_e(SGeom g, SFields F, SCpml cp, SSrc src, SProbe prb)
 {
+    __shared__ float tab[CM == kCeTable ? 768 : 1];
+    if (CM == kCeTable) stage_ce_table<SB>(tab, F.lut);
     const unsigned f = (g.remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * SB + threadIdx.x;
     if (f < g.nquads)
-        e_quad<HC, PML>(g, f, F.e[0], F.e[1], F.e[2], F.e[3], F.e[4], F.e[5], F.ce[0], F.ce[1], F.ce[2], cp, src, prb);
+        e_quad<CM, PML>(g, f, F.e[0], F.e[1], F.e[2], F.e[3], F.e[4], F.e[5], F.ce[0], F.ce[1], F.ce[2], cp, src, prb,
+                        F.cm[0], F.cm[1], F.cm[2], tab);
 }
 
 template <bool PML>
@@ -52,6 +56,15 @@ int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps, con
     if (hc && (!d->ce[1] || !d->ce[2])) return fail(MIFDTD_ENULL, "CE arrays must be all set or all NULL");
     if (hc && (!aligned16(d->ce[0]) || !aligned16(d->ce[1]) || !aligned16(d->ce[2])))
         return fail(MIFDTD_EALIGN, "CE arrays not 16-byte aligned");
+    const bool tbl = d->ce_mat[0] != nullptr;
+    if (tbl) {
+        if (!d->ce_mat[1] || !d->ce_mat[2] || !d->ce_lut)
+            return fail(MIFDTD_ENULL, "CE material indices need all three index arrays and the table");
+        for (int q = 0; q < 3; q++)
+            if (reinterpret_cast<uintptr_t>(d->ce_mat[q]) & 3u)
+                return fail(MIFDTD_EALIGN, "CE material index arrays not 4-byte aligned");
+    }
+    const int cem = tbl ? kCeTable : hc ? kCeArrays : kCeConst;
     const int nz = nzp - pad;
     if (d->nsrc < 0 || d->nsrc > MIFDTD_FUSED_MAX_SRC) return fail(MIFDTD_EINVAL, "nsrc outside [0,%d]", MIFDTD_FUSED_MAX_SRC);
     if (d->nprobe < 0 || d->nprobe > MIFDTD_FUSED_MAX_PROBE)
@@ -111,9 +124,13 @@ int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps, con
     const int S = slab_planes < nx ? slab_planes : nx;
     if (plane_q * (unsigned long long)S + SB >= (1ull << 31)) return fail(MIFDTD_ERANGE, "slab of %d planes too large", S);
     hipStream_t st = to_stream(stream);
-    SFields F;
+    SFields F{};
     for (int c = 0; c < 6; c++) F.e[c] = d->e[c];
     for (int c = 0; c < 3; c++) F.ce[c] = d->ce[c];
+    if (tbl) {
+        for (int c = 0; c < 3; c++) F.cm[c] = d->ce_mat[c];
+        F.lut = d->ce_lut;
+    }
     const int ns = (nx + S - 1) / S;
     // Small slabs: give each XCD a contiguous run of rows (j+-1 reuse in its L2).
     const int remap = S <= 8;
@@ -160,10 +177,15 @@ int mifdtd_slab_run(const mifdtd_fused_desc *d, int slab_planes, int nsteps, con
         for (int sl = 0; sl < ns; sl++) {
             const SGeom g = geom(sl);
             const unsigned grid = (g.nquads + SB - 1) / SB;
-            if (hc && pml) k_slab_e<true, true><<<grid, SB, 0, st>>>(g, F, cp, se, pen);
-            else if (hc) k_slab_e<true, false><<<grid, SB, 0, st>>>(g, F, cp, se, pen);
-            else if (pml) k_slab_e<false, true><<<grid, SB, 0, st>>>(g, F, cp, se, pen);
-            else k_slab_e<false, false><<<grid, SB, 0, st>>>(g, F, cp, se, pen);
+#define MIFDTD_SLAB_E(CM)                                                   \
+    do {                                                                    \
+        if (pml) k_slab_e<CM, true><<<grid, SB, 0, st>>>(g, F, cp, se, pen);  \
+        else k_slab_e<CM, false><<<grid, SB, 0, st>>>(g, F, cp, se, pen);    \
+    } while (0)
+            if (cem == kCeTable) MIFDTD_SLAB_E(kCeTable);
+            else if (cem == kCeArrays) MIFDTD_SLAB_E(kCeArrays);
+            else MIFDTD_SLAB_E(kCeConst);
+#undef MIFDTD_SLAB_E
             MIFDTD_CHECK_LAUNCH("slab E launch");
             if (pml) k_slab_h<true><<<grid, SB, 0, st>>>(g, F, cp, sh, phn);
             else k_slab_h<false><<<grid, SB, 0, st>>>(g, F, cp, sh, phn);

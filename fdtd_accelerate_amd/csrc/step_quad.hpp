@@ -231,6 +231,20 @@ __device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int
 // memory round trip even on CPML cells (psi pointers may alias the fields as far as the
 // compiler knows, so a load placed after a field store would wait for it).
 
+// How CE reaches the E update: the constant 0.5, three float arrays, or three uint8
+// material-index arrays into a 3 x 256 float table held in LDS (same float values, so
+// the products are identical; 3 B per cell instead of 12).
+enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2 };
+
+// Stages the 3 x 256 CE table in LDS (every thread of the block takes part; the block
+// synchronises before any thread reads it).
+template <int NT>
+__device__ __forceinline__ void stage_ce_table(float *tab, const float *__restrict__ lut)
+{
+    for (int t = threadIdx.x; t < 768; t += NT) tab[t] = lut[t];
+    __syncthreads();
+}
+
 // CE of a quad as uint8 material indices m[0..3] into a 256-entry float table.
 __device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m, const float *tab)
 {
@@ -464,19 +478,23 @@ __device__ __forceinline__ void e_quad_at(
     sprobe(prb, 2, i, j, k0, e2);
 }
 
-// E pass on quad f of the slab described by g (in place).
-template <bool HC, bool PML>
+// E pass on quad f of the slab described by g (in place); CE as in e_quad_at (CM = 2:
+// the material indices mx.. into tab).
+template <int CM, bool PML>
 __device__ __forceinline__ void e_quad(
     const SGeom &g, unsigned f, float *ex, float *ey, float *ez,
     const float *__restrict__ hx, const float *__restrict__ hy, const float *__restrict__ hz,
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
-    const SCpml &cp, const SSrc &src, const SProbe &prb)
+    const SCpml &cp, const SSrc &src, const SProbe &prb, const unsigned char *__restrict__ mx = nullptr,
+    const unsigned char *__restrict__ my = nullptr, const unsigned char *__restrict__ mz = nullptr,
+    const float *tab = nullptr)
 {
     const unsigned row = f / g.Q;
     const int k0 = (int)(f - row * g.Q) * 4;
     const int di = (int)(row / (unsigned)g.ny);
     const int j = (int)row - di * g.ny;
-    e_quad_at<HC, PML>(g, g.i0 + di, j, k0, ex, ey, ez, ex, ey, ez, hx, hy, hz, cex, cey, cez, cp, src, prb);
+    e_quad_at<CM, PML>(g, g.i0 + di, j, k0, ex, ey, ez, ex, ey, ez, hx, hy, hz, cex, cey, cez, cp, src, prb, mx, my,
+                       mz, tab);
 }
 
 // H pass on the quad (i, j, k0..k0+3): reads H from hxi.. and writes it to hxo.. (the
@@ -694,6 +712,8 @@ __device__ __forceinline__ void h_quad(
 struct SFields {
     float *e[6];
     const float *ce[3];
+    const unsigned char *cm[3];  // CE as material indices (kCeTable), else NULL
+    const float *lut;            // their 3 x 256 table
 };
 
 }  // namespace mifdtd

@@ -184,7 +184,7 @@ class FusedStepper:
         if f.ce_on:
             for c in range(3):
                 d.ce[c] = f.ces[c].data_ptr()
-        if self.alt and f.ce_mat is not None:  # CE as material indices + table (fused only)
+        if f.ce_on and f.ce_mat is not None:  # CE as material indices + table
             for c in range(3):
                 d.ce_mat[c] = f.ce_mat[c].data_ptr()
             d.ce_lut = f.ce_lut.data_ptr()
@@ -341,7 +341,10 @@ class SlabStepper(FusedStepper):
 
     @property
     def bytes_per_cell(self):
-        return 60.0 if self.sim.fields.ce_on else 48.0
+        """Algorithmic HBM bytes per cell of one step (without CPML), as the fused
+        engine's: CE as uint8 indices 3 B, as float arrays 12 B."""
+        f = self.sim.fields
+        return 51.0 if f.ce_on and f.ce_mat is not None else 60.0 if f.ce_on else 48.0
 
     def run(self, nsteps, buf, time_kernel=False):
         """Advance nsteps; ``buf`` is an optional device (nsteps, nprobe) probe buffer.

@@ -291,9 +291,10 @@ typedef struct {
     int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
     int rows_per_thread;  /* fused: y rows owned by one work-group, 1..15 (T+1 waves) */
     int planes_per_chunk; /* fused: x planes marched by one work-group */
-    /* fused only: CE as uint8 material indices (nx, ny, nz_pitch) per component into
-     * ce_lut[3][256] (the float CE values), read by the sweep and the shell E pass;
-     * when ce_mat[0] is set, ce[] is ignored.  Index arrays must be 4-byte aligned. */
+    /* CE as uint8 material indices (nx, ny, nz_pitch) per component into
+     * ce_lut[3][256] (the float CE values), read by the fused sweep, the shell E pass
+     * and the slab E pass; when ce_mat[0] is set, ce[] is ignored.  Index arrays must
+     * be 4-byte aligned. */
     const unsigned char *ce_mat[3];
     const float *ce_lut;
     /* fused only, with CPML: a third buffer set (ex..hz).  When set, steps advance in
@@ -390,6 +391,11 @@ typedef struct {
     int nprobe;
     int probe_comp[MIFDTD_FUSED_MAX_PROBE];
     int probe_ijk[MIFDTD_FUSED_MAX_PROBE][3];
+    /* optional: CE as uint8 material indices (each array like a field, 4-byte aligned)
+     * into a DEVICE table of 3 x 256 floats (rows cex, cey, cez) of the instance, read
+     * instead of ce[]; set on every instance or on none */
+    const unsigned char *ce_mat[3];
+    const float *ce_lut;
 } mifdtd_instance;
 
 typedef struct mifdtd_batch mifdtd_batch; /* opaque; holds device copies of the descriptors */

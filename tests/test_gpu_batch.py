@@ -19,7 +19,9 @@ def gpu():
     _lib.load()
 
 
-def _mk(shapes, npml, seed, ce=True):
+def _mk(shapes, npml, seed, ce=True, materials=False):
+    """Instances with random fields, CE (random values, or with ``materials`` a few
+    values per instance), two sources and two probes."""
     from fdtd_accelerate_amd.sim import YeeSim
     from fdtd_accelerate_amd.sources import gaussian_pulse
     rng = np.random.default_rng(seed)
@@ -28,7 +30,10 @@ def _mk(shapes, npml, seed, ce=True):
         s = YeeSim(nx, ny, nz, npml=npml, coeff_use="e" if ce else "")
         f = s.fields
         f.set_ehs(*[rng.standard_normal((nx, ny, nz)).astype(np.float32) for _ in range(6)])
-        if ce:
+        if ce and materials:
+            vals = (0.5 / rng.uniform(1, 12, 3 + n)).astype(np.float32)
+            f.set_ces(*[vals[rng.integers(0, len(vals), (nx, ny, nz))] for _ in range(3)])
+        elif ce:
             f.set_ces(*[(0.5 / rng.uniform(1, 6, (nx, ny, nz))).astype(np.float32) for _ in range(3)])
         tf = gaussian_pulse(3.0 + n, 2.0, 7.0)
         s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
@@ -62,6 +67,32 @@ def test_batch_matches_single_runs(use_graph, remap):
             for c in range(2):
                 assert sa.cpml.psiE[key][c].cpu().numpy().tobytes() == sb.cpml.psiE[key][c].cpu().numpy().tobytes()
                 assert sa.cpml.psiH[key][c].cpu().numpy().tobytes() == sb.cpml.psiH[key][c].cpu().numpy().tobytes()
+    bt.close()
+
+
+@pytest.mark.parametrize("compressed", ["all", "some"])
+def test_batch_ce_tables_match_single_runs(compressed):
+    """CE as uint8 material indices + a table per instance (Fields.compress_ce): the batch
+    reads them when every instance has them, else the float arrays of all; either way
+    byte-equal to the instances run alone on the in-place engine with float CE arrays."""
+    from fdtd_accelerate_amd.batch import BatchedYee
+    steps = 6
+    a = _mk(SHAPES, 4, 2, materials=True)
+    b = _mk(SHAPES, 4, 2, materials=True)
+    for n, s in enumerate(b):
+        if compressed == "all" or n % 2:
+            assert s.fields.compress_ce()
+    series_a = [s.run(steps) for s in a]
+    bt = BatchedYee(b)
+    series_b = bt.run(steps)
+    torch.cuda.synchronize()
+    for n, (sa, sb) in enumerate(zip(a, b)):
+        for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            assert sa.field(name).cpu().numpy().tobytes() == sb.field(name).cpu().numpy().tobytes(), (n, name)
+        assert series_a[n].tobytes() == series_b[n].tobytes(), n
+        for key in sa.cpml.psiE:
+            for c in range(2):
+                assert sa.cpml.psiE[key][c].cpu().numpy().tobytes() == sb.cpml.psiE[key][c].cpu().numpy().tobytes()
     bt.close()
 
 

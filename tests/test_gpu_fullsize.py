@@ -181,9 +181,10 @@ def test_s4_512cube_spheres_vs_oracle():
 def test_uneven_64cube_batch_equals_single_runs():
     _need(40)
     from fdtd_accelerate_amd import batch as B
-    a = B.make_uneven_batch(seed=3)
-    b = B.make_uneven_batch(seed=3)
-    assert len(a) == 64
+    a = B.make_uneven_batch(seed=3, ce_table=False)  # single runs read the float CE arrays,
+    b = B.make_uneven_batch(seed=3)                  # the batch the material indices + tables
+    assert len(a) == 64 and all(s.fields.ce_mat is None for s in a)
+    assert all(s.fields.ce_mat is not None for s in b)
     for n, (sa, sb) in enumerate(zip(a, b)):
         _randomize(sa.fields.ehs, 100 + n)
         for x, y in zip(sa.fields.ehs, sb.fields.ehs):

@@ -330,6 +330,41 @@ def test_slab_matches_inplace(ns, npml, S, steps, ce):
                 assert sims[0].cpml.psiH[key][c].cpu().numpy().tobytes() == sims[1].cpml.psiH[key][c].cpu().numpy().tobytes(), ("psiH", key, c)
 
 
+@pytest.mark.parametrize("ns,npml,S,steps", [
+    ((24, 20, 32), 0, 5, 4),
+    ((30, 26, 40), 4, 1, 5),
+    ((29, 31, 57), 5, 4, 7),
+    ((26, 70, 300), 5, 26, 2),
+])
+def test_slab_ce_table_matches_inplace(ns, npml, S, steps):
+    """CE held as uint8 material indices + table (Fields.compress_ce) on the slab engine:
+    byte-equal to the in-place engine reading the float CE arrays (fields, psi, probes)."""
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    sims = [_sim(ns, npml, "kemp"), _slab(ns, npml, S)]
+    nx, ny, nz = ns
+    rng = np.random.default_rng(8)
+    vals = (0.5 / np.array([1.0, 1.7, 2.25, 4.0, 6.5, 9.0, 12.0])).astype(np.float32)
+    mats = [rng.integers(0, len(vals), (nx, ny, nz)) for _ in range(3)]
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    for s in sims:
+        _fill(s, 13)
+        s.fields.set_ces(*[vals[m] for m in mats])
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_probe("ey", (nx // 2, ny // 2 + 1, nz // 2))
+        s.add_probe("ex", (1, 1, 1))
+    assert sims[1].fields.compress_ce()
+    series = [s.run(steps) for s in sims]
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert a.tobytes() == b.tobytes(), (name, np.abs(a - b).max())
+    assert series[0].tobytes() == series[1].tobytes()
+    if npml:
+        for key in sims[0].cpml.psiE:
+            for c in range(2):
+                assert sims[0].cpml.psiE[key][c].cpu().numpy().tobytes() == sims[1].cpml.psiE[key][c].cpu().numpy().tobytes(), ("psiE", key, c)
+
+
 def test_slab_dipole128_vs_oracle():
     """The 128^3 vacuum dipole run (CPML 10, point source, probe) on the slab engine, 400
     steps: probe series and fields byte-equal to the float32 oracle."""

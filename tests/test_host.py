@@ -259,6 +259,31 @@ def test_static_batch_rejects_beyond_its_limit_without_gpu():
     assert rc == -4 and "at most 128" in _lib.last_error()
 
 
+def test_batch_ce_index_arguments_rejected_without_gpu():
+    """CE material indices on some instances only, an incomplete index set or a
+    misaligned index array: refused before any device work (field pointers are fake,
+    16-byte aligned host values)."""
+    from fdtd_accelerate_amd import _lib
+    lib = _lib.load()
+    descs = (_lib.InstanceDesc * 2)()
+    for d in descs:
+        d.nx, d.ny, d.nz_pitch, d.pad = 8, 8, 8, 0
+        for c in range(6):
+            d.e[c] = 4096 * (c + 1)
+    out = ctypes.c_void_p()
+    descs[0].ce_mat[0] = descs[0].ce_mat[1] = descs[0].ce_mat[2] = 65536
+    descs[0].ce_lut = 65536
+    rc = lib.mifdtd_batch_create(descs, 2, None, None, None, None, ctypes.byref(out))
+    assert rc == -3 and "instance 1" in _lib.last_error() and "every instance" in _lib.last_error()
+    descs[1].ce_mat[0] = descs[1].ce_mat[1] = 65536
+    rc = lib.mifdtd_batch_create(descs, 2, None, None, None, None, ctypes.byref(out))
+    assert rc == -3 and "all three" in _lib.last_error()
+    descs[1].ce_mat[2], descs[1].ce_lut = 65538, 65536
+    rc = lib.mifdtd_batch_create(descs, 2, None, None, None, None, ctypes.byref(out))
+    assert rc == -2 and "4-byte aligned" in _lib.last_error()
+    assert not out.value
+
+
 def test_ctypes_mirrors_match_header(tmp_path):
     """Every ctypes mirror in _lib has the size and field offsets of its C struct in
     include/mifdtd.h (a C probe compiled with gcc against the header prints them)."""
