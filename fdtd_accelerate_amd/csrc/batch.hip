@@ -38,13 +38,12 @@ __device__ __forceinline__ unsigned xcd_rows(unsigned b, unsigned nb, unsigned R
     return (sl / R) * span + x * R + sl % R;
 }
 
-// CM: CeMode of the E pass (kCeTable stages the block's instance table in LDS).
+// CM: CeMode of the E pass (kCeTable: the instance's material indices and table).
 template <int CM, bool PML, bool EHALF>
 __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, const unsigned *__restrict__ first_block,
                                              int ninst, const float *__restrict__ tv_row, float *probe_row,
                                              unsigned remap_rows)
 {
-    __shared__ float tab[CM == kCeTable ? 768 : 1];
     const unsigned b = blockIdx.x;
     int lo = 0, hi = ninst;  // first_block[lo] <= b < first_block[hi]
     while (hi - lo > 1) {
@@ -53,7 +52,7 @@ __global__ __launch_bounds__(SB) void k_batch(const BInst *__restrict__ inst, co
         else hi = mid;
     }
     const BInst &I = inst[lo];
-    if (CM == kCeTable) stage_ce_table<SB>(tab, I.F.lut);  // the instance is the whole block's
+    const float *tab = I.F.lut;
     const unsigned nb = first_block[lo + 1] - first_block[lo];
     const unsigned f = xcd_rows(b - first_block[lo], nb, remap_rows) * SB + threadIdx.x;
     if (f >= I.g.nquads) return;

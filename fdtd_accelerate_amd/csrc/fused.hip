@@ -1020,7 +1020,7 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
     k0 = 4 * (s.q0[b] + (int)q);
 }
 
-// CE: CEM as in the sweep (kCeTable stages the 3 x 256 table in LDS per block).
+// CE: CEM as in the sweep (kCeTable reads the 3 x 256 table from global memory).
 template <int CEM, int FEAT>
 __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigned total, const float *ex, const float *ey,
                                                  const float *ez, float *oex, float *oey, float *oez,
@@ -1032,11 +1032,7 @@ __global__ __launch_bounds__(256) void k_shell_e(SGeom g, ShellBoxes sb, unsigne
                                                  const unsigned char *__restrict__ mz, const float *__restrict__ lut,
                                                  SCpml cp, SSrc src, SProbe prb, SDrude dr, STfsf tfe)
 {
-    __shared__ float tab[CEM == kCeTable ? 768 : 1];
-    if (CEM == kCeTable) {
-        for (int t = threadIdx.x; t < 768; t += 256) tab[t] = lut[t];
-        __syncthreads();
-    }
+    const float *tab = lut;  // read from global memory (L1/L2 resident)
     const unsigned f = blockIdx.x * 256u + threadIdx.x;
     if (f >= total) return;
     int i, j, k0;

@@ -14,12 +14,11 @@
 
 namespace mifdtd {
 
-// CM: CeMode (kCeTable stages the 3 x 256 CE table in LDS per block).
+// CM: CeMode (kCeTable: CE from the material indices and the 3 x 256 table).
 template <int CM, bool PML>
 __global__ __launch_bounds__(SB) void k_slab_e(SGeom g, SFields F, SCpml cp, SSrc src, SProbe prb)
 {
-    __shared__ float tab[CM == kCeTable ? 768 : 1];
-    if (CM == kCeTable) stage_ce_table<SB>(tab, F.lut);
+    const float *tab = F.lut;
     const unsigned f = (g.remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * SB + threadIdx.x;
     if (f < g.nquads)
         e_quad<CM, PML>(g, f, F.e[0], F.e[1], F.e[2], F.e[3], F.e[4], F.e[5], F.ce[0], F.ce[1], F.ce[2], cp, src, prb,

@@ -232,18 +232,10 @@ __device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int
 // compiler knows, so a load placed after a field store would wait for it).
 
 // How CE reaches the E update: the constant 0.5, three float arrays, or three uint8
-// material-index arrays into a 3 x 256 float table held in LDS (same float values, so
-// the products are identical; 3 B per cell instead of 12).
+// material-index arrays into a 3 x 256 float table (same float values, so the products
+// are identical; 3 B per cell instead of 12).  The per-quad passes read the 3 KiB table
+// from global memory, where it stays in L1/L2; the sweep kernels stage it in LDS.
 enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2 };
-
-// Stages the 3 x 256 CE table in LDS (every thread of the block takes part; the block
-// synchronises before any thread reads it).
-template <int NT>
-__device__ __forceinline__ void stage_ce_table(float *tab, const float *__restrict__ lut)
-{
-    for (int t = threadIdx.x; t < 768; t += NT) tab[t] = lut[t];
-    __syncthreads();
-}
 
 // CE of a quad as uint8 material indices m[0..3] into a 256-entry float table.
 __device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m, const float *tab)
