@@ -271,6 +271,19 @@ int read_i64_attr(hid_t loc, const char *name, long long *v)
     return 0;
 }
 
+// Runs an entry point's body; no C++ exception crosses the C interface.
+template <class F>
+int guarded(const char *what, F &&body)
+{
+    try {
+        return body();
+    } catch (const std::exception &e) {
+        return fail(MIFDTD_H5_EIO, "%s: %s", what, e.what());
+    } catch (...) {
+        return fail(MIFDTD_H5_EIO, "%s: unknown C++ exception", what);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -289,72 +302,74 @@ int mifdtd_h5_write_fields(const char *path, int ndset, const char *const *names
     if (int rc = check_dims(rank, dims, hd)) return rc;
     for (int n = 0; n < ndset; ++n)
         if (!names[n] || !data[n]) return fail(MIFDTD_H5_EINVAL, "dataset %d: null name or data", n);
-    try {
+    return guarded(path, [&]() -> int {
         Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
         if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot create %s", path);
         for (int n = 0; n < ndset; ++n)
             if (int rc = write_dataset(file.id, names[n], data[n], f64, rank, hd, gzip_level, nthreads)) return rc;
-    } catch (const std::exception &e) {  // no C++ exception crosses the C interface
-        return fail(MIFDTD_H5_EIO, "writing %s: %s", path, e.what());
-    }
-    return 0;
+        return 0;
+    });
 }
 
 int mifdtd_h5_write_geometry(const char *path, long long nx, long long ny, long long nz, const char *coeff_use)
 {
     quiet();
     if (!path || !coeff_use) return fail(MIFDTD_H5_EINVAL, "null argument");
-    Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
-    if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot create %s", path);
-    if (int rc = write_i64_attr(file.id, "nx", nx)) return rc;
-    if (int rc = write_i64_attr(file.id, "ny", ny)) return rc;
-    if (int rc = write_i64_attr(file.id, "nz", nz)) return rc;
-    const size_t len = std::strlen(coeff_use);
-    std::vector<char> buf(std::max<size_t>(len, 1), 0);
-    std::memcpy(buf.data(), coeff_use, len);
-    Hid t(H5Tcopy(H5T_C_S1), H5Tclose);
-    Hid sp(H5Screate(H5S_SCALAR), H5Sclose);
-    if (!t.ok() || H5Tset_size(t.id, buf.size()) < 0 || H5Tset_strpad(t.id, H5T_STR_NULLPAD) < 0)
-        return fail(MIFDTD_H5_EIO, "coeff_use string type");
-    Hid a(H5Acreate2(file.id, "coeff_use", t.id, sp.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose);
-    if (!a.ok() || H5Awrite(a.id, t.id, buf.data()) < 0) return fail(MIFDTD_H5_EIO, "cannot write coeff_use");
-    return 0;
+    return guarded(path, [&]() -> int {
+        Hid file(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+        if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot create %s", path);
+        if (int rc = write_i64_attr(file.id, "nx", nx)) return rc;
+        if (int rc = write_i64_attr(file.id, "ny", ny)) return rc;
+        if (int rc = write_i64_attr(file.id, "nz", nz)) return rc;
+        const size_t len = std::strlen(coeff_use);
+        std::vector<char> buf(std::max<size_t>(len, 1), 0);
+        std::memcpy(buf.data(), coeff_use, len);
+        Hid t(H5Tcopy(H5T_C_S1), H5Tclose);
+        Hid sp(H5Screate(H5S_SCALAR), H5Sclose);
+        if (!t.ok() || H5Tset_size(t.id, buf.size()) < 0 || H5Tset_strpad(t.id, H5T_STR_NULLPAD) < 0)
+            return fail(MIFDTD_H5_EIO, "coeff_use string type");
+        Hid a(H5Acreate2(file.id, "coeff_use", t.id, sp.id, H5P_DEFAULT, H5P_DEFAULT), H5Aclose);
+        if (!a.ok() || H5Awrite(a.id, t.id, buf.data()) < 0) return fail(MIFDTD_H5_EIO, "cannot write coeff_use");
+        return 0;
+    });
 }
 
 int mifdtd_h5_read_geometry(const char *path, long long *nx, long long *ny, long long *nz, char *coeff_use, int cap)
 {
     quiet();
     if (!path || !nx || !ny || !nz || !coeff_use || cap < 1) return fail(MIFDTD_H5_EINVAL, "null argument");
-    Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
-    if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
-    if (int rc = read_i64_attr(file.id, "nx", nx)) return rc;
-    if (int rc = read_i64_attr(file.id, "ny", ny)) return rc;
-    if (int rc = read_i64_attr(file.id, "nz", nz)) return rc;
-    Hid a(H5Aopen(file.id, "coeff_use", H5P_DEFAULT), H5Aclose);
-    if (!a.ok()) return fail(MIFDTD_H5_EFORMAT, "attribute 'coeff_use' is missing");
-    Hid ft(H5Aget_type(a.id), H5Tclose);
-    if (H5Tget_class(ft.id) != H5T_STRING) return fail(MIFDTD_H5_EFORMAT, "attribute 'coeff_use' is not a string");
-    std::string s;
-    if (H5Tis_variable_str(ft.id) > 0) {
-        Hid mt(H5Tcopy(H5T_C_S1), H5Tclose);
-        H5Tset_size(mt.id, H5T_VARIABLE);
-        char *p = nullptr;
-        if (H5Aread(a.id, mt.id, &p) < 0) return fail(MIFDTD_H5_EIO, "cannot read coeff_use");
-        s = p ? p : "";
-        H5free_memory(p);
-    } else {
-        const size_t n = H5Tget_size(ft.id);
-        std::vector<char> buf(n + 1, 0);
-        Hid mt(H5Tcopy(H5T_C_S1), H5Tclose);
-        H5Tset_size(mt.id, n);
-        H5Tset_strpad(mt.id, H5T_STR_NULLPAD);
-        if (H5Aread(a.id, mt.id, buf.data()) < 0) return fail(MIFDTD_H5_EIO, "cannot read coeff_use");
-        s.assign(buf.data(), std::strlen(buf.data()));
-        while (!s.empty() && s.back() == ' ') s.pop_back();   // a SPACEPAD string
-    }
-    if ((int)s.size() >= cap) return fail(MIFDTD_H5_EINVAL, "coeff_use '%s' longer than %d", s.c_str(), cap - 1);
-    std::memcpy(coeff_use, s.c_str(), s.size() + 1);
-    return 0;
+    return guarded(path, [&]() -> int {
+        Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+        if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
+        if (int rc = read_i64_attr(file.id, "nx", nx)) return rc;
+        if (int rc = read_i64_attr(file.id, "ny", ny)) return rc;
+        if (int rc = read_i64_attr(file.id, "nz", nz)) return rc;
+        Hid a(H5Aopen(file.id, "coeff_use", H5P_DEFAULT), H5Aclose);
+        if (!a.ok()) return fail(MIFDTD_H5_EFORMAT, "attribute 'coeff_use' is missing");
+        Hid ft(H5Aget_type(a.id), H5Tclose);
+        if (H5Tget_class(ft.id) != H5T_STRING) return fail(MIFDTD_H5_EFORMAT, "attribute 'coeff_use' is not a string");
+        std::string s;
+        if (H5Tis_variable_str(ft.id) > 0) {
+            Hid mt(H5Tcopy(H5T_C_S1), H5Tclose);
+            H5Tset_size(mt.id, H5T_VARIABLE);
+            char *p = nullptr;
+            if (H5Aread(a.id, mt.id, &p) < 0) return fail(MIFDTD_H5_EIO, "cannot read coeff_use");
+            s = p ? p : "";
+            H5free_memory(p);
+        } else {
+            const size_t n = H5Tget_size(ft.id);
+            std::vector<char> buf(n + 1, 0);
+            Hid mt(H5Tcopy(H5T_C_S1), H5Tclose);
+            H5Tset_size(mt.id, n);
+            H5Tset_strpad(mt.id, H5T_STR_NULLPAD);
+            if (H5Aread(a.id, mt.id, buf.data()) < 0) return fail(MIFDTD_H5_EIO, "cannot read coeff_use");
+            s.assign(buf.data(), std::strlen(buf.data()));
+            while (!s.empty() && s.back() == ' ') s.pop_back();   // a SPACEPAD string
+        }
+        if ((int)s.size() >= cap) return fail(MIFDTD_H5_EINVAL, "coeff_use '%s' longer than %d", s.c_str(), cap - 1);
+        std::memcpy(coeff_use, s.c_str(), s.size() + 1);
+        return 0;
+    });
 }
 
 int mifdtd_h5_dataset_info(const char *path, const char *name, int *rank, long long *dims, int *f64,
@@ -362,49 +377,56 @@ int mifdtd_h5_dataset_info(const char *path, const char *name, int *rank, long l
 {
     quiet();
     if (!path || !name || !rank || !dims || !f64 || !gzip_level) return fail(MIFDTD_H5_EINVAL, "null argument");
-    Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
-    if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
-    Hid d(H5Dopen2(file.id, name, H5P_DEFAULT), H5Dclose);
-    if (!d.ok()) return fail(MIFDTD_H5_EFORMAT, "no dataset '%s' in %s", name, path);
-    Hid sp(H5Dget_space(d.id), H5Sclose);
-    const int r = H5Sget_simple_extent_ndims(sp.id);
-    if (r < 1 || r > MIFDTD_H5_MAX_RANK) return fail(MIFDTD_H5_EFORMAT, "%s: rank %d", name, r);
-    hsize_t hd[MIFDTD_H5_MAX_RANK];
-    H5Sget_simple_extent_dims(sp.id, hd, nullptr);
-    *rank = r;
-    for (int i = 0; i < r; ++i) dims[i] = (long long)hd[i];
-    Hid t(H5Dget_type(d.id), H5Tclose);
-    if (H5Tget_class(t.id) != H5T_FLOAT) return fail(MIFDTD_H5_EFORMAT, "%s is not a floating-point dataset", name);
-    *f64 = H5Tget_size(t.id) == 8;
-    *gzip_level = -1;
-    Hid pl(H5Dget_create_plist(d.id), H5Pclose);
-    const int nf = H5Pget_nfilters(pl.id);
-    for (int i = 0; i < nf; ++i) {
-        unsigned flags = 0, cd[8] = {0}, fc = 0;
-        size_t ncd = 8;
-        char fname[64];
-        if (H5Pget_filter2(pl.id, (unsigned)i, &flags, &ncd, cd, sizeof(fname), fname, &fc) == H5Z_FILTER_DEFLATE)
-            *gzip_level = ncd > 0 ? (int)cd[0] : 0;
-    }
-    return 0;
+    return guarded(path, [&]() -> int {
+        Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+        if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
+        Hid d(H5Dopen2(file.id, name, H5P_DEFAULT), H5Dclose);
+        if (!d.ok()) return fail(MIFDTD_H5_EFORMAT, "no dataset '%s' in %s", name, path);
+        Hid sp(H5Dget_space(d.id), H5Sclose);
+        const int r = H5Sget_simple_extent_ndims(sp.id);
+        if (r < 1 || r > MIFDTD_H5_MAX_RANK) return fail(MIFDTD_H5_EFORMAT, "%s: rank %d", name, r);
+        hsize_t hd[MIFDTD_H5_MAX_RANK];
+        H5Sget_simple_extent_dims(sp.id, hd, nullptr);
+        *rank = r;
+        for (int i = 0; i < r; ++i) dims[i] = (long long)hd[i];
+        Hid t(H5Dget_type(d.id), H5Tclose);
+        if (H5Tget_class(t.id) != H5T_FLOAT) return fail(MIFDTD_H5_EFORMAT, "%s is not a floating-point dataset", name);
+        *f64 = H5Tget_size(t.id) == 8;
+        *gzip_level = -1;
+        Hid pl(H5Dget_create_plist(d.id), H5Pclose);
+        const int nf = H5Pget_nfilters(pl.id);
+        for (int i = 0; i < nf; ++i) {
+            unsigned flags = 0, cd[8] = {0}, fc = 0;
+            size_t ncd = 8;
+            char fname[64];
+            if (H5Pget_filter2(pl.id, (unsigned)i, &flags, &ncd, cd, sizeof(fname), fname, &fc) == H5Z_FILTER_DEFLATE)
+                *gzip_level = ncd > 0 ? (int)cd[0] : 0;
+        }
+        return 0;
+    });
 }
 
 int mifdtd_h5_read_dataset(const char *path, const char *name, void *out, long long nbytes, int f64)
 {
     quiet();
     if (!path || !name || !out) return fail(MIFDTD_H5_EINVAL, "null argument");
-    Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
-    if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
-    Hid d(H5Dopen2(file.id, name, H5P_DEFAULT), H5Dclose);
-    if (!d.ok()) return fail(MIFDTD_H5_EFORMAT, "no dataset '%s' in %s", name, path);
-    Hid sp(H5Dget_space(d.id), H5Sclose);
-    const long long n = (long long)H5Sget_simple_extent_npoints(sp.id);
-    if (n * (f64 ? 8 : 4) != nbytes)
-        return fail(MIFDTD_H5_EINVAL, "%s holds %lld elements, not %lld bytes of %s", name, n, nbytes,
-                    f64 ? "float64" : "float32");
-    if (H5Dread(d.id, f64 ? H5T_NATIVE_DOUBLE : H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0)
-        return fail(MIFDTD_H5_EIO, "cannot read %s", name);
-    return 0;
+    return guarded(path, [&]() -> int {
+        Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+        if (!file.ok()) return fail(MIFDTD_H5_EIO, "cannot open %s as HDF5", path);
+        Hid d(H5Dopen2(file.id, name, H5P_DEFAULT), H5Dclose);
+        if (!d.ok()) return fail(MIFDTD_H5_EFORMAT, "no dataset '%s' in %s", name, path);
+        Hid t(H5Dget_type(d.id), H5Tclose);
+        if (!t.ok() || H5Tget_class(t.id) != H5T_FLOAT)
+            return fail(MIFDTD_H5_EFORMAT, "%s is not a floating-point dataset", name);
+        Hid sp(H5Dget_space(d.id), H5Sclose);
+        const long long n = (long long)H5Sget_simple_extent_npoints(sp.id);
+        if (n * (f64 ? 8 : 4) != nbytes)
+            return fail(MIFDTD_H5_EINVAL, "%s holds %lld elements, not %lld bytes of %s", name, n, nbytes,
+                        f64 ? "float64" : "float32");
+        if (H5Dread(d.id, f64 ? H5T_NATIVE_DOUBLE : H5T_NATIVE_FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0)
+            return fail(MIFDTD_H5_EIO, "cannot read %s", name);
+        return 0;
+    });
 }
 
 }  // extern "C"
