@@ -11,7 +11,9 @@
 //      (h_quad_at).
 // The interior box is the set of cells whose E and H updates have no CPML term:
 // planes, rows and z cells [npml+1, n-1-npml) per axis (z rounded inward to 128-byte
-// lines, 32 cells, so that the shell's z pieces and the sweep share no line).  Its
+// line boundaries of the field arrays, so that the shell's z pieces and the sweep share
+// no line; when the arrays start a multiple of 16 bytes past a line, the boundaries sit
+// at k = 32m - phase, and the caller can place them close to the CPML).  Its
 // E halo (plane xb-1, row yb-1) holds no CPML term either, so the sweep recomputes it;
 // the E_new at z = kb-1 it needs comes from the shell E pass, and the shell H pass reads
 // the box's E_new.  Without CPML the box is the whole grid and only the sweep runs.
@@ -74,8 +76,9 @@ struct FGeom {
     int nchunks; // chunks over [xb, xe)
     int xb, xe;  // planes of the interior box
     int yb, ye;  // rows of the interior box
-    int kb, ke;  // z range of the interior box (multiples of 32)
-    int zb0;     // first z segment start: kb rounded down to 256 (segments sit on 1 KiB)
+    int kb, ke;  // z range of the interior box (on 128-byte lines of the field arrays)
+    int zb0;     // first z segment start: kb rounded down to 256 (segments sit on 1 KiB),
+                 // or kb itself when the arrays are not line-aligned (segments on lines)
     int nseg;    // z segments of 256 floats from zb0 covering [kb, ke)
     int nblocks;
 };
@@ -1067,6 +1070,19 @@ inline void add_box(ShellBoxes &sb, unsigned long long &total, int i0, int i1, i
 
 // The cells outside g's box as six boxes: x low/high slabs (all rows), y low/high (the
 // box's planes), z low/high (the box's planes and rows).
+// Line phase of the field arrays in floats: k = 0 of every row sits `phase` floats past
+// a 128-byte line (0 when rows do not all start at the same phase, nz_pitch % 32 != 0).
+inline int line_phase(const float *e0, int nzp)
+{
+    if (nzp % 32) return 0;
+    return (int)((reinterpret_cast<uintptr_t>(e0) >> 2) & 31u);
+}
+// The first k >= lo and the last k <= hi on a line boundary of arrays of phase ph.
+inline int line_up(int lo, int ph) { return 32 * ((lo + ph + 31) / 32) - ph; }
+inline int line_down(int hi, int ph) { return hi + ph < 0 ? -1 : 32 * ((hi + ph) / 32) - ph; }
+// Segment origin: on 1 KiB boundaries for line-aligned arrays, else at kb (a line).
+inline int seg_origin(int kb, int ph) { return ph ? kb : 256 * (kb / 256); }
+
 inline void add_shell(ShellBoxes &sb, unsigned long long &total, const FGeom &g, int Q)
 {
     add_box(sb, total, 0, g.xb, 0, g.ny, 0, Q);
@@ -1184,6 +1200,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     }
 
     // interior box: no CPML term in its E and H updates nor in its E halo
+    const int zph = line_phase(d->e[0], nzp);
     FGeom g;
     g.nx = nx; g.ny = ny; g.nzp = nzp; g.nz = nz;
     g.T = T;
@@ -1199,7 +1216,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         };
         bounds(0, nx, g.xb, g.xe);
         bounds(1, ny, g.yb, g.ye);
-        if ((pml_axes >> 2) & 1) { g.kb = 32 * ((np_ + 1 + 31) / 32); g.ke = 32 * ((nz - 1 - np_) / 32); }
+        if ((pml_axes >> 2) & 1) { g.kb = line_up(np_ + 1, zph); g.ke = line_down(nz - 1 - np_, zph); }
         else if ((periodic >> 2) & 1) { g.kb = 0; g.ke = 32 * ((nz - 1) / 32); }
         else { g.kb = 0; g.ke = nzp; }
         if (g.xb >= g.xe || g.yb >= g.ye || g.kb >= g.ke) {  // no interior: the shell is the grid
@@ -1213,7 +1230,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     const bool sweep = g.xe > g.xb;
     g.nrb = sweep ? (g.ye - g.yb + T - 1) / T : 0;
     g.nchunks = sweep ? (g.xe - g.xb + CX - 1) / CX : 0;
-    g.zb0 = 256 * (g.kb / 256);
+    g.zb0 = ((pml_axes >> 2) & 1) ? seg_origin(g.kb, zph) : 256 * (g.kb / 256);
     g.nseg = sweep ? (g.ke - g.zb0 + 255) / 256 : 0;
     const long long nbl = (long long)g.nrb * g.nchunks;
     if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
@@ -1257,12 +1274,12 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         g2 = g;
         g2.xb = np_ + 2; g2.xe = nx - 2 - np_;
         g2.yb = np_ + 2; g2.ye = ny - 2 - np_;
-        g2.kb = 32 * ((np_ + 2 + 31) / 32); g2.ke = 32 * ((nz - 2 - np_) / 32);
+        g2.kb = line_up(np_ + 2, zph); g2.ke = line_down(nz - 2 - np_, zph);
         pairs = g2.xe - g2.xb >= 4 && g2.ye - g2.yb >= 4 && g2.ke > g2.kb;  // else single steps only
         if (pairs) {
             g2.nrb = (g2.ye - g2.yb + T - 1) / T;
             g2.nchunks = (g2.xe - g2.xb + CX - 1) / CX;
-            g2.zb0 = 256 * (g2.kb / 256);
+            g2.zb0 = seg_origin(g2.kb, zph);
             g2.nseg = (g2.ke - g2.zb0 + 255) / 256;
             const long long nb2 = (long long)g2.nrb * g2.nchunks;
             if (nb2 >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");

@@ -59,17 +59,54 @@ def _check_whole_step(sim, engine, allow=(), cpml_axes=False):
         raise ValueError(f"the {engine} engine supports at most {_lib.MAX_PROBE} probes")
 
 
-def _ping_pong_sets(fields):
+def z_line_phase(nz, nzp, npml):
+    """The line phase (floats) for field arrays with CPML on the z faces that brings
+    128-byte line boundaries closest to the CPML: the sweep's z range runs from the first
+    line boundary at or after k = npml+1 to the last one at or before k = nz-1-npml, and
+    the cells between those and the CPML go to the per-quad shell passes.  A phase p
+    puts the boundaries at k = 32m - p (p a multiple of 4: 16-byte quads).  0 when
+    nz_pitch is not a multiple of 32 (rows would not share one phase) or no phase leaves
+    a z range.  At nz = 1024, npml = 12: phase 16, z range [16, 1008) instead of
+    [32, 992)."""
+    if nzp % 32:
+        return 0
+    best, waste = 0, None
+    for ph in range(0, 32, 4):
+        kb = 32 * ((npml + 1 + ph + 31) // 32) - ph
+        ke = 32 * ((nz - 1 - npml + ph) // 32) - ph
+        if ke <= kb:
+            continue
+        w = (kb - npml - 1) + (nz - 1 - npml - ke)
+        if waste is None or w < waste:
+            best, waste = ph, w
+    return best
+
+
+def _line_phase(t, nzp):
+    """Line phase (floats) of a field array as the C side reads it."""
+    return (t.data_ptr() >> 2) & 31 if nzp % 32 == 0 else 0
+
+
+def _phased(t, phase):
+    """A copy of the (contiguous) tensor t whose first element sits ``phase`` elements past
+    its allocation's alignment."""
+    buf = torch.empty(t.numel() + phase, dtype=t.dtype, device=t.device)[phase:].view(t.shape)
+    buf.copy_(t)
+    return buf
+
+
+def _ping_pong_sets(fields, phase=0):
     """The current and the alternate field set of the ping-pong step (6 arrays each):
     ex, ey, ez, hx, hy, hz of the current set, then of the alternate set, carved together
     by ``staggered_arrays`` (one allocation with staggered starts for arrays of 1 GiB and
-    more); the current values are copied in.  Smaller arrays keep the current set and get
-    a separately allocated alternate set."""
+    more), every array starting ``phase`` floats past a 128-byte line; the current values
+    are copied in.  Smaller arrays at phase 0 keep the current set and get a separately
+    allocated alternate set."""
     cur = fields.ehs
     t0 = cur[0]
-    if t0.numel() * t0.element_size() < _fields.ARENA_MIN_BYTES:
+    if not phase and t0.numel() * t0.element_size() < _fields.ARENA_MIN_BYTES:
         return list(cur), [torch.empty_like(t) for t in cur]
-    views = _fields.staggered_arrays(12, tuple(t0.shape), t0.dtype, t0.device, zero=False)
+    views = _fields.staggered_arrays(12, tuple(t0.shape), t0.dtype, t0.device, zero=False, phase=phase)
     with torch.cuda.stream(fields.stream):
         for v, t in zip(views[:6], cur):
             v.copy_(t)
@@ -107,7 +144,7 @@ def _check_tfsf_sources(sim, tfsf):
 class FusedStepper:
     kernel_name = "k_fused"
 
-    def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None):
+    def __init__(self, sim, rows_per_thread=None, planes_per_chunk=None, step_pairs=None, z_phase=None):
         from .drude import Drude
         from .pbc import Pbc
         from .tfsf import Tfsf
@@ -145,7 +182,19 @@ class FusedStepper:
             raise ValueError("step pairs need CPML on all six faces and no periodic axis")
         self.pairs = bool(full_cpml and self.drude is None and self.tfsf is None and
                           (step_pairs if step_pairs is not None else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
-        f.ehs, self.alt = _ping_pong_sets(f)
+        # field arrays placed so that the sweep's z range reaches close to the z CPML
+        # (z_phase overrides the choice; it changes where the work goes, not the results)
+        if z_phase is None:
+            z_phase = z_line_phase(f.nz, f.nz_pitch, npml) if self.cpml_axes & 4 else 0
+        if z_phase % 4 or not 0 <= z_phase < 32:
+            raise ValueError(f"z_phase must be a multiple of 4 in [0, 32) : {z_phase}")
+        self.z_phase = z_phase
+        f.ehs, self.alt = _ping_pong_sets(f, z_phase)
+        if z_phase and f.ce_mat is not None and f.nz_pitch % 128 == 0:
+            # CE indices (1 B per cell) placed so that the sweep's segments start on lines
+            kb = 32 * ((npml + 1 + z_phase + 31) // 32) - z_phase
+            with torch.cuda.stream(f.stream):
+                f.ce_mat = [_phased(m, (-kb) % 128) for m in f.ce_mat]
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
         self.T = rows_per_thread or T
         self.CX = planes_per_chunk or CX
@@ -160,7 +209,8 @@ class FusedStepper:
                 self.T = min(self.T, 7)
             if self.T > 7:
                 raise ValueError(f"rows_per_thread {self.T}: at most 7 with a Drude box")
-        self.mid = [torch.zeros_like(t) for t in f.ehs] if self.pairs else None
+        self.mid = (_fields.staggered_arrays(6, tuple(f.ehs[0].shape), f.ehs[0].dtype, f.device, phase=z_phase)
+                    if self.pairs else None)
         # the Drude current of step n+1 (the two sets swap with the field sets)
         self.j_alt = [torch.zeros_like(t) for t in self.drude.J] if self.drude is not None else None
         if cp is not None:
@@ -284,8 +334,16 @@ class FusedStepper:
         nx, ny, nz = f.nx, f.ny, f.nz
         cp = self.sim.cpml
         p = cp.npml if cp is not None else 0
+        ph = _line_phase(f.ehs[0], f.nz_pitch)
+
+        def line_up(lo):
+            return 32 * ((lo + ph + 31) // 32) - ph
+
+        def line_down(hi):
+            return 32 * ((hi + ph) // 32) - ph if hi + ph >= 0 else -1
+
         if cp is not None and (self.pairs if pairs is None else pairs):
-            box = (p + 2, nx - 2 - p, p + 2, ny - 2 - p, 32 * ((p + 33) // 32), 32 * ((nz - 2 - p) // 32))
+            box = (p + 2, nx - 2 - p, p + 2, ny - 2 - p, line_up(p + 2), line_down(nz - 2 - p))
             if box[1] - box[0] >= 4 and box[3] - box[2] >= 4 and box[5] > box[4]:
                 return box
         axes = getattr(self, "cpml_axes", 7) if cp is not None else 0
@@ -298,7 +356,7 @@ class FusedStepper:
 
         (xb, xe), (yb, ye) = bounds(0, nx), bounds(1, ny)
         if axes & 4:
-            kb, ke = 32 * ((p + 32) // 32), 32 * ((nz - 1 - p) // 32)
+            kb, ke = line_up(p + 1), line_down(nz - 1 - p)
         else:
             kb, ke = (0, 32 * ((nz - 1) // 32)) if 2 in per else (0, f.nz_pitch)
         if xb >= xe or yb >= ye or kb >= ke:

@@ -124,9 +124,10 @@ def test_yee_1024cube_spheres_engines_and_oracle():
 
 def test_yee_1024cube_step_pairs_equal_inplace():
     """Step pairs on the 1024^3 workload (two-step sweep over the box 14 .. 1010 in x and
-    y, 32 .. 992 in z, 1776 blocks of 12 waves; shell and fringe passes into the third
-    buffer set): equal to the in-place engine after 2 steps (one pair), 5 more (two pairs
-    and a single step) and 51 more, fields, psi and probe series."""
+    y, 16 .. 1008 in z with the field arrays at line phase 16, 1776 blocks of 12 waves;
+    shell and fringe passes into the third buffer set): equal to the in-place engine
+    after 2 steps (one pair), 5 more (two pairs and a single step) and 51 more, fields,
+    psi and probe series."""
     _need(160)
     from fdtd_accelerate_amd.fused import FusedStepper
     ref = _yee("kemp", False)
@@ -136,7 +137,7 @@ def test_yee_1024cube_step_pairs_equal_inplace():
         b.copy_(a)
     torch.cuda.synchronize()
     fus._fused = FusedStepper(fus, step_pairs=True)
-    assert fus._fused.pairs and fus._fused.sweep_box() == (14, 1010, 14, 1010, 32, 992)
+    assert fus._fused.pairs and fus._fused.sweep_box() == (14, 1010, 14, 1010, 16, 1008)
     for steps in (2, 5, 51):
         a = ref.run(steps)
         b = fus.run(steps)
