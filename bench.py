@@ -83,6 +83,8 @@ def parse():
     p.add_argument("--pairs", action="store_true",
                    help="fused engine: advance steps in pairs (two-step interior sweep; slower on MI355X)")
     p.add_argument("--slab-planes", type=int, default=None)
+    p.add_argument("--z-phase", type=int, default=None,
+                   help="fused: 128-byte line phase (floats) of the field arrays (default: chosen from npml)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     p.add_argument("--workload", default="yee", choices=("yee", "s4", "batch", "adjoint", "pbc", "tfsf", "drude", "snapshot"))
     p.add_argument("--instances", type=int, default=64)
@@ -157,7 +159,7 @@ def time_kemp(sim, steps, warmup):
     return ms / steps, wall, kern_ms, "update_e", 48.0
 
 
-def time_fused(sim, steps, warmup, pairs=False):
+def time_fused(sim, steps, warmup, pairs=False, z_phase=None):
     """Fused engine: events bracket the whole run (all launches of the K steps); a second
     run of K steps brackets, inside the C loop, every interior-sweep launch (the dominant
     kernel) with timing-only HIP events on the fields' stream.
@@ -165,7 +167,7 @@ def time_fused(sim, steps, warmup, pairs=False):
     cells / grid cells), so that bytes x cells / sweep time is the sweep's rate."""
     import torch
     from fdtd_accelerate_amd.fused import FusedStepper
-    st = FusedStepper(sim, step_pairs=pairs)
+    st = FusedStepper(sim, step_pairs=pairs, z_phase=z_phase)
     st.run(warmup, None)
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
@@ -189,7 +191,8 @@ def time_fused(sim, steps, warmup, pairs=False):
         name = (f"k_fused2 (two-step interior sweep {sweep} cells, 60 B/cell per launch = 30 B/cell per step "
                 f"algorithmic, {st.bytes_per_cell:g} B/cell per launch read/written)")
         return ms / steps, wall, kern_ms, name, 30.0 * sweep / cells
-    name = f"k_fused (interior sweep {sweep} cells, 60 B/cell algorithmic, {st.bytes_per_cell:g} B/cell read/written)"
+    name = (f"k_fused (interior sweep {sweep} cells, field arrays at z line phase {st.z_phase}, 60 B/cell "
+            f"algorithmic, {st.bytes_per_cell:g} B/cell read/written)")
     return ms / steps, wall, kern_ms, name, 60.0 * sweep / cells
 
 
@@ -672,7 +675,7 @@ def main():
         ms, wall, kern_ms, kname, kbytes = time_slab(sim, args.steps, args.warmup, args.slab_planes)
     else:
         if args.engine == "fused":
-            ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup, args.pairs)
+            ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup, args.pairs, args.z_phase)
         else:
             ms, wall, kern_ms, kname, kbytes = time_kemp(sim, args.steps, args.warmup)
     n = args.n

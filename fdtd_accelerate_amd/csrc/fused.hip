@@ -77,8 +77,8 @@ struct FGeom {
     int xb, xe;  // planes of the interior box
     int yb, ye;  // rows of the interior box
     int kb, ke;  // z range of the interior box (on 128-byte lines of the field arrays)
-    int zb0;     // first z segment start: kb rounded down to 256 (segments sit on 1 KiB),
-                 // or kb itself when the arrays are not line-aligned (segments on lines)
+    int zb0;     // first z segment start, at or below kb (may be negative: lanes below
+                 // kb idle); segments sit on 1 KiB boundaries of the arrays
     int nseg;    // z segments of 256 floats from zb0 covering [kb, ke)
     int nblocks;
 };
@@ -1080,8 +1080,15 @@ inline int line_phase(const float *e0, int nzp)
 // The first k >= lo and the last k <= hi on a line boundary of arrays of phase ph.
 inline int line_up(int lo, int ph) { return 32 * ((lo + ph + 31) / 32) - ph; }
 inline int line_down(int hi, int ph) { return hi + ph < 0 ? -1 : 32 * ((hi + ph) / 32) - ph; }
-// Segment origin: on 1 KiB boundaries for line-aligned arrays, else at kb (a line).
-inline int seg_origin(int kb, int ph) { return ph ? kb : 256 * (kb / 256); }
+// Segment origin for arrays off the line alignment (kb on a line): the last k <= kb on a
+// 1 KiB boundary of the arrays, so that every segment load of a wave covers one aligned
+// KiB (the lanes before kb stay idle: clamped loads, no stores); kb itself when that k
+// would be negative.  Arrays placed with kb on a 1 KiB boundary start the segments at kb.
+inline int seg_origin(int kb, const float *e0)
+{
+    const int z = kb - (int)((kb + (reinterpret_cast<uintptr_t>(e0) >> 2)) & 255u);
+    return z >= 0 ? z : kb;
+}
 
 inline void add_shell(ShellBoxes &sb, unsigned long long &total, const FGeom &g, int Q)
 {
@@ -1230,7 +1237,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     const bool sweep = g.xe > g.xb;
     g.nrb = sweep ? (g.ye - g.yb + T - 1) / T : 0;
     g.nchunks = sweep ? (g.xe - g.xb + CX - 1) / CX : 0;
-    g.zb0 = ((pml_axes >> 2) & 1) ? seg_origin(g.kb, zph) : 256 * (g.kb / 256);
+    g.zb0 = ((pml_axes >> 2) & 1) && zph ? seg_origin(g.kb, d->e[0]) : 256 * (g.kb / 256);
     g.nseg = sweep ? (g.ke - g.zb0 + 255) / 256 : 0;
     const long long nbl = (long long)g.nrb * g.nchunks;
     if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
@@ -1279,7 +1286,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (pairs) {
             g2.nrb = (g2.ye - g2.yb + T - 1) / T;
             g2.nchunks = (g2.xe - g2.xb + CX - 1) / CX;
-            g2.zb0 = seg_origin(g2.kb, zph);
+            g2.zb0 = zph ? seg_origin(g2.kb, d->e[0]) : 256 * (g2.kb / 256);
             g2.nseg = (g2.ke - g2.zb0 + 255) / 256;
             const long long nb2 = (long long)g2.nrb * g2.nchunks;
             if (nb2 >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");

@@ -29,16 +29,16 @@ ARENA_MIN_BYTES = 1 << 30
 _ARENA_ROUND = 2 << 20
 
 
-def staggered_arrays(count, shape, dtype, device, zero=True, phase=0):
+def staggered_arrays(count, shape, dtype, device, zero=True, offset=0):
     """``count`` arrays of ``shape``.  When one array holds 1 GiB or more they are views
     of one allocation, array k starting k * (size + gap) bytes in, gap = size / 8 rounded
     up to 2 MiB, which fixes their relative placement: twelve separately allocated 4 GiB
     arrays ran the 1024^3 fused step in 11.6-14.0 ms depending on where they landed
     (13.7 ms as the first allocation of a process), carved this way 12.81-12.87 ms in six
     allocations on two GPUs; the in-place step 16.47-16.54 ms against 16.08-16.97 ms.
-    Smaller arrays are allocated one by one.  ``phase`` (elements) starts every array
-    that far past its allocation's alignment (512 B or more), so that its first element
-    sits phase * element-size bytes past a 128-byte line."""
+    Smaller arrays are allocated one by one.  ``offset`` (elements) starts every array
+    that far past an alignment of 2 MiB (arena) or 512 B (own allocation), so that its
+    first element sits offset * element-size bytes past a 1 KiB boundary."""
     n = 1
     for v in shape:
         n *= v
@@ -46,13 +46,13 @@ def staggered_arrays(count, shape, dtype, device, zero=True, phase=0):
     nbytes = n * esize
     make = torch.zeros if zero else torch.empty
     if nbytes < ARENA_MIN_BYTES:
-        if not phase:
+        if not offset:
             return [make(shape, dtype=dtype, device=device) for _ in range(count)]
-        return [make(n + phase, dtype=dtype, device=device)[phase:].view(shape) for _ in range(count)]
+        return [make(n + offset, dtype=dtype, device=device)[offset:].view(shape) for _ in range(count)]
     gap = -(-(nbytes // 8) // _ARENA_ROUND) * _ARENA_ROUND
     step = (nbytes + gap) // esize
-    arena = make((count - 1) * step + n + phase, dtype=dtype, device=device)
-    return [arena[k * step + phase:k * step + phase + n].view(shape) for k in range(count)]
+    arena = make((count - 1) * step + n + offset, dtype=dtype, device=device)
+    return [arena[k * step + offset:k * step + offset + n].view(shape) for k in range(count)]
 
 
 class Fields:

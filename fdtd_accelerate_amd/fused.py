@@ -87,26 +87,26 @@ def _line_phase(t, nzp):
     return (t.data_ptr() >> 2) & 31 if nzp % 32 == 0 else 0
 
 
-def _phased(t, phase):
-    """A copy of the (contiguous) tensor t whose first element sits ``phase`` elements past
-    its allocation's alignment."""
-    buf = torch.empty(t.numel() + phase, dtype=t.dtype, device=t.device)[phase:].view(t.shape)
+def _offset_copy(t, offset):
+    """A copy of the (contiguous) tensor t whose first element sits ``offset`` elements
+    past its allocation's alignment (512 B or more)."""
+    buf = torch.empty(t.numel() + offset, dtype=t.dtype, device=t.device)[offset:].view(t.shape)
     buf.copy_(t)
     return buf
 
 
-def _ping_pong_sets(fields, phase=0):
+def _ping_pong_sets(fields, offset=0):
     """The current and the alternate field set of the ping-pong step (6 arrays each):
     ex, ey, ez, hx, hy, hz of the current set, then of the alternate set, carved together
     by ``staggered_arrays`` (one allocation with staggered starts for arrays of 1 GiB and
-    more), every array starting ``phase`` floats past a 128-byte line; the current values
-    are copied in.  Smaller arrays at phase 0 keep the current set and get a separately
-    allocated alternate set."""
+    more), every array starting ``offset`` floats past a 1 KiB boundary; the current
+    values are copied in.  Smaller arrays at offset 0 keep the current set and get a
+    separately allocated alternate set."""
     cur = fields.ehs
     t0 = cur[0]
-    if not phase and t0.numel() * t0.element_size() < _fields.ARENA_MIN_BYTES:
+    if not offset and t0.numel() * t0.element_size() < _fields.ARENA_MIN_BYTES:
         return list(cur), [torch.empty_like(t) for t in cur]
-    views = _fields.staggered_arrays(12, tuple(t0.shape), t0.dtype, t0.device, zero=False, phase=phase)
+    views = _fields.staggered_arrays(12, tuple(t0.shape), t0.dtype, t0.device, zero=False, offset=offset)
     with torch.cuda.stream(fields.stream):
         for v, t in zip(views[:6], cur):
             v.copy_(t)
@@ -189,12 +189,16 @@ class FusedStepper:
         if z_phase % 4 or not 0 <= z_phase < 32:
             raise ValueError(f"z_phase must be a multiple of 4 in [0, 32) : {z_phase}")
         self.z_phase = z_phase
-        f.ehs, self.alt = _ping_pong_sets(f, z_phase)
+        # the arrays start (-kb) mod 256 floats past a 1 KiB boundary: line phase z_phase,
+        # and k = kb (the sweep's first z cell) on a 1 KiB boundary, where its 1 KiB
+        # segments start
+        kb = 32 * ((npml + 1 + z_phase + 31) // 32) - z_phase
+        offset = (-kb) % 256 if z_phase else 0
+        f.ehs, self.alt = _ping_pong_sets(f, offset)
         if z_phase and f.ce_mat is not None and f.nz_pitch % 128 == 0:
             # CE indices (1 B per cell) placed so that the sweep's segments start on lines
-            kb = 32 * ((npml + 1 + z_phase + 31) // 32) - z_phase
             with torch.cuda.stream(f.stream):
-                f.ce_mat = [_phased(m, (-kb) % 128) for m in f.ce_mat]
+                f.ce_mat = [_offset_copy(m, (-kb) % 128) for m in f.ce_mat]
         T, CX = fused_tiles(f.nx, f.ny, f.nz_pitch, npml, self.pairs)
         self.T = rows_per_thread or T
         self.CX = planes_per_chunk or CX
@@ -209,7 +213,7 @@ class FusedStepper:
                 self.T = min(self.T, 7)
             if self.T > 7:
                 raise ValueError(f"rows_per_thread {self.T}: at most 7 with a Drude box")
-        self.mid = (_fields.staggered_arrays(6, tuple(f.ehs[0].shape), f.ehs[0].dtype, f.device, phase=z_phase)
+        self.mid = (_fields.staggered_arrays(6, tuple(f.ehs[0].shape), f.ehs[0].dtype, f.device, offset=offset)
                     if self.pairs else None)
         # the Drude current of step n+1 (the two sets swap with the field sets)
         self.j_alt = [torch.zeros_like(t) for t in self.drude.J] if self.drude is not None else None
