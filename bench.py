@@ -358,7 +358,7 @@ def run_pbc(args):
     sim.add_probe("ez", (c + n // 4, c, c))
     sim.fields.compress_ce()
     torch.cuda.synchronize()
-    runner = time_fused if fused else time_kemp
+    runner = (lambda s, k, w: time_fused(s, k, w, z_phase=args.z_phase)) if fused else time_kemp
     ms0, _, _, _, _ = runner(sim, args.steps, args.warmup)
     Pbc(sim.fields, "xy")
     ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
@@ -396,7 +396,7 @@ def run_tfsf(args):
     geometry.paint_ce(sim.fields.ces, spheres, (n, n, n))
     sim.fields.compress_ce()
     torch.cuda.synchronize()
-    runner = time_fused if fused else time_kemp
+    runner = (lambda s, k, w: time_fused(s, k, w, z_phase=args.z_phase)) if fused else time_kemp
     ms0, _, _, _, _ = runner(sim, args.steps, args.warmup)
     Tfsf(sim.fields, (lo_box,) * 3, (hi_box,) * 3, ("fb", "fb", "fb"), 20.0, ("normal", "x"), 0.3)
     ms, wall, kern_ms, kname, kbytes = runner(sim, args.steps, args.warmup)
@@ -437,9 +437,9 @@ def run_drude(args):
     if fused:
         # the sweep and shell E pass carry the Drude update: the sweep's 60 B per cell plus
         # 84 B per Drude cell (the box lies inside the sweep box)
-        ms0, _, _, _, _ = time_fused(sim, args.steps, args.warmup)
+        ms0, _, _, _, _ = time_fused(sim, args.steps, args.warmup, z_phase=args.z_phase)
         Drude(sim.fields, p0, p1, 0.5, 0.05)
-        ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup)
+        ms, wall, kern_ms, kname, kbytes = time_fused(sim, args.steps, args.warmup, z_phase=args.z_phase)
         kbytes += 84.0 * nd / cells
         kname = kname.replace("k_fused (", "k_fused with a Drude box (", 1)
     else:

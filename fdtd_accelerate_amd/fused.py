@@ -183,9 +183,13 @@ class FusedStepper:
         self.pairs = bool(full_cpml and self.drude is None and self.tfsf is None and
                           (step_pairs if step_pairs is not None else fused_pairs(f.nx, f.ny, f.nz_pitch, npml)))
         # field arrays placed so that the sweep's z range reaches close to the z CPML
-        # (z_phase overrides the choice; it changes where the work goes, not the results)
+        # (z_phase overrides the choice; it changes where the work goes, not the results).
+        # A Drude box keeps phase 0: its z edges then stay on the sweep's 256-cell segment
+        # boundaries (1024^3 with a [256, 767]^3 box, same box A/B: 16.0 ms/step at phase 0,
+        # 16.4-17.0 at phase 16)
         if z_phase is None:
-            z_phase = z_line_phase(f.nz, f.nz_pitch, npml) if self.cpml_axes & 4 else 0
+            z_phase = (z_line_phase(f.nz, f.nz_pitch, npml)
+                       if self.cpml_axes & 4 and self.drude is None else 0)
         if z_phase % 4 or not 0 <= z_phase < 32:
             raise ValueError(f"z_phase must be a multiple of 4 in [0, 32) : {z_phase}")
         self.z_phase = z_phase

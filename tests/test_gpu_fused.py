@@ -254,6 +254,50 @@ def test_fused_continues_across_calls_and_engines():
         assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
 
 
+@pytest.mark.parametrize("z_phase,drude", [(None, False), (0, False), (8, False), (16, False), (28, False),
+                                            (16, True), (None, True)])
+def test_fused_line_phases_match_inplace(z_phase, drude):
+    """Field arrays placed at each 128-byte line phase (FusedStepper z_phase; None: the
+    automatic choice, 0 with a Drude box) move the sweep's z range and the segment
+    origin, not the results: fields, psi and probes equal the in-place engine, with and
+    without a Drude box crossing segment boundaries."""
+    from fdtd_accelerate_amd import Drude
+    from fdtd_accelerate_amd.fused import FusedStepper, z_line_phase
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, npml, steps = (24, 30, 416), 6, 5
+    nx, ny, nz = ns
+    sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused")]
+    tf = gaussian_pulse(5.0, 3.0, 8.0)
+    rng = np.random.default_rng(9)
+    vals = (0.5 / np.array([1.0, 2.0, 3.5, 7.0, 11.0])).astype(np.float32)
+    mats = [rng.integers(0, len(vals), ns) for _ in range(3)]
+    for s in sims:
+        _fill(s, 17)
+        s.fields.set_ces(*[vals[m] for m in mats])
+        s.add_source("ez", (nx // 2, ny // 2, nz // 2), (nx // 2, ny // 2, nz // 2), tf)
+        s.add_probe("ey", (nx // 2, ny // 2, 20))
+        s.add_probe("hx", (9, 9, 405))
+        if drude:  # across the shell and the 256-cell segment boundaries
+            Drude(s.fields, (5, 5, 20), (18, 24, 300), 0.5, 0.05)
+    assert sims[1].fields.compress_ce()
+    st = FusedStepper(sims[1], rows_per_thread=5, planes_per_chunk=8, z_phase=z_phase)
+    sims[1]._fused = st
+    want = (0 if drude else z_line_phase(nz, sims[1].fields.nz_pitch, npml)) if z_phase is None else z_phase
+    assert st.z_phase == want
+    box = st.sweep_box()
+    assert (box[4] + st.z_phase) % 32 == 0 and box[4] >= npml + 1 and box[5] <= nz - 1 - npml
+    series = [s.run(steps) for s in sims]
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        a = sims[0].field(name).cpu().numpy()
+        b = sims[1].field(name).cpu().numpy()
+        assert same_values(a, b), (name, np.abs(a - b).max())
+    assert same_values(series[0], series[1])
+    for key in sims[0].cpml.psiE:
+        for c in range(2):
+            assert same_values(sims[0].cpml.psiE[key][c].cpu().numpy(), sims[1].cpml.psiE[key][c].cpu().numpy())
+            assert same_values(sims[0].cpml.psiH[key][c].cpu().numpy(), sims[1].cpml.psiH[key][c].cpu().numpy())
+
+
 def test_fused_dipole128_vs_oracle():
     """The 128^3 vacuum dipole run (CPML 10, point source, probe) on the fused engine, 400 steps:
     probe series and fields equal the float32 oracle."""
