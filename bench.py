@@ -57,12 +57,15 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernel (GB), from two rocprofv3 --pmc runs
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors a plain 1 GiB copy gives), keyed by
-# (engine, n, npml, CE storage); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt,
+# (engine, n, npml, CE storage, --z-phase); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt,
 # profiles/update_e_lut_yee_cpml12_1024cube_hbm_counters.txt and
-# profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep).
-MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32"): 64.69,
-                       ("kemp", 1024, 12, "uint8 index + table"): 53.24,
-                       ("fused", 1024, 12, "uint8 index + table"): 51.83}
+# profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep; the
+# automatic line phase is 16 there, the sweep box z [16, 1008); at phase 0 it is [32, 992)).
+MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32", None): 64.69,
+                       ("kemp", 1024, 12, "uint8 index + table", None): 53.24,
+                       ("fused", 1024, 12, "uint8 index + table", None): 53.25,
+                       ("fused", 1024, 12, "uint8 index + table", 16): 53.25,
+                       ("fused", 1024, 12, "uint8 index + table", 0): 51.83}
 # The 4th-order E stage (k_s4_e_zm, CE as uint8 material indices), same method, keyed by
 # (n, npml); profiles/k_s4_zmarch_symplectic_cpml10_512cube_hbm_counters.txt.
 MEASURED_S4_TRAFFIC_GB = {(512, 10): 5.81}
@@ -705,7 +708,8 @@ def main():
         "hbm_frac_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine + ("-pairs" if args.pairs else ""), n, args.npml, ce_storage)), "traffic_unit": "GB per launch",
+                     "traffic": MEASURED_TRAFFIC_GB.get((args.engine + ("-pairs" if args.pairs else ""), n, args.npml, ce_storage,
+                                                         args.z_phase)), "traffic_unit": "GB per launch",
                      "bytes_per_cell": round(kbytes, 3), "avg_launch_ms": round(kern_ms, 4)},
         "wall_s": round(wall, 3),
         "device": torch.cuda.get_device_name(0),
