@@ -994,10 +994,14 @@ __global__ __launch_bounds__(768) void k_fused2(
 // (planes [i0,i1), rows [j0,j1), quads [q0,q1)) with a thread per quad; start[b] is the
 // first thread of box b.  Quads run fastest, then the box's shorter one of planes and
 // rows, so that the i+1 / i-1 and j+1 / j-1 neighbour rows a quad reads are read by
-// nearby threads too (L2 hits).
-constexpr int kMaxShellBoxes = 12;
+// nearby threads too (L2 hits).  A quad range may run past the row's Q quads into the
+// next row (q >= Q: row j+1, quad q-Q): a "seam" box holds the z-high piece of row j
+// and the z-low piece of row j+1, which are adjacent in memory and share a line when
+// the arrays are off the line alignment.
+constexpr int kMaxShellBoxes = 16;  // at most 7 shell boxes + 3 low-fringe + 3 high-fringe boxes
 struct ShellBoxes {
     int n;
+    int Q;  // quads per row
     int i0[kMaxShellBoxes], i1[kMaxShellBoxes], j0[kMaxShellBoxes], j1[kMaxShellBoxes], q0[kMaxShellBoxes],
         q1[kMaxShellBoxes];
     int i_inner[kMaxShellBoxes];  // planes before rows in the thread order
@@ -1018,9 +1022,11 @@ __device__ __forceinline__ void shell_quad(const ShellBoxes &s, unsigned f, int 
     const bool ii = s.i_inner[b] != 0;
     const unsigned inner = ii ? ni : nj;
     const int a = (int)(r % inner), o = (int)(r / inner);
+    const int qq = s.q0[b] + (int)q;
+    const int wrap = qq >= s.Q ? 1 : 0;  // seam boxes: past the row end, the next row
     i = s.i0[b] + (ii ? a : o);
-    j = s.j0[b] + (ii ? o : a);
-    k0 = 4 * (s.q0[b] + (int)q);
+    j = s.j0[b] + (ii ? o : a) + wrap;
+    k0 = 4 * (qq - wrap * s.Q);
 }
 
 // CE: CEM as in the sweep (kCeTable reads the 3 x 256 table from global memory).
@@ -1068,8 +1074,6 @@ inline void add_box(ShellBoxes &sb, unsigned long long &total, int i0, int i1, i
     total += (unsigned long long)(i1 - i0) * (j1 - j0) * (q1 - q0);
 }
 
-// The cells outside g's box as six boxes: x low/high slabs (all rows), y low/high (the
-// box's planes), z low/high (the box's planes and rows).
 // Line phase of the field arrays in floats: k = 0 of every row sits `phase` floats past
 // a 128-byte line (0 when rows do not all start at the same phase, nz_pitch % 32 != 0).
 inline int line_phase(const float *e0, int nzp)
@@ -1090,14 +1094,27 @@ inline int seg_origin(int kb, const float *e0)
     return z >= 0 ? z : kb;
 }
 
+// The cells outside g's box: x low/high slabs (all rows), y low/high (the box's
+// planes), and the z pieces of the box's planes and rows -- as seams (the z-high piece
+// of row j with the z-low piece of row j+1, one contiguous run) plus the z-low piece of
+// the first row and the z-high piece of the last when both pieces exist, else as z
+// low/high boxes.
 inline void add_shell(ShellBoxes &sb, unsigned long long &total, const FGeom &g, int Q)
 {
+    sb.Q = Q;
     add_box(sb, total, 0, g.xb, 0, g.ny, 0, Q);
     add_box(sb, total, g.xe, g.nx, 0, g.ny, 0, Q);
     add_box(sb, total, g.xb, g.xe, 0, g.yb, 0, Q);
     add_box(sb, total, g.xb, g.xe, g.ye, g.ny, 0, Q);
-    add_box(sb, total, g.xb, g.xe, g.yb, g.ye, 0, g.kb / 4);
-    add_box(sb, total, g.xb, g.xe, g.yb, g.ye, g.ke / 4, Q);
+    const int ql = g.kb / 4, qh = g.ke / 4;
+    if (ql > 0 && qh < Q && g.ye - g.yb >= 2) {
+        add_box(sb, total, g.xb, g.xe, g.yb, g.yb + 1, 0, ql);
+        add_box(sb, total, g.xb, g.xe, g.yb, g.ye - 1, qh, Q + ql);
+        add_box(sb, total, g.xb, g.xe, g.ye - 1, g.ye, qh, Q);
+    } else {
+        add_box(sb, total, g.xb, g.xe, g.yb, g.ye, 0, ql);
+        add_box(sb, total, g.xb, g.xe, g.yb, g.ye, qh, Q);
+    }
 }
 
 // The box's low fringe (plane xb, row yb, quad kb/4: the box cells whose step n+1 H the
