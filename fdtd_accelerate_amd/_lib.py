@@ -25,6 +25,7 @@ class CpmlHalf(ctypes.Structure):
 MAX_SRC = 4
 MAX_PROBE = 8
 TFSF_MAX_PATCH = 12
+MAX_CE_TRI = 4096  # MIFDTD_FUSED_MAX_CE_TRI
 
 
 class TfsfPatches(ctypes.Structure):
@@ -51,6 +52,7 @@ class FusedDesc(ctypes.Structure):
         ("tfsf_e", TfsfPatches), ("tfsf_h", TfsfPatches), ("tfsf_line_e", _P), ("tfsf_line_h", _P),
         ("tfsf_abc", _P), ("tfsf_line_n", _I), ("tfsf_src", _P),
         ("cpml_axes", _I), ("npbc", _I), ("pbc_axes", _I * 3),
+        ("ce_tri", _P), ("ce_tri_lut", _P), ("ce_tri_n", _I),
     ]
 
 

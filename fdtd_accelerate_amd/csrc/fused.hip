@@ -81,13 +81,22 @@ struct FGeom {
                  // kb idle); segments sit on 1 KiB boundaries of the arrays
     int nseg;    // z segments of 256 floats from zb0 covering [kb, ke)
     int nblocks;
+    int ntri;    // kCeTriple: CE triples in the table
 };
+
+// Floats of the CE table at the start of the sweep's LDS (a multiple of 4: the row
+// exchange buffers after it take 16-byte stores).
+__host__ __device__ constexpr int ce_tab_floats(int cem, int ntri)
+{
+    return cem == kCeTable ? 768 : cem == kCeTriple ? 4 * ((3 * ntri + 3) / 4) : 0;
+}
 
 struct Ctx {  // buffers of one launch (restrict-qualified at the kernel boundary)
     const float *ex, *ey, *ez, *hx, *hy, *hz;   // step n
     float *oex, *oey, *oez, *ohx, *ohy, *ohz;   // step n+1
     const float *cex, *cey, *cez;               // CE arrays (kCeArrays)
     const unsigned char *mx, *my, *mz;          // CE material indices (kCeTable)
+    const unsigned short *mt;                   // CE triple indices (kCeTriple)
 };
 
 // CE of the quad at element rb + lc (rb uniform, lc the lane's element offset)
@@ -156,9 +165,21 @@ __device__ __forceinline__ void compute_E(const FGeom &g, const Ctx &c, const fl
     const Q4 e0 = ld4(c.ex + rb + lc);
     const Q4 e1 = ld4(c.ey + rb + lc);
     const Q4 e2 = ld4(c.ez + rb + lc);
-    const Q4 c0 = load_ce<CEM>(c.cex, c.mx, tab, rb, lc);
-    const Q4 c1 = load_ce<CEM>(c.cey, c.my, tab + 256, rb, lc);
-    const Q4 c2 = load_ce<CEM>(c.cez, c.mz, tab + 512, rb, lc);
+    Q4 c0, c1, c2;
+    if (CEM == kCeTriple) {  // one uint16 per cell into (cex, cey, cez) triples
+        const uint2 u = *reinterpret_cast<const uint2 *>(c.mt + rb + lc);
+        const unsigned t[4] = {u.x & 0xffffu, u.x >> 16, u.y & 0xffffu, u.y >> 16};
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            c0.v[l] = tab[3 * t[l]];
+            c1.v[l] = tab[3 * t[l] + 1];
+            c2.v[l] = tab[3 * t[l] + 2];
+        }
+    } else {
+        c0 = load_ce<CEM>(c.cex, c.mx, tab, rb, lc);
+        c1 = load_ce<CEM>(c.cey, c.my, tab + 256, rb, lc);
+        c2 = load_ce<CEM>(c.cez, c.mz, tab + 512, rb, lc);
+    }
     // k+1 neighbours of H(i,j)
     float hy1 = __shfl_down(hc[1].v[0], 1);
     float hx1 = __shfl_down(hc[0].v[0], 1);
@@ -298,18 +319,18 @@ __global__ __launch_bounds__((FEAT & kFeatDrude) ? 64 * (kDrudeMaxRows + 1) : WI
     const float *__restrict__ cex, const float *__restrict__ cey, const float *__restrict__ cez,
     const unsigned char *__restrict__ mx, const unsigned char *__restrict__ my,
     const unsigned char *__restrict__ mz, const float *__restrict__ lut, FSrc src, FProbe prb, SDrude dr,
-    STfsf tfe, STfsf tfh)
+    STfsf tfe, STfsf tfh, const unsigned short *__restrict__ mt)
 {
     constexpr bool DR = (FEAT & kFeatDrude) != 0;
     constexpr bool TF = (FEAT & kFeatTfsf) != 0;
-    const Ctx c{ex, ey, ez, hx, hy, hz, oex, oey, oez, ohx, ohy, ohz, cex, cey, cez, mx, my, mz};
+    const Ctx c{ex, ey, ez, hx, hy, hz, oex, oey, oez, ohx, ohy, ohz, cex, cey, cez, mx, my, mz, mt};
     extern __shared__ float lds[];
     __shared__ FSrc s_src;
     __shared__ FProbe s_prb;
     __shared__ STfsf s_tf[TF ? 2 : 1];  // TFSF E and H patch tables (read from LDS, not kernel arguments)
     const int T = g.T;
-    float *tab = lds;                                 // [3][256] CE table (kCeTable)
-    float *ebuf = lds + (CEM == kCeTable ? 768 : 0);  // [2][T][2][256]: ex, ez of rows w = 0..T-1
+    float *tab = lds;  // [3][256] CE table (kCeTable) or [ntri][3] triples (kCeTriple)
+    float *ebuf = lds + ce_tab_floats(CEM, g.ntri);  // [2][T][2][256]: ex, ez of rows w = 0..T-1
     float *hbuf = ebuf + 2 * T * 512;                 // [2][T][2][256]: hx, hz of H(i+1) of rows w = 1..T
     float *cbuf = hbuf + 2 * T * 512;                 // [cx+1][T+1][2]: ex, ey at the segment's last cell
     float *tline = cbuf + (size_t)(g.cx + 1) * (T + 1) * 2;  // TFSF: [2][line_n] the incident line's H, E
@@ -347,8 +368,8 @@ __global__ __launch_bounds__((FEAT & kFeatDrude) ? 64 * (kDrudeMaxRows + 1) : WI
     if (TF)
         for (int t = threadIdx.x; t < 2 * tfe.line_n; t += blockDim.x)
             tline[t] = t < tfe.line_n ? tfe.line[t] : tfh.line[t - tfe.line_n];
-    if (CEM == kCeTable)
-        for (int t = threadIdx.x; t < 768; t += blockDim.x) tab[t] = lut[t];
+    if (CEM == kCeTable || CEM == kCeTriple)
+        for (int t = threadIdx.x; t < (CEM == kCeTable ? 768 : 3 * g.ntri); t += blockDim.x) tab[t] = lut[t];
     __syncthreads();
     const FSrc *sp = src_hit ? &s_src : nullptr;
 
@@ -1259,7 +1280,18 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
     const long long nbl = (long long)g.nrb * g.nchunks;
     if (nbl >= (1ll << 31)) return fail(MIFDTD_ERANGE, "too many blocks");
     g.nblocks = (int)nbl;
-    const size_t lds_bytes = sizeof(float) * ((cem == kCeTable ? 768 : 0) + (size_t)2 * 2 * T * 2 * 256 +
+    // CE of the plain sweep as one uint16 triple index per cell (the shell keeps cem)
+    if (d->ce_tri) {
+        if (!d->ce_tri_lut) return fail(MIFDTD_ENULL, "CE triple table is NULL");
+        if (d->ce_tri_n < 1 || d->ce_tri_n > MIFDTD_FUSED_MAX_CE_TRI)
+            return fail(MIFDTD_EINVAL, "ce_tri_n %d outside [1,%d]", d->ce_tri_n, MIFDTD_FUSED_MAX_CE_TRI);
+        if (reinterpret_cast<uintptr_t>(d->ce_tri) & 7u) return fail(MIFDTD_EALIGN, "CE triple indices not 8-byte aligned");
+        if (cem != kCeTable) return fail(MIFDTD_EINVAL, "CE triple indices need the material indices (ce_mat) too");
+    }
+    const bool plain_sweep = d->drude_j[0] == nullptr && d->tfsf_line_n <= 0;
+    const int sweep_cem = d->ce_tri && plain_sweep ? (int)kCeTriple : cem;
+    g.ntri = sweep_cem == kCeTriple ? d->ce_tri_n : 0;
+    const size_t lds_bytes = sizeof(float) * ((size_t)ce_tab_floats(sweep_cem, g.ntri) + (size_t)2 * 2 * T * 2 * 256 +
                                               (size_t)(CX + 1) * (T + 1) * 2 +
                                               (d->tfsf_line_n > 0 ? 2 * (size_t)d->tfsf_line_n : 0));
     if (lds_bytes + sizeof(FSrc) + sizeof(FProbe) + 2 * sizeof(STfsf) > kMaxLds)
@@ -1561,14 +1593,16 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
 #define MIFDTD_FUSED_LAUNCH_W(CEV, FV, W)                                                                            \
     k_fused<CEV, FV, W><<<g.nblocks, nthr, lds_bytes, st>>>(g, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], \
                                                             out[2], out[3], out[4], out[5], d->ce[0], d->ce[1], d->ce[2], \
-                                                            d->ce_mat[0], d->ce_mat[1], d->ce_mat[2], d->ce_lut, fs, fp,  \
-                                                            sdr, tfe, tfh)
+                                                            d->ce_mat[0], d->ce_mat[1], d->ce_mat[2],                     \
+                                                            sweep_cem == kCeTriple ? d->ce_tri_lut : d->ce_lut, fs, fp,   \
+                                                            sdr, tfe, tfh, d->ce_tri)
 #define MIFDTD_FUSED_LAUNCH(CEV, FV)                                                                              \
     do {                                                                                                          \
         if (((FV) & kFeatDrude) || !wide) MIFDTD_FUSED_LAUNCH_W(CEV, FV, false);                                  \
         else MIFDTD_FUSED_LAUNCH_W(CEV, FV, !((FV) & kFeatDrude));                                                \
     } while (0)
-            if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTable, feat);
+            if (sweep_cem == kCeTriple) MIFDTD_FUSED_LAUNCH(kCeTriple, 0);
+            else if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTable, feat);
             else if (cem == kCeArrays) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeArrays, feat);
             else MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeConst, feat);
 #undef MIFDTD_FUSED_LAUNCH

@@ -235,7 +235,7 @@ __device__ __forceinline__ void sprobe(const SProbe &p, int c, int i, int j, int
 // material-index arrays into a 3 x 256 float table (same float values, so the products
 // are identical; 3 B per cell instead of 12).  The per-quad passes read the 3 KiB table
 // from global memory, where it stays in L1/L2; the sweep kernels stage it in LDS.
-enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2 };
+enum CeMode { kCeConst = 0, kCeArrays = 1, kCeTable = 2, kCeTriple = 3 };  // kCeTriple: the fused sweep only
 
 // CE of a quad as uint8 material indices m[0..3] into a 256-entry float table.
 __device__ __forceinline__ Q4 ce_from_table(const unsigned char *__restrict__ m, const float *tab)

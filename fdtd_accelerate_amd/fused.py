@@ -223,12 +223,49 @@ class FusedStepper:
         self.j_alt = [torch.zeros_like(t) for t in self.drude.J] if self.drude is not None else None
         if cp is not None:
             cp.alt_psiE()
+        # CE of the plain sweep as one uint16 triple index per cell (rebuilt in _desc
+        # whenever the material indices change)
+        self._tri_kb = kb if z_phase else 0
+        self.ce_tri = self.ce_tri_lut = self._tri_src = None
+        self._build_ce_triples()
+
+    def _build_ce_triples(self):
+        """One uint16 index per cell into the distinct (cex, cey, cez) triples of the
+        material indices (``Fields.compress_ce``), read by the plain sweep instead of the
+        three uint8 indices (2 B per cell instead of 3; the same float values).  Skipped
+        with a Drude box or a TFSF source, more than MAX_CE_TRI triples, or a table that
+        does not fit the sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
+        the sweep's segments start on lines."""
+        f = self.sim.fields
+        self.ce_tri = self.ce_tri_lut = None
+        self._tri_src = f.ce_mat
+        if f.ce_mat is None or self.drude is not None or self.tfsf is not None:
+            return
+        m = f.ce_mat
+        with torch.cuda.stream(f.stream):
+            key = m[0].to(torch.int32) | (m[1].to(torch.int32) << 8) | (m[2].to(torch.int32) << 16)
+            vals, inv = torch.unique(key.view(-1), return_inverse=True)
+            del key
+            n = vals.numel()
+            if n > _lib.MAX_CE_TRI or not lds_fits(self.T, self.CX, True, n):
+                return
+            lut = torch.stack([f.ce_lut[c][(vals >> (8 * c)) & 255] for c in range(3)], dim=1).contiguous()
+            off = ((-2 * self._tri_kb) % 128) // 2
+            buf = torch.empty(inv.numel() + off, dtype=torch.int16, device=f.device)
+            buf[off:].copy_(inv)
+            del inv
+            self.ce_tri = buf[off:].view(m[0].shape)
+            self.ce_tri_lut = lut
+        f.stream.synchronize()
 
     @property
     def bytes_per_cell(self):
         """Algorithmic HBM bytes per cell of one step (without CPML): E, H read and
-        written (48 B) plus CE read as float arrays (12 B) or uint8 indices (3 B)."""
+        written (48 B) plus CE read as float arrays (12 B), uint8 indices (3 B) or one
+        uint16 triple index (2 B)."""
         f = self.sim.fields
+        if getattr(self, "ce_tri", None) is not None:
+            return 50.0
         return 51.0 if f.ce_mat is not None else 60.0 if f.ce_on else 48.0
 
     def _desc(self):
@@ -246,6 +283,13 @@ class FusedStepper:
             for c in range(3):
                 d.ce_mat[c] = f.ce_mat[c].data_ptr()
             d.ce_lut = f.ce_lut.data_ptr()
+            if hasattr(self, "ce_tri"):
+                if f.ce_mat is not self._tri_src:  # new material indices: new triples
+                    self._build_ce_triples()
+                if self.ce_tri is not None:
+                    d.ce_tri = self.ce_tri.data_ptr()
+                    d.ce_tri_lut = self.ce_tri_lut.data_ptr()
+                    d.ce_tri_n = self.ce_tri_lut.shape[0]
         cp = sim.cpml
         if cp is not None:
             d.npml = cp.npml

@@ -334,7 +334,16 @@ typedef struct {
     int cpml_axes;
     int npbc;
     int pbc_axes[3];
+    /* fused only, optional with ce_mat: CE as one uint16 index per cell (an array like a
+     * field, 8-byte aligned) into a DEVICE table of ce_tri_n <= MIFDTD_FUSED_MAX_CE_TRI
+     * (cex, cey, cez) float triples -- the same values as ce_mat/ce_lut, 2 B per cell
+     * instead of 3.  Read by the interior sweep of a plain step (no Drude box, no TFSF
+     * source, single steps); the shell passes keep ce_mat. */
+    const unsigned short *ce_tri;
+    const float *ce_tri_lut;
+    int ce_tri_n;
 } mifdtd_fused_desc;
+#define MIFDTD_FUSED_MAX_CE_TRI 4096
 
 /* Run nsteps fused steps.  tvals: HOST array (nsteps, nsrc) of the float32 time values
  * of each source; probe_series: DEVICE array (nsteps, nprobe).  On return *parity is 1

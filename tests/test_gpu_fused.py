@@ -100,8 +100,9 @@ def test_fused_matches_inplace(ns, npml, T, CX, steps):
     ((29, 40, 300), 5, 15, 16, 3),
 ])
 def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
-    """CE held as uint8 material indices + table (Fields.compress_ce) on the fused engine:
-    equal to the in-place engine reading the float CE arrays."""
+    """CE held as uint8 material indices + table (Fields.compress_ce) on the fused engine,
+    the sweep reading one uint16 triple index per cell: equal to the in-place engine
+    reading the float CE arrays, also after new CE values mid-run (indices rebuilt)."""
     from fdtd_accelerate_amd.sources import gaussian_pulse
     sims = [_sim(ns, npml, "kemp"), _sim(ns, npml, "fused", T, CX)]
     nx, ny, nz = ns
@@ -116,10 +117,25 @@ def test_fused_ce_table_matches_inplace(ns, npml, T, CX, steps):
         s.add_probe("ey", (nx // 2, ny // 2 + 1, nz // 2))
     assert sims[1].fields.compress_ce()
     series = [s.run(steps) for s in sims]
+    st = sims[1]._fused
+    # the plain sweep reads one uint16 triple index per cell (7 values per component)
+    assert st.ce_tri is not None and st.ce_tri_lut.shape[0] <= 7 ** 3 and st.bytes_per_cell == 50.0
     for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
         a = sims[0].field(name).cpu().numpy()
         b = sims[1].field(name).cpu().numpy()
         assert same_values(a, b), (name, np.abs(a - b).max())
+    assert same_values(series[0], series[1])
+    # new CE values mid-run: the material indices and the triples are rebuilt
+    vals2 = (0.5 / np.array([1.3, 2.0, 5.0])).astype(np.float32)
+    mats2 = [rng.integers(0, len(vals2), (nx, ny, nz)) for _ in range(3)]
+    old = st.ce_tri
+    for s in sims:
+        s.fields.set_ces(*[vals2[m] for m in mats2])
+    assert sims[1].fields.compress_ce()
+    series = [s.run(steps) for s in sims]
+    assert st.ce_tri is not None and st.ce_tri is not old and st.ce_tri_lut.shape[0] <= 4 ** 3  # pads keep 0.5
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert same_values(sims[0].field(name).cpu().numpy(), sims[1].field(name).cpu().numpy()), ("new CE", name)
     assert same_values(series[0], series[1])
 
 
