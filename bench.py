@@ -60,11 +60,12 @@ HBM_PEAK_GBS = 8000.0
 # (engine, n, npml, CE storage, --z-phase); profiles/update_e_yee_cpml12_1024cube_hbm_counters.txt,
 # profiles/update_e_lut_yee_cpml12_1024cube_hbm_counters.txt and
 # profiles/k_fused_sweep_yee_cpml12_1024cube_hbm_counters.txt (the interior sweep; the
-# automatic line phase is 16 there, the sweep box z [16, 1008); at phase 0 it is [32, 992)).
+# automatic line phase is 16 there, the sweep box z [16, 1008), CE read as uint16 triple
+# indices; at phase 0 the box is [32, 992) and the measurement predates the triple indices).
 MEASURED_TRAFFIC_GB = {("kemp", 1024, 12, "float32", None): 64.69,
                        ("kemp", 1024, 12, "uint8 index + table", None): 53.24,
-                       ("fused", 1024, 12, "uint8 index + table", None): 53.25,
-                       ("fused", 1024, 12, "uint8 index + table", 16): 53.25,
+                       ("fused", 1024, 12, "uint8 index + table", None): 52.36,
+                       ("fused", 1024, 12, "uint8 index + table", 16): 52.36,
                        ("fused", 1024, 12, "uint8 index + table", 0): 51.83}
 # The 4th-order E stage (k_s4_e_zm, CE as uint8 material indices), same method, keyed by
 # (n, npml); profiles/k_s4_zmarch_symplectic_cpml10_512cube_hbm_counters.txt.
@@ -188,7 +189,7 @@ def time_fused(sim, steps, warmup, pairs=False, z_phase=None):
     sweep = st.sweep_cells()
     # algorithmic bytes on the per-cell basis of the plain Yee step: read 6 fields and 3 CE,
     # write 6 fields = 60 B (the sweep box holds no CPML cell); the kernel itself reads CE
-    # as 3 B of material indices, so it moves 51 B of those 60
+    # as 3 B of material indices or a 2 B triple index, so it moves 51 or 50 B of those 60
     if st.pairs:
         # one launch advances its box two steps: state and CE read once, state written once
         name = (f"k_fused2 (two-step interior sweep {sweep} cells, 60 B/cell per launch = 30 B/cell per step "
