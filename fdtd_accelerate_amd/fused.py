@@ -241,20 +241,33 @@ class FusedStepper:
         self._tri_src = f.ce_mat
         if f.ce_mat is None or self.drude is not None or self.tfsf is not None:
             return
-        m = f.ce_mat
+        flat = [x.view(-1) for x in f.ce_mat]
+        total = flat[0].numel()
+        chunk = 1 << 26
+
+        def keys(s):  # the index triple of each cell as one 24-bit key
+            return (flat[0][s:s + chunk].to(torch.int32) | (flat[1][s:s + chunk].to(torch.int32) << 8) |
+                    (flat[2][s:s + chunk].to(torch.int32) << 16)).long()
+
         with torch.cuda.stream(f.stream):
-            key = m[0].to(torch.int32) | (m[1].to(torch.int32) << 8) | (m[2].to(torch.int32) << 16)
-            vals, inv = torch.unique(key.view(-1), return_inverse=True)
-            del key
+            seen = torch.zeros(1 << 24, dtype=torch.bool, device=f.device)
+            for s in range(0, total, chunk):
+                seen[keys(s)] = True
+            vals = torch.nonzero(seen).view(-1)  # the distinct keys, ascending
+            del seen
             n = vals.numel()
             if n > _lib.MAX_CE_TRI or not lds_fits(self.T, self.CX, True, n):
                 return
             lut = torch.stack([f.ce_lut[c][(vals >> (8 * c)) & 255] for c in range(3)], dim=1).contiguous()
+            code = torch.zeros(1 << 24, dtype=torch.int16, device=f.device)
+            code[vals] = torch.arange(n, dtype=torch.int16, device=f.device)
             off = ((-2 * self._tri_kb) % 128) // 2
-            buf = torch.empty(inv.numel() + off, dtype=torch.int16, device=f.device)
-            buf[off:].copy_(inv)
-            del inv
-            self.ce_tri = buf[off:].view(m[0].shape)
+            buf = torch.empty(total + off, dtype=torch.int16, device=f.device)
+            tri = buf[off:]
+            for s in range(0, total, chunk):
+                tri[s:s + chunk] = code[keys(s)]
+            del code
+            self.ce_tri = tri.view(f.ce_mat[0].shape)
             self.ce_tri_lut = lut
         f.stream.synchronize()
 
