@@ -284,6 +284,31 @@ def test_batch_ce_index_arguments_rejected_without_gpu():
     assert not out.value
 
 
+def test_z_line_phase_choice():
+    """The fused engine's line phase: a multiple of 4 (16-byte quads) putting the sweep's
+    z range [kb, ke) on line boundaries (k + phase a multiple of 32) inside the CPML-free
+    cells [npml+1, nz-1-npml), with the fewest cells left between (checked against every
+    phase, and against a brute-force scan of line boundaries); 0 when rows of nz_pitch
+    floats do not share one phase."""
+    from fdtd_accelerate_amd.fused import z_line_phase
+    assert z_line_phase(1024, 1024, 12) == 16
+    assert z_line_phase(1000, 1008, 4) == 0 and z_line_phase(40, 48, 4) == 0
+
+    def box(nz, p, ph):  # brute force: the first and last line boundaries in the CPML-free range
+        lines = [k for k in range(nz + 1) if (k + ph) % 32 == 0 and p + 1 <= k <= nz - 1 - p]
+        return (lines[0], lines[-1]) if len(lines) >= 2 else None
+
+    for nz, nzp, p in [(1024, 1024, 12), (160, 160, 6), (416, 416, 6), (512, 512, 10), (128, 128, 10),
+                       (57, 64, 5), (300, 320, 5), (96, 96, 8)]:
+        ph = z_line_phase(nz, nzp, p)
+        assert ph % 4 == 0 and 0 <= ph < 32
+        waste = {q: (nz - 2 - 2 * p) - (b[1] - b[0]) for q in range(0, 32, 4) if (b := box(nz, p, q))}
+        if not waste:
+            assert ph == 0
+            continue
+        assert ph in waste and waste[ph] == min(waste.values()), (nz, p, ph, waste)
+
+
 def test_ctypes_mirrors_match_header(tmp_path):
     """Every ctypes mirror in _lib has the size and field offsets of its C struct in
     include/mifdtd.h (a C probe compiled with gcc against the header prints them)."""
