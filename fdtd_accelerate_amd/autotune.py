@@ -32,12 +32,13 @@ def _load():
     return _table
 
 
-def lds_fits(T, CX, table, ntri=0):
+def lds_fits(T, CX, table, ntri=0, extra=0):
     """Whether the sweep's LDS (E and H row exchange 2 x 2 x T x 2 x 256 floats, the k-1
-    carry (CX+1)(T+1)2 floats, the 3 x 256 CE table or ``ntri`` CE triples, ~0.4 KiB of
-    source/probe tables) fits the 160 KiB of one gfx950 work-group."""
+    carry (CX+1)(T+1)2 floats, the 3 x 256 CE table or ``ntri`` CE triples, ``extra``
+    more floats, ~0.4 KiB of source/probe tables) fits the 160 KiB of one gfx950
+    work-group."""
     tab = 4 * ((3 * ntri + 3) // 4) if ntri else (768 if table else 0)
-    return 4 * (tab + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2) + 512 <= 160 * 1024
+    return 4 * (tab + 2 * 2 * T * 2 * 256 + (CX + 1) * (T + 1) * 2 + extra) + 512 <= 160 * 1024
 
 
 def lds_fits_pairs(T, CX, table):

@@ -233,13 +233,13 @@ class FusedStepper:
         """One uint16 index per cell into the distinct (cex, cey, cez) triples of the
         material indices (``Fields.compress_ce``), read by the plain sweep instead of the
         three uint8 indices (2 B per cell instead of 3; the same float values).  Skipped
-        with a Drude box or a TFSF source, more than MAX_CE_TRI triples, or a table that
-        does not fit the sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
+        with a Drude box, more than MAX_CE_TRI triples, or a table that does not fit the
+        sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
         the sweep's segments start on lines."""
         f = self.sim.fields
         self.ce_tri = self.ce_tri_lut = None
         self._tri_src = f.ce_mat
-        if f.ce_mat is None or self.drude is not None or self.tfsf is not None:
+        if f.ce_mat is None or self.drude is not None:
             return
         flat = [x.view(-1) for x in f.ce_mat]
         total = flat[0].numel()
@@ -256,7 +256,9 @@ class FusedStepper:
             vals = torch.nonzero(seen).view(-1)  # the distinct keys, ascending
             del seen
             n = vals.numel()
-            if n > _lib.MAX_CE_TRI or not lds_fits(self.T, self.CX, True, n):
+            # a TFSF source stages its line (2 floats per point) and two patch tables in LDS
+            extra = 2 * self.tfsf.n_line + 1024 if self.tfsf is not None else 0
+            if n > _lib.MAX_CE_TRI or not lds_fits(self.T, self.CX, True, n, extra):
                 return
             lut = torch.stack([f.ce_lut[c][(vals >> (8 * c)) & 255] for c in range(3)], dim=1).contiguous()
             code = torch.zeros(1 << 24, dtype=torch.int16, device=f.device)
