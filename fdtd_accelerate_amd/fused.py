@@ -223,7 +223,7 @@ class FusedStepper:
         self.j_alt = [torch.zeros_like(t) for t in self.drude.J] if self.drude is not None else None
         if cp is not None:
             cp.alt_psiE()
-        # CE of the plain sweep as one uint16 triple index per cell (rebuilt in _desc
+        # CE of the sweep as one uint16 triple index per cell (rebuilt in _desc
         # whenever the material indices change)
         self._tri_kb = kb if z_phase else 0
         self.ce_tri = self.ce_tri_lut = self._tri_src = None
@@ -231,7 +231,7 @@ class FusedStepper:
 
     def _build_ce_triples(self):
         """One uint16 index per cell into the distinct (cex, cey, cez) triples of the
-        material indices (``Fields.compress_ce``), read by the plain sweep instead of the
+        material indices (``Fields.compress_ce``), read by the sweep instead of the
         three uint8 indices (2 B per cell instead of 3; the same float values).  Skipped
         with a Drude box, more than MAX_CE_TRI triples, or a table that does not fit the
         sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
