@@ -132,6 +132,13 @@ class YeeSim:
 
     def _advance(self, nsteps, buf):
         f = self.fields
+        # the whole-step engines take the update instances (CPML faces, periodic axes, a
+        # Drude box, a TFSF source) when they are built: a changed instance list builds
+        # them again (the state carries over)
+        instances = tuple(id(i) for i in f.instance_list)
+        if instances != getattr(self, "_stepper_instances", instances):
+            self._fused = self._slab = None
+        self._stepper_instances = instances
         if self.engine == "fused":
             from .fused import FusedStepper
             if self._fused is None:

@@ -270,6 +270,35 @@ def test_fused_continues_across_calls_and_engines():
         assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
 
 
+def test_instances_added_mid_run():
+    """A Drude box added after the whole-step engines have run rebuilds them (the state
+    carries over): fused and slab equal the in-place engine, and a slab run with the box
+    is refused."""
+    from fdtd_accelerate_amd import Drude
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, npml = (26, 28, 72), 4
+    tf = gaussian_pulse(4.0, 2.0, 8.0)
+    a = _sim(ns, npml, "kemp")
+    b = _sim(ns, npml, "fused", 3, 8)
+    for s in (a, b):
+        _fill(s, 21)
+        s.add_source("ez", (13, 14, 30), (13, 14, 30), tf)
+        s.run(3)
+    first = b._fused
+    for s in (a, b):
+        Drude(s.fields, (6, 6, 20), (18, 20, 50), 0.5, 0.05)
+        s.run(4)
+    assert b._fused is not first and b._fused.drude is not None
+    for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert same_values(a.field(name).cpu().numpy(), b.field(name).cpu().numpy()), name
+    c = _slab(ns, npml, 4)
+    _fill(c, 21)
+    c.run(2)
+    Drude(c.fields, (6, 6, 20), (18, 20, 50), 0.5, 0.05)
+    with pytest.raises(ValueError, match="Drude"):
+        c.run(1)
+
+
 @pytest.mark.parametrize("z_phase,drude", [(None, False), (0, False), (8, False), (16, False), (28, False),
                                             (16, True), (None, True)])
 def test_fused_line_phases_match_inplace(z_phase, drude):
