@@ -98,6 +98,40 @@ class BatchedYee:
     def __init__(self, sims, use_graph=None, remap_rows=None, stream=None):
         if not sims:
             raise ValueError("need at least one simulation")
+        self.sims = list(sims)
+        self.device = sims[0].fields.device
+        self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
+        self._h = None
+        self._create()
+        key = self.key()
+        tuned = _load_table().get(key, {})
+        self.use_graph = bool(tuned.get("use_graph", False)) if use_graph is None else bool(use_graph)
+        self.remap_rows = int(tuned.get("remap_rows", 0)) if remap_rows is None else int(remap_rows)
+
+    def _state(self):
+        """What the device descriptors hold: every buffer address, the source and probe
+        counts and the update instances of each simulation."""
+        out = []
+        for s in self.sims:
+            f = s.fields
+            out += [t.data_ptr() for t in f.ehs]
+            out += [t.data_ptr() for t in f.ces] if f.ce_on else []
+            out += [t.data_ptr() for t in f.ce_mat] + [f.ce_lut.data_ptr()] if f.ce_mat is not None else [0]
+            out += [s.cpml.psiE_buf.data_ptr(), s.cpml.psiH_buf.data_ptr()] if s.cpml is not None else [0]
+            out += [0 if src.sv is None else src.sv.data_ptr() for src in s.sources]
+            out += [len(s.sources), len(s.probe_points), tuple(id(i) for i in f.instance_list)]
+        return tuple(out)
+
+    def _refresh(self):
+        """Build the device descriptors again when a simulation's buffers, sources,
+        probes or update instances changed since they were built (another engine may have
+        moved the field arrays or swapped the psi buffers)."""
+        if self._state() != self._built:
+            self.close()
+            self._create()
+
+    def _create(self):
+        sims = self.sims
         npmls = {s.cpml.npml for s in sims if s.cpml is not None}
         if len(npmls) > 1:
             raise ValueError(f"all instances must share one npml : {sorted(npmls)}")
@@ -121,10 +155,7 @@ class BatchedYee:
             for cls, what in in_place_only():
                 if any(isinstance(i, cls) for i in s.fields.instance_list):
                     raise ValueError(f"the batched engine has no {what}")
-        self.sims = list(sims)
-        self.device = sims[0].fields.device
-        self.stream = stream if stream is not None else torch.cuda.Stream(self.device)
-        descs, tabs = instance_descs(self.sims)
+        descs, tabs = instance_descs(sims)
         h = ctypes.c_void_p()
         _lib.call("mifdtd_batch_create", descs, len(sims), *tabs, ctypes.byref(h))
         self._h = h
@@ -132,10 +163,7 @@ class BatchedYee:
         _lib.call("mifdtd_batch_sizes", h, ctypes.byref(ns), ctypes.byref(npb), ctypes.byref(nb))
         self.total_src, self.total_probe, self.total_blocks = ns.value, npb.value, nb.value
         self.cells = sum(s.fields.nx * s.fields.ny * s.fields.nz for s in sims)
-        key = self.key()
-        tuned = _load_table().get(key, {})
-        self.use_graph = bool(tuned.get("use_graph", False)) if use_graph is None else bool(use_graph)
-        self.remap_rows = int(tuned.get("remap_rows", 0)) if remap_rows is None else int(remap_rows)
+        self._built = self._state()
 
     def key(self):
         return json.dumps(sorted((s.fields.nx, s.fields.ny, s.fields.nz) for s in self.sims) +
@@ -158,6 +186,7 @@ class BatchedYee:
     def run(self, nsteps, collect=True):
         """Advance every instance nsteps steps; returns the per-instance probe series
         (list of host arrays) when ``collect``."""
+        self._refresh()
         tv = self._tvals(nsteps)
         buf = torch.zeros((max(nsteps, 1), max(self.total_probe, 1)), dtype=torch.float32, device=self.device)
         # the batch stream runs after the work already queued on the instances' stream
@@ -184,6 +213,7 @@ class BatchedYee:
 
     def time_steps(self, nsteps):
         """Average ms per step of nsteps steps (events on the batch stream)."""
+        self._refresh()
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         tv = self._tvals(nsteps)

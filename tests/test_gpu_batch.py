@@ -96,6 +96,41 @@ def test_batch_ce_tables_match_single_runs(compressed):
     bt.close()
 
 
+def test_batch_follows_buffers_moved_by_another_engine():
+    """An instance of the batch run alone on the fused engine between batched runs (which
+    carves its field sets anew and swaps its psi buffers): the batch builds its device
+    descriptors again and every instance stays byte-equal to the same simulation run
+    alone in place; a Drude box added to an instance is refused at the next run."""
+    from fdtd_accelerate_amd import Drude
+    from fdtd_accelerate_amd.batch import BatchedYee
+    a = _mk(SHAPES[:3], 4, 3)
+    b = _mk(SHAPES[:3], 4, 3)
+    bt = BatchedYee(b)
+    bt.run(3)
+    for s in a:
+        s.run(3)
+    b[1].engine = "fused"
+    b[1].run(3)  # odd: the state ends in the fused engine's other field set
+    a[1].run(3)
+    b[1].engine = "kemp"
+    bt.run(4)
+    for s in a:
+        s.run(4)
+    torch.cuda.synchronize()
+    for n, (sa, sb) in enumerate(zip(a, b)):
+        for name in ("ex", "ey", "ez", "hx", "hy", "hz"):
+            assert sa.field(name).cpu().numpy().tobytes() == sb.field(name).cpu().numpy().tobytes(), (n, name)
+        for key in sa.cpml.psiE:
+            for c in range(2):
+                assert sa.cpml.psiE[key][c].cpu().numpy().tobytes() == sb.cpml.psiE[key][c].cpu().numpy().tobytes()
+                assert sa.cpml.psiH[key][c].cpu().numpy().tobytes() == sb.cpml.psiH[key][c].cpu().numpy().tobytes()
+    nx, ny, nz = SHAPES[0]
+    Drude(b[0].fields, (6, 6, 6), (nx - 7, ny - 7, nz - 7), 0.5, 0.05)
+    with pytest.raises(ValueError, match="Drude"):
+        bt.run(1)
+    bt.close()
+
+
 def test_batch_continues_and_no_cpml():
     """Two run() calls (source time continues) on a CPML-free, CE-free batch."""
     from fdtd_accelerate_amd.batch import BatchedYee
