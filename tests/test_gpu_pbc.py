@@ -95,6 +95,45 @@ def test_pbc_matches_oracle(ns, axes, npml, seg, steps, engine, fp):
     assert np.array_equal(np.take(g, n - 1, axis=a), np.take(g, 0, axis=a))
 
 
+def test_pbc_fused_line_phase_and_ce_triples():
+    """Periodic x and y with CPML on z on the fused engine, as the 1024^3 periodic
+    workload runs it: field arrays at a nonzero line phase (nz_pitch 128) and CE as
+    material indices read by the sweep as triple indices; equal to the oracle."""
+    from fdtd_accelerate_amd import Pbc
+    from fdtd_accelerate_amd.sim import YeeSim
+    from fdtd_accelerate_amd.sources import gaussian_pulse
+    ns, axes, npml, steps = (20, 22, 128), "xy", 4, 9
+    nx, ny, nz = ns
+    sim = YeeSim(nx, ny, nz, npml=npml, coeff_use="e", cpml_faces=("", "", "fb"), engine="fused",
+                 fused_params={"rows_per_thread": 3, "planes_per_chunk": 6})
+    f = sim.fields
+    rng = np.random.default_rng(31)
+    init = [rng.standard_normal(ns).astype(np.float32) for _ in range(6)]
+    vals = (0.5 / np.array([1.0, 2.5, 6.0, 11.0])).astype(np.float32)
+    ces = [vals[rng.integers(0, len(vals), ns)] for _ in range(3)]
+    f.set_ehs(*init)
+    f.set_ces(*ces)
+    assert f.compress_ce()
+    Pbc(f, axes)
+    tf = gaussian_pulse(8.0, 3.0, 10.0)
+    src = (nx // 2, ny // 2, nz // 2)
+    sim.add_source("ez", src, src, tf)
+    sim.add_probe("ey", (0, ny // 2, 20))
+    o = O.OracleSim(nx, ny, nz, f.pad, npml, np.float32, ces=ces, cpml_faces=((False, False), (False, False), (True, True)),
+                    pbc=axes)
+    for k in range(6):
+        o.ehs[k][...] = init[k]
+    o.add_source("ez", src, src, tf)
+    o.add_probe("ey", (0, ny // 2, 20))
+    series = np.concatenate([sim.run(4), sim.run(steps - 4)])
+    st = sim._fused
+    assert st.z_phase != 0 and st.ce_tri is not None and st.sweep_box()[4] < 32
+    o.run(steps)
+    assert _same(series, np.array(o.series, np.float32))
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+
+
 def test_whole_step_engines_reject_periodic_grids():
     from fdtd_accelerate_amd import Pbc
     from fdtd_accelerate_amd.batch import BatchedYee
