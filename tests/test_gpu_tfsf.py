@@ -100,6 +100,27 @@ def test_tfsf_fused_matches_oracle(axis, angle, apply, npml, fp, table):
     assert np.abs(o.ehs[2]).max() > 0.1 or np.abs(o.ehs[1]).max() > 0.1
 
 
+def test_tfsf_fused_line_phase_and_ce_triples():
+    """The plane wave on the fused engine as the 1024^3 plane-wave workload runs it: field
+    arrays at a nonzero line phase (nz_pitch 128) and the sweep reading triple CE
+    indices; byte-equal to the oracle over 40 steps."""
+    ns = (30, 28, 128)
+    rng = np.random.default_rng(17)
+    vals = (0.5 / np.array([1.0, 2.0, 3.5, 8.0])).astype(np.float32)
+    ce = [vals[rng.integers(0, len(vals), ns)] for _ in range(3)]
+    sim, o = _pair(ns, 5, (8, 7, 20), (21, 20, 100), ("fb", "fb", "fb"), "x", 0.3, 10.0, ce, 64, "fused",
+                   {"rows_per_thread": 3, "planes_per_chunk": 8})
+    assert sim.fields.compress_ce()
+    sim.run(17)
+    sim.run(23)
+    st = sim._fused
+    assert st.tfsf is not None and st.z_phase != 0 and st.ce_tri is not None
+    o.run(40)
+    for k, name in enumerate(NAMES):
+        assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
+    assert np.abs(o.ehs[2]).max() > 0.1 or np.abs(o.ehs[1]).max() > 0.1
+
+
 def test_tfsf_fused_shell_only_matches_oracle():
     """A grid with no sweep box (nz = 33 with CPML 5): every patch is applied by the shell
     passes."""
