@@ -1288,8 +1288,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (reinterpret_cast<uintptr_t>(d->ce_tri) & 7u) return fail(MIFDTD_EALIGN, "CE triple indices not 8-byte aligned");
         if (cem != kCeTable) return fail(MIFDTD_EINVAL, "CE triple indices need the material indices (ce_mat) too");
     }
-    const bool tri_sweep = d->ce_tri && d->drude_j[0] == nullptr;  // the Drude sweep keeps ce_mat
-    const int sweep_cem = tri_sweep ? (int)kCeTriple : cem;
+    const int sweep_cem = d->ce_tri ? (int)kCeTriple : cem;
     g.ntri = sweep_cem == kCeTriple ? d->ce_tri_n : 0;
     const size_t lds_bytes = sizeof(float) * ((size_t)ce_tab_floats(sweep_cem, g.ntri) + (size_t)2 * 2 * T * 2 * 256 +
                                               (size_t)(CX + 1) * (T + 1) * 2 +
@@ -1601,8 +1600,7 @@ int mifdtd_fused_run_timed(const mifdtd_fused_desc *d, int nsteps, const float *
         if (((FV) & kFeatDrude) || !wide) MIFDTD_FUSED_LAUNCH_W(CEV, FV, false);                                  \
         else MIFDTD_FUSED_LAUNCH_W(CEV, FV, !((FV) & kFeatDrude));                                                \
     } while (0)
-            if (sweep_cem == kCeTriple && (feat & kFeatTfsf)) MIFDTD_FUSED_LAUNCH(kCeTriple, kFeatTfsf);
-            else if (sweep_cem == kCeTriple) MIFDTD_FUSED_LAUNCH(kCeTriple, 0);
+            if (sweep_cem == kCeTriple) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTriple, feat);
             else if (cem == kCeTable) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeTable, feat);
             else if (cem == kCeArrays) MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeArrays, feat);
             else MIFDTD_BY_FEAT(MIFDTD_FUSED_LAUNCH, kCeConst, feat);

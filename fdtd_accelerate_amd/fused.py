@@ -233,13 +233,12 @@ class FusedStepper:
         """One uint16 index per cell into the distinct (cex, cey, cez) triples of the
         material indices (``Fields.compress_ce``), read by the sweep instead of the
         three uint8 indices (2 B per cell instead of 3; the same float values).  Skipped
-        with a Drude box, more than MAX_CE_TRI triples, or a table that does not fit the
-        sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
+        with more than MAX_CE_TRI triples or a table that does not fit the sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
         the sweep's segments start on lines."""
         f = self.sim.fields
         self.ce_tri = self.ce_tri_lut = None
         self._tri_src = f.ce_mat
-        if f.ce_mat is None or self.drude is not None:
+        if f.ce_mat is None:
             return
         flat = [x.view(-1) for x in f.ce_mat]
         total = flat[0].numel()

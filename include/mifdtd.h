@@ -337,8 +337,8 @@ typedef struct {
     /* fused only, optional with ce_mat: CE as one uint16 index per cell (an array like a
      * field, 8-byte aligned) into a DEVICE table of ce_tri_n <= MIFDTD_FUSED_MAX_CE_TRI
      * (cex, cey, cez) float triples -- the same values as ce_mat/ce_lut, 2 B per cell
-     * instead of 3.  Read by the interior sweep of single steps without a Drude box; the
-     * shell passes keep ce_mat. */
+     * instead of 3.  Read by the interior sweep of single steps; the shell passes keep
+     * ce_mat. */
     const unsigned short *ce_tri;
     const float *ce_tri_lut;
     int ce_tri_n;
