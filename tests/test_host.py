@@ -309,6 +309,22 @@ def test_z_line_phase_choice():
         assert ph in waste and waste[ph] == min(waste.values()), (nz, p, ph, waste)
 
 
+def test_fused_shell_boxes_cover_the_grid(tmp_path):
+    """The fused step's CPML shell decomposition (csrc/shell_boxes.hpp, as the kernels use
+    it, built here for the host): for 10 grids, CPML widths 0-12, every line phase and
+    with or without the step-pair fringes, the shell boxes decoded thread by thread cover
+    each quad outside the interior box exactly once, none inside it, and never leave the
+    grid (tests/shell_cover.cpp)."""
+    import subprocess
+    src = os.path.join(os.path.dirname(__file__), "shell_cover.cpp")
+    exe = str(tmp_path / "shell_cover")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    src, "-o", exe], check=True, capture_output=True, timeout=600)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and "shell cover ok" in out.stdout, out.stdout + out.stderr
+
+
 def test_ctypes_mirrors_match_header(tmp_path):
     """Every ctypes mirror in _lib has the size and field offsets of its C struct in
     include/mifdtd.h (a C probe compiled with gcc against the header prints them)."""
