@@ -355,3 +355,15 @@ def test_ctypes_mirrors_match_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+@pytest.mark.parametrize("argv, env", [(["--gpus", "2"], {}), ([], {"WORLD_SIZE": "8"})])
+def test_bench_refuses_more_than_one_gpu(argv, env):
+    """bench.py measures one GPU: asked for N > 1 (by flag or by a launcher's WORLD_SIZE)
+    it exits with status 2 and a message, before touching torch or a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *argv], capture_output=True, text=True,
+                       env={**os.environ, **env}, timeout=120)
+    assert r.returncode == 2 and r.stdout == ""
+    assert "one GPU only" in r.stderr
