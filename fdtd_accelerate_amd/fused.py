@@ -217,6 +217,15 @@ class FusedStepper:
                 self.T = min(self.T, 7)
             if self.T > 7:
                 raise ValueError(f"rows_per_thread {self.T}: at most 7 with a Drude box")
+        # a TFSF sweep takes 13 rows per block where the grid is tall enough: its z-face
+        # patches meet every row of two 256-cell segments, and at 1024^3 (CX = 64) the
+        # sweep ran 11.13-11.21 ms against 11.82-11.88 ms for 9 rows (alternating in one
+        # process)
+        self._tfsf_rows = bool(self.tfsf is not None and rows_per_thread is None and self.drude is None
+                               and self.T == 9 and f.ny >= 8 * 13
+                               and lds_fits(13, self.CX, table, extra=2 * self.tfsf.n_line + 1024))
+        if self._tfsf_rows:
+            self.T = 13
         self.mid = (_fields.staggered_arrays(6, tuple(f.ehs[0].shape), f.ehs[0].dtype, f.device, offset=offset)
                     if self.pairs else None)
         # the Drude current of step n+1 (the two sets swap with the field sets)
@@ -233,8 +242,9 @@ class FusedStepper:
         """One uint16 index per cell into the distinct (cex, cey, cez) triples of the
         material indices (``Fields.compress_ce``), read by the sweep instead of the
         three uint8 indices (2 B per cell instead of 3; the same float values).  Skipped
-        with more than MAX_CE_TRI triples or a table that does not fit the sweep's LDS.  Placed (-2 kb) mod 128 bytes past a line so that
-        the sweep's segments start on lines."""
+        with more than MAX_CE_TRI triples or a table that does not fit the sweep's LDS (a
+        TFSF sweep raised to 13 rows goes back to 9 where that lets the table fit).  Placed
+        (-2 kb) mod 128 bytes past a line so that the sweep's segments start on lines."""
         f = self.sim.fields
         self.ce_tri = self.ce_tri_lut = None
         self._tri_src = f.ce_mat
@@ -257,8 +267,12 @@ class FusedStepper:
             n = vals.numel()
             # a TFSF source stages its line (2 floats per point) and two patch tables in LDS
             extra = 2 * self.tfsf.n_line + 1024 if self.tfsf is not None else 0
-            if n > _lib.MAX_CE_TRI or not lds_fits(self.T, self.CX, True, n, extra):
+            if n > _lib.MAX_CE_TRI:
                 return
+            if not lds_fits(self.T, self.CX, True, n, extra):
+                if not (self._tfsf_rows and lds_fits(9, self.CX, True, n, extra)):
+                    return
+                self.T = 9  # the triples fit beside the plane wave's line with the smaller tile
             lut = torch.stack([f.ce_lut[c][(vals >> (8 * c)) & 255] for c in range(3)], dim=1).contiguous()
             code = torch.zeros(1 << 24, dtype=torch.int16, device=f.device)
             code[vals] = torch.arange(n, dtype=torch.int16, device=f.device)

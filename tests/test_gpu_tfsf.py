@@ -100,21 +100,26 @@ def test_tfsf_fused_matches_oracle(axis, angle, apply, npml, fp, table):
     assert np.abs(o.ehs[2]).max() > 0.1 or np.abs(o.ehs[1]).max() > 0.1
 
 
-def test_tfsf_fused_line_phase_and_ce_triples():
+@pytest.mark.parametrize("ns,fp,rows", [
+    ((30, 28, 128), {"rows_per_thread": 3, "planes_per_chunk": 8}, 3),
+    ((30, 28, 128), {"rows_per_thread": 13, "planes_per_chunk": 16}, 13),
+    ((24, 110, 128), None, 13),
+], ids=["T3", "T13", "auto_T13"])
+def test_tfsf_fused_line_phase_and_ce_triples(ns, fp, rows):
     """The plane wave on the fused engine as the 1024^3 plane-wave workload runs it: field
-    arrays at a nonzero line phase (nz_pitch 128) and the sweep reading triple CE
-    indices; byte-equal to the oracle over 40 steps."""
-    ns = (30, 28, 128)
+    arrays at a nonzero line phase (nz_pitch 128), the sweep reading triple CE indices,
+    13 rows per block (the 1024-thread build; the automatic choice on a grid at least 104
+    rows tall); byte-equal to the oracle over 40 steps."""
     rng = np.random.default_rng(17)
     vals = (0.5 / np.array([1.0, 2.0, 3.5, 8.0])).astype(np.float32)
     ce = [vals[rng.integers(0, len(vals), ns)] for _ in range(3)]
-    sim, o = _pair(ns, 5, (8, 7, 20), (21, 20, 100), ("fb", "fb", "fb"), "x", 0.3, 10.0, ce, 64, "fused",
-                   {"rows_per_thread": 3, "planes_per_chunk": 8})
+    sim, o = _pair(ns, 5, (8, 7, 20), (ns[0] - 9, ns[1] - 8, 100), ("fb", "fb", "fb"), "x", 0.3, 10.0, ce, 64,
+                   "fused", fp)
     assert sim.fields.compress_ce()
     sim.run(17)
     sim.run(23)
     st = sim._fused
-    assert st.tfsf is not None and st.z_phase != 0 and st.ce_tri is not None
+    assert st.tfsf is not None and st.z_phase != 0 and st.ce_tri is not None and st.T == rows
     o.run(40)
     for k, name in enumerate(NAMES):
         assert _same(sim.field(name).cpu().numpy(), o.ehs[k]), name
